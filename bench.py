@@ -1,0 +1,241 @@
+"""Benchmark: GDOF/s per RHS evaluation of the DG advection operator on one GPU.
+
+Workload: 3D linear advection u_t + a.grad u = 0, a = (1, 0.7, 0.3), on the
+structured periodic 32^3 hexahedral mesh of [0,1]^3 (32768 elements), uniform
+p = 3 (64 basis functions, 2,097,152 DOF), upwind flux, RK4; initial data:
+8 seeded sine modes with k in [1,6]^3 and amplitudes U(-1,1), L2-projected.
+
+One timed step = one RHS evaluation as an RK stage performs it: the stage
+input u + (dt/2) k, the volume and face integrals and the inverse-mass apply
+(hpGEM's computeTimeDerivative), one fused kernel launch on state resident in
+HBM. Both arithmetic paths are timed: FP64 and mixed (FP32 contractions and
+stage derivatives, FP64 state, geometry and RK update); the mixed path's L2
+error against the FP64 path after the full 50-step RK4 run is measured in the
+same run. The CPU baseline is the FP64 oracle (a plain restatement of hpGEM's
+HpgemAPILinear algorithm) timed on a bounded sample on the host.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GDOF/s per RHS eval, FP64 vs mixed-MFMA at L2err≤1e-6; p50/p99 solve latency"
+HBM_PEAK_GBS = 8000.0
+N_CELLS = 32
+P = 3
+VELOCITY = [1.0, 0.7, 0.3]
+SEED = 1
+
+
+class HipEvents:
+    """hipEvent timing on a given stream, through the HIP runtime already
+    loaded by libhgdg.so."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so.7", mode=ctypes.RTLD_GLOBAL)
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def event(self):
+        e = ctypes.c_void_p()
+        if self.hip.hipEventCreate(ctypes.byref(e)) != 0:
+            raise RuntimeError("hipEventCreate failed")
+        return e
+
+    def record(self, e, stream):
+        if self.hip.hipEventRecord(e, ctypes.c_void_p(stream)) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
+    def elapsed_ms(self, a, b):
+        self.hip.hipEventSynchronize(b)
+        ms = ctypes.c_float()
+        if self.hip.hipEventElapsedTime(ctypes.byref(ms), a, b) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return float(ms.value)
+
+
+def workload_modes(dim=3, n_modes=8, kmax=6, seed=SEED):
+    rng = np.random.default_rng(seed)
+    amp = rng.uniform(-1.0, 1.0, n_modes)
+    k = rng.integers(1, kmax + 1, size=(n_modes, dim)).astype(np.float64)
+    return amp, k
+
+
+def algorithmic_bytes(n_dof, n_elem, n_faces, in_bytes, out_bytes):
+    """Minimal HBM bytes of one RHS evaluation: each input vector read once
+    and the output written once, plus per-element geometry (det J, J^{-1}:
+    80 B) and per-face connectivity/normal data (40 B)."""
+    return n_dof * (sum(in_bytes) + out_bytes) + 80 * n_elem + 40 * n_faces
+
+
+def time_rhs(api, events, steps, warmup, in_ids, coefs, out_id):
+    stream = api.stream()
+    for _ in range(warmup):
+        api.computeTimeDerivative(in_ids, coefs, out_id)
+    api.synchronize()
+    e0, e1 = events.event(), events.event()
+    t0 = time.perf_counter()
+    events.record(e0, stream)
+    for _ in range(steps):
+        api.computeTimeDerivative(in_ids, coefs, out_id)
+    events.record(e1, stream)
+    ms = events.elapsed_ms(e0, e1)
+    api.synchronize()
+    wall = time.perf_counter() - t0
+    return ms / steps, wall
+
+
+def run_gpu(steps, warmup):
+    import hpgem_amd
+    from hpgem_amd import HpgemAPIDevice, Mesh, RK4
+
+    events = HipEvents()
+    mesh = Mesh.structured([N_CELLS] * 3)
+    amp, k = workload_modes()
+    h = 1.0 / N_CELLS
+    dt = 0.1 * h / float(np.linalg.norm(VELOCITY))
+    results = {}
+    finals = {}
+    for prec in ("fp64", "mixed"):
+        api = HpgemAPIDevice(mesh, P, velocity=VELOCITY, precision=prec, tableau=RK4, n_extra_vectors=2)
+        api.setInitialSolutionModes(amp, k)
+        if prec == "mixed":
+            for v in (1, 2):
+                api.setStorage(v, "mixed")
+        api.computeTimeDerivative([0], [1.0], 1)  # k1 = f(u)
+        ms, wall = time_rhs(api, events, steps, warmup, [0, 1], [1.0, 0.5 * dt], 2)
+        in_b = [8, 8 if prec == "fp64" else 4]
+        out_b = 8 if prec == "fp64" else 4
+        bytes_alg = algorithmic_bytes(api.n_dof, api.n_elements, api.n_faces, in_b, out_b)
+        # whole RK4 run: 50 steps, hipGraph-replayed
+        api.setInitialSolutionModes(amp, k)
+        stream = api.stream()
+        e0, e1 = events.event(), events.event()
+        api.synchronize()
+        events.record(e0, stream)
+        api.computeOneTimeStep(0.0, dt, 50)
+        events.record(e1, stream)
+        run_ms = events.elapsed_ms(e0, e1)
+        finals[prec] = api.getTimeIntegrationVector(0)
+        results[prec] = {
+            "gdofs": api.n_dof / (ms * 1e-3) / 1e9,
+            "ms_per_rhs": ms,
+            "bytes_alg_per_rhs": bytes_alg,
+            "achieved_gbs": bytes_alg / (ms * 1e-3) / 1e9,
+            "rk4_50_steps_ms": run_ms,
+            "rk4_step_ms": run_ms / 50,
+            "host_wall_s": wall,
+        }
+        n_dof, n_elem, n_faces = api.n_dof, api.n_elements, api.n_faces
+        if prec == "fp64":
+            ref_api = api
+        else:
+            api.close()
+    # mixed vs FP64 after the 50-step run (mass-weighted L2 on the device)
+    ref_api.setTimeIntegrationVector(5, finals["mixed"])
+    l2_diff, max_diff = ref_api.difference(0, 5)
+    ref_api.setTimeIntegrationVector(6, np.zeros(n_dof))
+    l2_norm, _ = ref_api.difference(0, 6)
+    ref_api.close()
+    results["mixed"]["l2_err_vs_fp64"] = l2_diff
+    results["mixed"]["rel_l2_err_vs_fp64"] = l2_diff / l2_norm
+    results["mixed"]["max_coef_err_vs_fp64"] = max_diff
+    return results, dict(n_dof=n_dof, n_elem=n_elem, n_faces=n_faces, dt=dt)
+
+
+def run_cpu_baseline(target_seconds=15.0):
+    """FP64 oracle (HpgemAPILinear restatement: dense element/face mat-vecs and
+    a per-element LU mass solve per call) on a bounded sample: the same
+    element type and operator on a 12^3 periodic hex mesh, RHS evaluations
+    repeated for ~target_seconds after one untimed assembly."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as orc
+
+    n = 12
+    shape, X, nodes = orc.structured_mesh([n, n, n])
+    prob = orc.Problem(shape, X, nodes, P)
+    prob.set_advection(VELOCITY)
+    rng = np.random.default_rng(SEED)
+    u = rng.standard_normal(prob.n_dof)
+    kk = rng.standard_normal(prob.n_dof)
+    prob.time_derivative([u, kk], [1.0, 1e-3])  # assembles the matrices
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        prob.time_derivative([u, kk], [1.0, 1e-3])
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > target_seconds or reps >= 200:
+            break
+    per = el / reps
+    return {"value": prob.n_dof / per / 1e9, "unit": "GDOF/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle (HpgemAPILinear algorithm), 12^3 periodic hex p=3 "
+                      f"({prob.n_dof} DOF), {reps} RHS evals in {el:.1f} s, 1 thread",
+            "ms_per_rhs": per * 1e3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the operator shards by element partition with a face-trace halo; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    gpu, info = run_gpu(args.steps, args.warmup)
+    cpu = None if args.no_cpu else run_cpu_baseline(args.cpu_seconds)
+
+    mixed_ok = gpu["mixed"]["rel_l2_err_vs_fp64"] <= 1e-6
+    head = gpu["mixed"] if mixed_ok else gpu["fp64"]
+    roof = {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": head["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "tensor_advection_rhs<3,4," + ("float" if mixed_ok else "double") + ">",
+            "bytes_alg_per_launch": head["bytes_alg_per_rhs"]}
+    out = {
+        "metric": METRIC,
+        "value": head["gdofs"],
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_rhs"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 contractions, f64 state/geometry/update" if mixed_ok else "f64",
+        "data": "synthetic (seeded sine modes, L2-projected)",
+        "config": {"workload": "3D linear advection, periodic 32^3 hex, p=3, upwind, RK4 stage RHS "
+                               "(stage combine + volume + faces + inverse mass)",
+                   "elements": info["n_elem"], "dof": info["n_dof"], "faces": info["n_faces"],
+                   "velocity": VELOCITY, "dt": info["dt"], "seed": SEED},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fp64": gpu["fp64"],
+        "mixed": gpu["mixed"],
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = {"fp64": gpu["fp64"]["gdofs"] / cpu["value"],
+                                 "mixed": gpu["mixed"]["gdofs"] / cpu["value"]}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

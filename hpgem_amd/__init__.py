@@ -1,0 +1,13 @@
+"""MI355X-native explicit DG operator with hpGEM's RHS/mass/RK interface.
+
+The compute path is libhgdg.so (hand-written HIP for gfx950) behind the C ABI
+in include/hgdg.h; this package is the host-side mirror of hpGEM's API.
+"""
+from ._lib import (HgdgError, OP_ACOUSTIC_CENTRAL, OP_ADVECTION_UPWIND, PREC_FP64, PREC_MIXED_FP32,
+                   SHAPE_CUBE, SHAPE_SQUARE, SHAPE_TETRAHEDRON, load)
+from .api import HpgemAPIDevice, Mesh, basis_eval, basis_size
+from .tableaux import DORMAND_PRINCE, FORWARD_EULER, RK4, ButcherTableau
+
+__all__ = ["HgdgError", "HpgemAPIDevice", "Mesh", "basis_eval", "basis_size", "load", "RK4", "DORMAND_PRINCE",
+           "FORWARD_EULER", "ButcherTableau", "SHAPE_SQUARE", "SHAPE_CUBE", "SHAPE_TETRAHEDRON",
+           "OP_ADVECTION_UPWIND", "OP_ACOUSTIC_CENTRAL", "PREC_FP64", "PREC_MIXED_FP32"]

@@ -1,0 +1,92 @@
+"""ctypes binding to libhgdg.so, the HIP library behind include/hgdg.h.
+
+There is no CPU fallback: if the library is missing, loading fails loudly.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhgdg.so")
+
+SHAPE_SQUARE, SHAPE_CUBE, SHAPE_TETRAHEDRON = 2, 4, 5
+OP_ADVECTION_UPWIND, OP_ACOUSTIC_CENTRAL = 1, 2
+PREC_FP64, PREC_MIXED_FP32 = 0, 1
+
+ERRORS = {1: "invalid argument", 2: "HIP error", 3: "unsupported", 4: "call out of order", 5: "memory"}
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+_INT = ctypes.POINTER(ctypes.c_int)
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_int32), ("dim", ctypes.c_int32), ("n_elements", ctypes.c_int64),
+                ("coords", _D), ("node_ids", _I64), ("order", _I32), ("uniform_order", ctypes.c_int32)]
+
+
+class OpDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("precision", ctypes.c_int32), ("velocity", ctypes.c_double * 3),
+                ("c_inv", _D)]
+
+
+# (name, restype, argtypes) of every entry point declared in include/hgdg.h
+SIGNATURES = [
+    ("hgdg_last_error", ctypes.c_char_p, []),
+    ("hgdg_ctx_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
+    ("hgdg_ctx_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("hgdg_mesh_upload", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MeshDesc)]),
+    ("hgdg_mesh_info", ctypes.c_int, [ctypes.c_void_p, _I64, _I64, _I64, _I64]),
+    ("hgdg_operator_set", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OpDesc)]),
+    ("hgdg_state_alloc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hgdg_state_set", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
+    ("hgdg_state_get", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
+    ("hgdg_state_set_storage", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("hgdg_time_derivative", ctypes.c_int, [ctypes.c_void_p, _INT, _D, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_double]),
+    ("hgdg_scale_and_add", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
+    ("hgdg_rk_steps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _INT, ctypes.c_int, _D, _D,
+                                     ctypes.c_double, ctypes.c_int]),
+    ("hgdg_project_modes", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, _D]),
+    ("hgdg_state_diff", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, _D]),
+    ("hgdg_sync", ctypes.c_int, [ctypes.c_void_p]),
+    ("hgdg_get_stream", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    ("hgdg_basis_size", ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    ("hgdg_basis_eval", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _D]),
+    ("hgdg_structured_mesh", ctypes.c_int, [ctypes.c_int, _I32, _D, _D, _I32, ctypes.c_int, _I64, _I32, _I32,
+                                            _D, _I64]),
+]
+
+_lib = None
+
+
+class HgdgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hgdg error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+def load():
+    """Load libhgdg.so (built by __graft_entry__.build()); raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build the HIP extension first "
+                          "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        raise HgdgError(rc, load().hgdg_last_error().decode())
+
+
+def dptr(a):
+    return a.ctypes.data_as(_D)

@@ -1,0 +1,169 @@
+"""hpGEM-shaped host API over the C ABI (include/hgdg.h).
+
+Method names and argument meaning follow hpGEM's explicit solver API
+(kernel/Base/HpgemAPISimplified.h:190-442, HpgemAPILinear.h:168-301):
+time-integration vectors are addressed by id, computeTimeDerivative takes
+input ids and coefficients, computeOneTimeStep advances (time, dt). The
+integrands are chosen declaratively (operator descriptor), since a C++/Python
+virtual integrand cannot run inside a GPU kernel. All arithmetic runs in the
+HIP library; this module only marshals arguments.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr
+from .tableaux import RK4, ButcherTableau
+
+SHAPES = {"square": _lib.SHAPE_SQUARE, "cube": _lib.SHAPE_CUBE, "tetrahedron": _lib.SHAPE_TETRAHEDRON}
+PRECISIONS = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED_FP32, "mixed_fp32": _lib.PREC_MIXED_FP32}
+
+
+class Mesh:
+    """Element vertex coordinates + topological node ids (a read .hpgem mesh)."""
+
+    def __init__(self, shape, coords, node_ids):
+        self.shape = int(shape)
+        self.coords = np.ascontiguousarray(coords, dtype=np.float64)
+        self.node_ids = np.ascontiguousarray(node_ids, dtype=np.int64)
+        self.n_elements, self.n_vertices, self.dim = self.coords.shape
+
+    @classmethod
+    def structured(cls, n, lower=None, upper=None, periodic=True, simplices=False):
+        """hpGEM Preprocessor structured mesh (applications/Preprocessor/lib/hpgem.cpp:100-235)."""
+        lib = _lib.load()
+        dim = len(n)
+        nn = np.ascontiguousarray(n, dtype=np.int32)
+        lo = np.zeros(dim) if lower is None else np.ascontiguousarray(lower, dtype=np.float64)
+        hi = np.ones(dim) if upper is None else np.ascontiguousarray(upper, dtype=np.float64)
+        per = np.ascontiguousarray(np.broadcast_to(np.asarray(periodic, dtype=np.int32), (dim,)))
+        ne, nv, shape = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        args = [dim, nn.ctypes.data_as(_lib._I32), dptr(lo), dptr(hi), per.ctypes.data_as(_lib._I32),
+                int(bool(simplices))]
+        check(lib.hgdg_structured_mesh(*args, ctypes.byref(ne), ctypes.byref(nv), ctypes.byref(shape), None, None))
+        coords = np.zeros((ne.value, nv.value, dim))
+        nodes = np.zeros((ne.value, nv.value), dtype=np.int64)
+        check(lib.hgdg_structured_mesh(*args, ctypes.byref(ne), ctypes.byref(nv), ctypes.byref(shape),
+                                       dptr(coords), nodes.ctypes.data_as(_lib._I64)))
+        return cls(shape.value, coords, nodes)
+
+
+def basis_size(shape, p):
+    return _lib.load().hgdg_basis_size(int(shape), int(p))
+
+
+def basis_eval(shape, p, point):
+    lib = _lib.load()
+    n = basis_size(shape, p)
+    out = np.zeros(n)
+    x = np.ascontiguousarray(point, dtype=np.float64)
+    check(lib.hgdg_basis_eval(int(shape), int(p), dptr(x), dptr(out)))
+    return out
+
+
+class HpgemAPIDevice:
+    """Explicit DG solver whose RHS, mass solve and RK loop run on one GPU.
+
+    Mirrors HpgemAPISimplified/HpgemAPILinear: vector 0 is the solution
+    (solutionVectorId_), vectors 1..stages are the stage derivatives
+    (auxiliaryVectorIds_), extra vectors follow.
+    """
+
+    def __init__(self, mesh, polynomial_order, velocity=None, precision="fp64", tableau=RK4,
+                 n_extra_vectors=0, device=0, kind=_lib.OP_ADVECTION_UPWIND, c_inv=None):
+        self.lib = _lib.load()
+        self.mesh = mesh
+        self.p = polynomial_order
+        self.tableau = tableau
+        self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        h = ctypes.c_void_p()
+        check(self.lib.hgdg_ctx_create(ctypes.byref(h), device))
+        self.h = h
+        self._order = np.ascontiguousarray(np.broadcast_to(np.asarray(polynomial_order, dtype=np.int32),
+                                                           (mesh.n_elements,)))
+        desc = _lib.MeshDesc(mesh.shape, mesh.dim, mesh.n_elements, dptr(mesh.coords),
+                             mesh.node_ids.ctypes.data_as(_lib._I64), self._order.ctypes.data_as(_lib._I32),
+                             int(self._order[0]))
+        check(self.lib.hgdg_mesh_upload(self.h, ctypes.byref(desc)))
+        ne, nf, ni, nd = (ctypes.c_int64() for _ in range(4))
+        check(self.lib.hgdg_mesh_info(self.h, ctypes.byref(ne), ctypes.byref(nf), ctypes.byref(ni), ctypes.byref(nd)))
+        self.n_elements, self.n_faces, self.n_internal_faces, self.n_dof = ne.value, nf.value, ni.value, nd.value
+        vel = (ctypes.c_double * 3)(*(list(velocity or []) + [0.0, 0.0, 0.0])[:3])
+        self._cinv = None if c_inv is None else np.ascontiguousarray(c_inv, dtype=np.float64)
+        op = _lib.OpDesc(kind, self.precision, vel, dptr(self._cinv) if self._cinv is not None else None)
+        check(self.lib.hgdg_operator_set(self.h, ctypes.byref(op)))
+        self.solutionVectorId_ = 0
+        self.auxiliaryVectorIds_ = list(range(1, tableau.stages + 1))
+        self.n_vectors = 1 + tableau.stages + n_extra_vectors
+        check(self.lib.hgdg_state_alloc(self.h, self.n_vectors))
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(self.lib.hgdg_ctx_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- time-integration vectors (reference layout) ----------------------
+    def setTimeIntegrationVector(self, vec, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.size != self.n_dof:
+            raise ValueError(f"expected {self.n_dof} values, got {v.size}")
+        check(self.lib.hgdg_state_set(self.h, vec, dptr(v)))
+
+    def getTimeIntegrationVector(self, vec):
+        out = np.zeros(self.n_dof)
+        check(self.lib.hgdg_state_get(self.h, vec, dptr(out)))
+        return out
+
+    def setStorage(self, vec, precision):
+        check(self.lib.hgdg_state_set_storage(self.h, vec, PRECISIONS[precision]))
+
+    def setInitialSolutionModes(self, amp, k, vec=None):
+        """L2 projection of sum_m amp_m sin(2 pi k_m . x) (setInitialSolution)."""
+        amp = np.ascontiguousarray(amp, dtype=np.float64)
+        kk = np.zeros((len(amp), 3))
+        kk[:, : np.asarray(k).shape[1]] = k
+        check(self.lib.hgdg_project_modes(self.h, self.solutionVectorId_ if vec is None else vec, len(amp),
+                                          dptr(amp), dptr(kk)))
+
+    # ---- the hot path ------------------------------------------------------
+    def computeTimeDerivative(self, inputVectorIds, coefficientsInputVectors, resultVectorId, time=0.0):
+        ids = (ctypes.c_int * len(inputVectorIds))(*inputVectorIds)
+        co = np.ascontiguousarray(coefficientsInputVectors, dtype=np.float64)
+        check(self.lib.hgdg_time_derivative(self.h, ids, dptr(co), len(inputVectorIds), resultVectorId, time))
+
+    def scaleAndAddVector(self, vectorToChangeId, vectorToAddId, scale):
+        check(self.lib.hgdg_scale_and_add(self.h, vectorToChangeId, vectorToAddId, scale))
+
+    def computeOneTimeStep(self, time, dt, n_steps=1):
+        """Advance n_steps of the tableau; returns the new time."""
+        t = self.tableau
+        aux = (ctypes.c_int * t.stages)(*self.auxiliaryVectorIds_)
+        a = np.ascontiguousarray(t.a_matrix(), dtype=np.float64)
+        b = np.ascontiguousarray(t.b, dtype=np.float64)
+        check(self.lib.hgdg_rk_steps(self.h, self.solutionVectorId_, aux, t.stages, dptr(a), dptr(b), dt, n_steps))
+        return time + n_steps * dt
+
+    def solve(self, start_time, end_time, dt):
+        n = int(round((end_time - start_time) / dt))
+        return self.computeOneTimeStep(start_time, dt, n)
+
+    def synchronize(self):
+        check(self.lib.hgdg_sync(self.h))
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        check(self.lib.hgdg_get_stream(self.h, ctypes.byref(s)))
+        return s.value
+
+    def difference(self, x, y):
+        """(mass-weighted L2 norm, max abs coefficient) of vector x - vector y."""
+        l2, mx = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.hgdg_state_diff(self.h, x, y, ctypes.byref(l2), ctypes.byref(mx)))
+        return l2.value, mx.value

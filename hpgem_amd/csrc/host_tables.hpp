@@ -1,0 +1,58 @@
+// Host-side tables of the library: Gauss-Legendre rules, the 1D hierarchical
+// functions behind hpGEM's square/cube DG bases, the map between hpGEM's basis
+// order and the library's tensor order, the 1D reference operators the tensor
+// kernels apply, and the structured mesh generator.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace hgdg {
+
+enum Shape { SHAPE_SQUARE = 2, SHAPE_CUBE = 4, SHAPE_TETRAHEDRON = 5 };
+
+int shape_dim(int shape);
+int shape_vertices(int shape);
+
+// n-point Gauss-Legendre rule on [-1, 1] (nodes ascending).
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w);
+
+// 1D functions psi_0 = (1-t)/2, psi_1 = (1+t)/2 and the bubbles
+// psi_{k+2} = (1-t^2)/4 * L_k(t), where L_k is hpGEM's Lobatto kernel
+// (kernel/FE/helperFunctions.cpp:49-108). The bubbles are the normalised
+// integrated Legendre polynomials, evaluated here as
+// (P_{k+2} - P_k) / sqrt(2 (2k + 3)) with derivative sqrt((2k+3)/2) P_{k+1}.
+double psi(int a, double t);
+double dpsi(int a, double t);
+
+// For the square (dim 2) and cube (dim 3) DG sets of order p: perm[i] is the
+// tensor index a + n1*b + n1^2*c (n1 = p+1) of hpGEM basis function i, whose
+// value is psi_a(x) psi_b(y) psi_c(z). The sequence follows the construction
+// loops of createDGBasisFunctionSet2DH1Square (BasisFunctions2DH1Conforming
+// Square.cpp:164-184) and createDGBasisFunctionSet3DH1Cube
+// (BasisFunctions3DH1ConformingCube.cpp:434-528); every mirroring sign in
+// those sets is +1, so the map carries no signs.
+std::vector<int> tensor_permutation(int dim, int p);
+
+// 1D reference operators for order p (n1 = p+1), row-major n1 x n1:
+//   M1(a,b)  = int psi_a psi_b,  G1(a,b) = int psi_b psi_a'  (derivative on
+//   the test index a),  D = M1^{-1} G1,  Mi = M1^{-1}.
+// Integrals use an exact Gauss rule; inverses are computed in long double.
+struct TensorOps {
+    int n1 = 0;
+    std::vector<double> M1, G1, D, Mi;
+};
+TensorOps tensor_ops(int p);
+
+// Structured mesh in hpGEM Preprocessor order
+// (applications/Preprocessor/lib/hpgem.cpp:100-235).
+struct StructuredMesh {
+    int shape = 0, dim = 0, nv = 0;
+    std::vector<double> coords;
+    std::vector<int64_t> nodes;
+};
+StructuredMesh structured_mesh(int dim, const int32_t* n, const double* lo, const double* hi,
+                               const int32_t* periodic, bool simplices);
+
+}  // namespace hgdg

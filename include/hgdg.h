@@ -1,0 +1,182 @@
+/*
+ * hgdg — C ABI of the MI355X explicit DG operator.
+ *
+ * The library replaces the right-hand-side path of hpGEM's explicit
+ * Runge-Kutta solvers: the per-element volume and per-face quadrature
+ * integrals, the block-diagonal inverse-mass apply and the RK stage
+ * combination/update, for linear upwind advection and the first-order acoustic
+ * wave system. Everything is plain C: opaque handles, host pointers and sizes,
+ * int status codes (0 = success), and a thread-local error string.
+ *
+ * Threading: one context per host thread. Calls enqueue work on the context's
+ * HIP stream; hgdg_state_get and hgdg_sync block.
+ *
+ * State layout seen by callers ("reference layout"): element-major, and inside
+ * an element field-major, iVB = iV * nB + iB, with the basis functions in
+ * hpGEM's own order (kernel/Base/ElementData.h:209-225;
+ * kernel/FE/BasisFunctions2DH1ConformingSquare.cpp:164-184,
+ * BasisFunctions3DH1ConformingCube.cpp:434-528). On the device the state is
+ * kept in the library's own layout; set/get convert.
+ */
+#ifndef HGDG_H
+#define HGDG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    HGDG_OK = 0,
+    HGDG_ERR_ARG = 1,         /* invalid argument / inconsistent sizes        */
+    HGDG_ERR_HIP = 2,         /* HIP runtime error (message has the detail)   */
+    HGDG_ERR_UNSUPPORTED = 3, /* valid input outside what the kernels handle  */
+    HGDG_ERR_STATE = 4,       /* call out of order (no mesh, no operator ...) */
+    HGDG_ERR_MEMORY = 5       /* device allocation failed / over budget       */
+};
+
+/* ---- element shapes (hpGEM reference geometries) ----------------------- */
+enum {
+    HGDG_SHAPE_SQUARE = 2,      /* kernel/Geometry/ReferenceSquare.cpp      */
+    HGDG_SHAPE_CUBE = 4,        /* kernel/Geometry/ReferenceCube.cpp        */
+    HGDG_SHAPE_TETRAHEDRON = 5  /* kernel/Geometry/ReferenceTetrahedron.cpp */
+};
+
+/* ---- operators ----------------------------------------------------------- */
+enum {
+    HGDG_OP_ADVECTION_UPWIND = 1, /* u_t + a.grad u = 0, upwind flux
+                                     (tests/self/Basic/080AdvectionLinear_SelfTest.cpp:108-170) */
+    HGDG_OP_ACOUSTIC_CENTRAL = 2  /* first-order wave system, central flux, reflecting walls
+                                     (applications/ExampleMultipleVariableProblem/AcousticWaveLinear_Impl.h:102-283) */
+};
+
+/* ---- arithmetic of the bulk contractions ------------------------------- */
+enum {
+    HGDG_PREC_FP64 = 0,       /* everything FP64 (the parity path)                  */
+    HGDG_PREC_MIXED_FP32 = 1  /* FP32 contractions, FP32 stage derivatives,
+                                 compensated FP32 face sums; FP64 geometry, state
+                                 and RK update                                      */
+};
+
+/*
+ * Mesh as hpGEM's MeshManipulator sees it after reading a .hpgem file:
+ * per element its vertices' physical coordinates (periodic copies already
+ * resolved) and its topological node ids (periodic images share an id), in
+ * the reference geometry's local node order. Faces are found from shared node
+ * ids, as kernel/Base/MeshManipulator_Impl.h:1860-1950 does.
+ */
+typedef struct hgdg_mesh_desc {
+    int32_t shape;            /* HGDG_SHAPE_*                                  */
+    int32_t dim;              /* 2 or 3 (must match the shape)                 */
+    int64_t n_elements;
+    const double* coords;     /* [n_elements][n_vertices][dim]                 */
+    const int64_t* node_ids;  /* [n_elements][n_vertices]                      */
+    const int32_t* order;     /* [n_elements] polynomial order p, or NULL      */
+    int32_t uniform_order;    /* used when order == NULL                       */
+} hgdg_mesh_desc;
+
+typedef struct hgdg_op_desc {
+    int32_t kind;             /* HGDG_OP_*                                     */
+    int32_t precision;        /* HGDG_PREC_*                                   */
+    double velocity[3];       /* advection velocity a                          */
+    const double* c_inv;      /* acoustic: 1/c^2-type weight per element (getCInv), [n_elements] */
+} hgdg_op_desc;
+
+typedef struct hgdg_ctx hgdg_ctx;
+
+/* Thread-local message of the last failing call on this thread. */
+const char* hgdg_last_error(void);
+
+/* Context on a HIP device; owns a stream, device buffers and graphs.
+ * Replaces the ownership role of HpgemAPIBase (kernel/Base/HpgemAPIBase_Impl.h:48-90). */
+int hgdg_ctx_create(hgdg_ctx** out, int device);
+int hgdg_ctx_destroy(hgdg_ctx* ctx);
+
+/* Upload the mesh; builds faces, geometry and reference tables.
+ * Replaces HpgemAPIBase::addMesh + MeshManipulator::readMesh face setup
+ * (kernel/Base/HpgemAPIBase_Impl.h:75-90, MeshManipulator_Impl.h:1505-1950). */
+int hgdg_mesh_upload(hgdg_ctx* ctx, const hgdg_mesh_desc* mesh);
+
+/* Sizes: elements, faces (internal + boundary), internal faces, DOF per field
+ * set (sum over elements of nB * n_fields). */
+int hgdg_mesh_info(hgdg_ctx* ctx, int64_t* n_elements, int64_t* n_faces, int64_t* n_internal_faces,
+                   int64_t* n_dof);
+
+/* Choose the operator (integrands) and the arithmetic.
+ * Replaces HpgemAPILinear::createMassMatrices/createStiffnessMatrices
+ * (kernel/Base/HpgemAPILinear_Impl.h:131-139,250-288). */
+int hgdg_operator_set(hgdg_ctx* ctx, const hgdg_op_desc* op);
+
+/* Allocate n time-integration vectors on the device (all zero).
+ * Replaces setNumberOfTimeIntegrationVectorsGlobally (HpgemAPIBase_Impl.h:141-156). */
+int hgdg_state_alloc(hgdg_ctx* ctx, int n_vectors);
+
+/* Copy a vector from / to the host in reference layout (n_dof doubles).
+ * Replaces Element::setTimeIntegrationVector / getTimeIntegrationVector. */
+int hgdg_state_set(hgdg_ctx* ctx, int vec, const double* host);
+int hgdg_state_get(hgdg_ctx* ctx, int vec, double* host);
+
+/* Storage of one vector: HGDG_PREC_FP64 (default) or HGDG_PREC_MIXED_FP32
+ * (FP32 storage, for stage derivatives of the mixed path). Contents are reset
+ * to zero. */
+int hgdg_state_set_storage(hgdg_ctx* ctx, int vec, int precision);
+
+/* out = M^{-1} r( sum_i coeffs[i] * vec[in_ids[i]] ): stage combination,
+ * volume + face integrals and inverse-mass apply in one pass.
+ * Replaces HpgemAPISimplified::computeTimeDerivative(ids, coeffs, out, t)
+ * (kernel/Base/HpgemAPISimplified_Impl.h:777-786), i.e.
+ * getLinearCombinationOfVectors + computeRightHandSide + solveMassMatrixEquations. */
+int hgdg_time_derivative(hgdg_ctx* ctx, const int* in_ids, const double* coeffs, int n_in, int out_id,
+                         double time);
+
+/* vec[to] += scale * vec[add]  (HpgemAPISimplified::scaleAndAddVector, :707-717). */
+int hgdg_scale_and_add(hgdg_ctx* ctx, int to, int add, double scale);
+
+/* n_steps steps of an explicit Runge-Kutta tableau with `stages` stages on
+ * vector `solution`, stage derivatives in vectors aux_ids[0..stages-1].
+ * a is stages x stages row-major (strictly lower), b has `stages` entries.
+ * Replaces HpgemAPISimplified::computeOneTimeStep (:788-820), looped.
+ * The step is captured once into a HIP graph and replayed. */
+int hgdg_rk_steps(hgdg_ctx* ctx, int solution, const int* aux_ids, int stages, const double* a,
+                  const double* b, double dt, int n_steps);
+
+/* vec = L2 projection of f(x) = sum_m amp[m] sin(2 pi k[m] . x), k given as
+ * [n_modes][3] (unused components ignored), with hpGEM's element rule of
+ * order 2p+1. Replaces HpgemAPISimplified::setInitialSolution
+ * (kernel/Base/HpgemAPISimplified_Impl.h:227-291,751-758) for this family of
+ * initial conditions. */
+int hgdg_project_modes(hgdg_ctx* ctx, int vec, int n_modes, const double* amp, const double* k);
+
+/* L2 norm of (vec[x] - vec[y]) under the element mass matrices, and the max
+ * abs coefficient difference (FP64, on the device). */
+int hgdg_state_diff(hgdg_ctx* ctx, int x, int y, double* l2, double* max_abs);
+
+/* Block until all queued work on the context's stream is done. */
+int hgdg_sync(hgdg_ctx* ctx);
+
+/* The context's hipStream_t, for callers timing with HIP events. */
+int hgdg_get_stream(hgdg_ctx* ctx, void** stream);
+
+/* ---- host-only utilities (no device needed) ------------------------------ */
+
+/* Number of DG basis functions of hpGEM's H1 set for shape and order. */
+int hgdg_basis_size(int shape, int p);
+
+/* Values of the library's basis (reference order) at one reference point. */
+int hgdg_basis_eval(int shape, int p, const double* point, double* values);
+
+/* Structured mesh of [lo,hi] with n[d] cells per direction, periodic flags,
+ * optional simplex split (5 tets per cube); element and node order of
+ * hpGEM's Preprocessor (applications/Preprocessor/lib/hpgem.cpp:100-235).
+ * Two-phase: call with coords == NULL to get n_elements and n_vertices. */
+int hgdg_structured_mesh(int dim, const int32_t* n, const double* lo, const double* hi,
+                         const int32_t* periodic, int simplices, int64_t* n_elements,
+                         int32_t* n_vertices, int32_t* shape, double* coords, int64_t* node_ids);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HGDG_H */
