@@ -5,18 +5,24 @@
 // (kernel/Base/HpgemAPILinear_Impl.h:131-139,250-288 with the integrands of
 // tests/self/Basic/080AdvectionLinear_SelfTest.cpp:108-170) factor into 1D
 // operators: with the basis psi_a(x) psi_b(y) psi_c(z), M = |J| M1 x M1 x M1,
-// so after the block-diagonal inverse mass apply the time derivative is
+// and after the block-diagonal inverse mass apply the weak form
 //
-//   k = sum_d ahat_d (D along d) w
-//       - sum_{faces (d,s)} (2s-1) ahat_d  Mi(:, s) (x) T_{d,s}
+//   k = sum_d ahat_d (D along d) w - sum_{faces (d,s)} (2s-1) ahat_d Mi(:,s) (x) T_{d,s}
 //
-// with ahat = J^{-1} a, D = M1^{-1} G1, Mi = M1^{-1}, and T_{d,s} the upwind
-// trace coefficients on face x_d = (2s-1) (the coefficients of the element or
-// of its neighbour whose psi index along d is s or 1-s). The 1D operators are
-// built once on the host in FP64 by exact Gauss quadrature; each element then
-// needs its stage input, three numbers of geometry and the upwind traces of its
-// inflow neighbours. One thread owns one coefficient; an element's stage input
-// sits in LDS for the 1D contractions.
+// (ahat = J^{-1} a, D = M1^{-1} G1, Mi = M1^{-1}, T_{d,s} the upwind trace
+// coefficients on face x_d = 2s-1) is evaluated in its integrated-by-parts
+// ("lifting") form, which is the same number up to rounding:
+//
+//   k = -sum_d ahat_d (Dr along d) w - sum_{faces} (2s-1) ahat_d Mi(:,s) (x) (T_{d,s} - own_{d,s})
+//
+// with Dr = M1^{-1} K the exact 1D derivative matrix (K(a,b) = int psi_a psi_b')
+// and own_{d,s} the element's own trace. Outflow faces drop out, and the face
+// terms only carry jumps, which are formed in FP64 from the FP64 stage input:
+// no large terms cancel, so the FP32 variant keeps ~1e-7 relative accuracy.
+// The 1D operators are built once on the host in long double by exact Gauss
+// quadrature; each element needs its stage input, `dim` numbers of geometry
+// and the traces of its inflow neighbours. One thread owns one coefficient;
+// an element's stage input sits in LDS for the 1D contractions.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -68,8 +74,8 @@ struct RhsParams {
     const double* ahat;      // [dim][n_elem]
     const int32_t* nbr;      // [2 dim][n_elem]
     const uint32_t* mode;    // [n_elem]
-    const double* D;         // [n1][n1]
-    const double* Mi;        // [n1][n1]
+    const double* Dr;        // [n1][n1] exact derivative matrix
+    const double* Mi;        // [n1][n1] inverse 1D mass
 };
 
 __device__ __forceinline__ double load_in(const RhsParams& p, int i, int64_t idx) {
@@ -119,40 +125,40 @@ struct Acc<float> {
 template <int DIM, int N1, typename T>
 __global__ __launch_bounds__(kBlock) void tensor_advection_rhs(const RhsParams p) {
     using S = Tensor<DIM, N1>;
-    __shared__ T s_w[S::EPB * S::NB];
+    __shared__ double s_w[S::EPB * S::NB];  // FP64 stage input
     __shared__ T s_D[N1 * N1];
     __shared__ T s_Mi[2 * N1];  // s_Mi[s * N1 + a] = Mi(a, s)
     const int tid = threadIdx.x;
-    if (tid < N1 * N1) s_D[tid] = T(p.D[tid]);
+    if (tid < N1 * N1) s_D[tid] = T(p.Dr[tid]);
     if (tid < 2 * N1) s_Mi[tid] = T(p.Mi[(tid % N1) * N1 + tid / N1]);
     const int le = tid / S::L;
     const int dof = tid % S::L;
     const int64_t e = int64_t(blockIdx.x) * S::EPB + le;
     const bool active = (e < p.n_elem) && (dof < S::NB);
-    if (active) s_w[le * S::NB + dof] = T(combine(p, e * S::NB + dof));
+    if (active) s_w[le * S::NB + dof] = combine(p, e * S::NB + dof);
     __syncthreads();
     if (!active) return;
 
     const int idx[3] = {dof % N1, (dof / N1) % N1, DIM == 3 ? dof / (N1 * N1) : 0};
     constexpr int stride[3] = {1, N1, N1 * N1};
-    const T* w = s_w + le * S::NB;
+    const double* w = s_w + le * S::NB;
     T ah[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) ah[d] = T(p.ahat[d * p.n_elem + e]);
 
-    // volume: sum_d ahat_d (D along d) w
+    // volume (strong form): -sum_d ahat_d (Dr along d) w
     T vol = T(0);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         const int row = dof - idx[d] * stride[d];
         T s = T(0);
 #pragma unroll
-        for (int j = 0; j < N1; ++j) s += s_D[idx[d] * N1 + j] * w[row + j * stride[d]];
-        vol += ah[d] * s;
+        for (int j = 0; j < N1; ++j) s += s_D[idx[d] * N1 + j] * T(w[row + j * stride[d]]);
+        vol -= ah[d] * s;
     }
     Acc<T> acc(vol);
 
-    // faces: -(2s-1) ahat_d Mi(idx_d, s) T_{d,s}
+    // faces: -(2s-1) ahat_d Mi(idx_d, s) (T_{d,s} - own_{d,s}); jumps in FP64
     const uint32_t md = p.mode[e];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
@@ -160,20 +166,20 @@ __global__ __launch_bounds__(kBlock) void tensor_advection_rhs(const RhsParams p
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const uint32_t m = (md >> (3 * (2 * d + s))) & 7u;
-            if (m == MODE_NONE) continue;
-            const T own = w[row + s * stride[d]];
-            T tr;
-            if (m == MODE_OWN) {
-                tr = own;
+            if (m == MODE_OWN) continue;
+            const double own = w[row + s * stride[d]];
+            double jump;
+            if (m == MODE_NONE) {
+                jump = -own;
             } else if (m == MODE_HALF_OWN) {
-                tr = T(0.5) * own;
+                jump = -0.5 * own;
             } else {
                 const int64_t nb = p.nbr[(2 * d + s) * p.n_elem + e];
-                const T other = T(combine(p, nb * S::NB + row + (1 - s) * stride[d]));
-                tr = (m == MODE_NBR) ? other : T(0.5) * (own + other);
+                const double other = combine(p, nb * S::NB + row + (1 - s) * stride[d]);
+                jump = (m == MODE_NBR) ? other - own : 0.5 * (other - own);
             }
             const T sgn = s ? T(-1) : T(1);
-            acc.add(sgn * ah[d] * s_Mi[s * N1 + idx[d]] * tr);
+            acc.add(sgn * ah[d] * s_Mi[s * N1 + idx[d]] * T(jump));
         }
     }
     const T r = acc.value();
@@ -354,7 +360,7 @@ struct hgdg_ctx {
     std::vector<int32_t> nbr;           // [2 dim][e]
     hgdg::TensorOps ops;
     hgdg::DevBuf<int32_t> d_nbr;
-    hgdg::DevBuf<double> d_absdet, d_ops;  // d_ops: D then Mi then M1
+    hgdg::DevBuf<double> d_absdet, d_ops;  // d_ops: Dr then Mi then M1
     // operator
     bool has_op = false;
     hgdg_op_desc op{};
@@ -385,7 +391,7 @@ void require(bool ok, int code, const std::string& msg) {
     if (!ok) throw Failure(code, msg);
 }
 
-const double* D_ptr(hgdg_ctx* c) { return c->d_ops.ptr; }
+const double* Dr_ptr(hgdg_ctx* c) { return c->d_ops.ptr; }
 const double* Mi_ptr(hgdg_ctx* c) { return c->d_ops.ptr + c->n1 * c->n1; }
 const double* M1_ptr(hgdg_ctx* c) { return c->d_ops.ptr + 2 * c->n1 * c->n1; }
 
@@ -538,7 +544,7 @@ void mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
 
     c->ops = tensor_ops(p);
     std::vector<double> opsv;
-    opsv.insert(opsv.end(), c->ops.D.begin(), c->ops.D.end());
+    opsv.insert(opsv.end(), c->ops.Dr.begin(), c->ops.Dr.end());
     opsv.insert(opsv.end(), c->ops.Mi.begin(), c->ops.Mi.end());
     opsv.insert(opsv.end(), c->ops.M1.begin(), c->ops.M1.end());
     c->d_ops.upload(opsv, c->stream);
@@ -649,7 +655,7 @@ void launch_rhs(hgdg_ctx* c, const int* in_ids, const double* coeffs, int n_in, 
     p.ahat = c->d_ahat.ptr;
     p.nbr = c->d_nbr.ptr;
     p.mode = c->d_mode.ptr;
-    p.D = D_ptr(c);
+    p.Dr = Dr_ptr(c);
     p.Mi = Mi_ptr(c);
     const bool mixed = c->op.precision == HGDG_PREC_MIXED_FP32;
     dispatch_tensor(c->dim, c->n1, [&](auto D, auto N) {

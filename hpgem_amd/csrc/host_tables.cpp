@@ -45,6 +45,28 @@ long double legendre(int n, long double t) {
 
 }  // namespace
 
+void gauss_legendre_ld(int n, std::vector<long double>& x, std::vector<long double>& w) {
+    x.assign(n, 0.0L);
+    w.assign(n, 0.0L);
+    if (n == 1) { w[0] = 2.0L; return; }
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int i = 0; i < n; ++i) {
+        long double t = -std::cos(pi * (i + 0.75L) / (n + 0.5L));
+        long double pn = 0, pn1 = 0, dp = 1;
+        for (int it = 0; it < 60; ++it) {
+            legendre_pair(n, t, pn, pn1);
+            dp = n * (t * pn - pn1) / (t * t - 1);
+            const long double dt = pn / dp;
+            t -= dt;
+            if (std::fabs(dt) < 1e-19L) break;
+        }
+        legendre_pair(n, t, pn, pn1);
+        dp = n * (t * pn - pn1) / (t * t - 1);
+        x[i] = t;
+        w[i] = 2.0L / ((1 - t * t) * dp * dp);
+    }
+}
+
 void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
     x.assign(n, 0.0);
     w.assign(n, 0.0);
@@ -65,6 +87,20 @@ void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
         x[i] = static_cast<double>(t);
         w[i] = static_cast<double>(2.0L / ((1 - t * t) * dp * dp));
     }
+}
+
+long double psi_ld(int a, long double t) {
+    if (a == 0) return (1 - t) / 2;
+    if (a == 1) return (1 + t) / 2;
+    const int k = a - 2;
+    return (legendre(k + 2, t) - legendre(k, t)) / std::sqrt(2.0L * (2 * k + 3));
+}
+
+long double dpsi_ld(int a, long double t) {
+    if (a == 0) return -0.5L;
+    if (a == 1) return 0.5L;
+    const int k = a - 2;
+    return std::sqrt((2.0L * k + 3) / 2) * legendre(k + 1, t);
 }
 
 double psi(int a, double t) {
@@ -180,28 +216,35 @@ TensorOps tensor_ops(int p) {
     TensorOps ops;
     const int n = p + 1;
     ops.n1 = n;
-    std::vector<double> x, w;
-    gauss_legendre(n + 1, x, w);  // exact for degree 2n+1 >= 2p
-    std::vector<long double> M(size_t(n) * n, 0), G(size_t(n) * n, 0);
+    std::vector<long double> x, w;
+    gauss_legendre_ld(n + 1, x, w);  // exact for degree 2n+1 >= 2p
+    std::vector<long double> M(size_t(n) * n, 0), G(size_t(n) * n, 0), K(size_t(n) * n, 0);
     for (size_t q = 0; q < x.size(); ++q)
         for (int a = 0; a < n; ++a)
             for (int b = 0; b < n; ++b) {
-                M[a * n + b] += static_cast<long double>(w[q]) * psi(a, x[q]) * psi(b, x[q]);
-                G[a * n + b] += static_cast<long double>(w[q]) * psi(b, x[q]) * dpsi(a, x[q]);
+                const long double wq = w[q];
+                M[a * n + b] += wq * psi_ld(a, x[q]) * psi_ld(b, x[q]);
+                G[a * n + b] += wq * psi_ld(b, x[q]) * dpsi_ld(a, x[q]);
+                K[a * n + b] += wq * psi_ld(a, x[q]) * dpsi_ld(b, x[q]);
             }
     std::vector<long double> Mi = invert(n, M);
     ops.M1.resize(size_t(n) * n);
     ops.G1.resize(size_t(n) * n);
     ops.Mi.resize(size_t(n) * n);
     ops.D.resize(size_t(n) * n);
+    ops.Dr.resize(size_t(n) * n);
     for (int a = 0; a < n; ++a)
         for (int b = 0; b < n; ++b) {
-            long double d = 0;
-            for (int c = 0; c < n; ++c) d += Mi[a * n + c] * G[c * n + b];
+            long double d = 0, dr = 0;
+            for (int c = 0; c < n; ++c) {
+                d += Mi[a * n + c] * G[c * n + b];
+                dr += Mi[a * n + c] * K[c * n + b];
+            }
             ops.M1[a * n + b] = static_cast<double>(M[a * n + b]);
             ops.G1[a * n + b] = static_cast<double>(G[a * n + b]);
             ops.Mi[a * n + b] = static_cast<double>(Mi[a * n + b]);
             ops.D[a * n + b] = static_cast<double>(d);
+            ops.Dr[a * n + b] = static_cast<double>(dr);
         }
     return ops;
 }

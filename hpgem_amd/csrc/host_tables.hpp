@@ -36,12 +36,14 @@ double dpsi(int a, double t);
 std::vector<int> tensor_permutation(int dim, int p);
 
 // 1D reference operators for order p (n1 = p+1), row-major n1 x n1:
-//   M1(a,b)  = int psi_a psi_b,  G1(a,b) = int psi_b psi_a'  (derivative on
-//   the test index a),  D = M1^{-1} G1,  Mi = M1^{-1}.
-// Integrals use an exact Gauss rule; inverses are computed in long double.
+//   M1(a,b) = int psi_a psi_b,  G1(a,b) = int psi_b psi_a' (derivative on the
+//   test index a),  K(a,b) = int psi_a psi_b',
+//   D = M1^{-1} G1 (weak derivative), Dr = M1^{-1} K (exact derivative of the
+//   interpolant: D = -Dr + Mi(:,1) e_1^T - Mi(:,0) e_0^T), Mi = M1^{-1}.
+// Integrals use an exact Gauss rule; everything is formed in long double.
 struct TensorOps {
     int n1 = 0;
-    std::vector<double> M1, G1, D, Mi;
+    std::vector<double> M1, G1, D, Dr, Mi;
 };
 TensorOps tensor_ops(int p);
 

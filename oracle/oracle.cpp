@@ -58,6 +58,15 @@
 
 namespace orc {
 
+#ifndef ORACLE_REAL
+#define ORACLE_REAL double
+#endif
+// Working precision: double (the checker of the FP64 path, rounding like the
+// reference) or long double (the same algorithm in extended precision, used
+// to measure how far the FP64 evaluations sit from the exact operator).
+typedef ORACLE_REAL Real;
+const Real kPi = Real(3.141592653589793238462643383279502884L);
+
 enum Shape { LINE = 1, SQUARE = 2, TRIANGLE = 3, CUBE = 4, TETRAHEDRON = 5 };
 
 int shape_dim(int s) {
@@ -92,13 +101,13 @@ int face_shape(int s) {
 struct Rule {
     int dim = 0;
     int order = 0;
-    std::vector<double> pts;  // [n][dim]
-    std::vector<double> w;
+    std::vector<Real> pts;  // [n][dim]
+    std::vector<Real> w;
     int n() const { return int(w.size()); }
 };
 
 // n-point Gauss-Legendre on [-1,1] by Newton iteration in long double.
-void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+void gauss_legendre(int n, std::vector<Real>& x, std::vector<Real>& w) {
     x.assign(n, 0.0);
     w.assign(n, 0.0);
     const long double pi = 3.141592653589793238462643383279502884L;
@@ -116,7 +125,7 @@ void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
             dp = n * (t * p1 - p0) / (t * t - 1);
             long double dt = p1 / dp;
             t -= dt;
-            if (std::fabs((double)dt) < 1e-22) break;
+            if (std::fabs((Real)dt) < 1e-22) break;
         }
         long double p0 = 1, p1 = t;
         for (int k = 2; k <= n; ++k) {
@@ -125,8 +134,8 @@ void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
             p1 = p2;
         }
         dp = n * (t * p1 - p0) / (t * t - 1);
-        x[n - 1 - i] = double(t);
-        w[n - 1 - i] = double(2.0L / ((1 - t * t) * dp * dp));
+        x[n - 1 - i] = Real(t);
+        w[n - 1 - i] = Real(2.0L / ((1 - t * t) * dp * dp));
     }
     if (n == 1) { x[0] = 0.0; w[0] = 2.0; }
 }
@@ -143,7 +152,7 @@ Rule get_rule(int shape, int order) {
     if (!is_simplex(shape)) {
         int n = std::max(1, (order + 2) / 2);
         if (n > 6) throw std::runtime_error("no tensor rule of that order");
-        std::vector<double> x, w;
+        std::vector<Real> x, w;
         gauss_legendre(n, x, w);
         Rule r;
         r.dim = shape_dim(shape);
@@ -152,7 +161,7 @@ Rule get_rule(int shape, int order) {
         for (int d = 0; d < r.dim; ++d) np *= n;
         for (int q = 0; q < np; ++q) {
             int rem = q;
-            double wt = 1;
+            Real wt = 1;
             for (int d = 0; d < r.dim; ++d) {
                 int i = rem % n;
                 rem /= n;
@@ -174,22 +183,22 @@ Rule get_rule(int shape, int order) {
 
 // ------------------------------------------------------------ Lobatto
 // kernel/FE/helperFunctions.cpp:49-108 (values to degree 4, derivatives to 3).
-double lobatto(int k, double x) {
+Real lobatto(int k, Real x) {
     switch (k) {
-        case 0: return std::sqrt(3. / 2.) * -2.;
-        case 1: return std::sqrt(5. / 2.) * x * -2.;
-        case 2: return std::sqrt(7. / 2.) * (5 * x * x - 1) / -2.;
-        case 3: return std::sqrt(9. / 2.) * (7 * x * x - 3) * x / -2.;
-        case 4: return std::sqrt(11. / 2.) * ((21 * x * x - 14) * x * x + 1) / -4.;
+        case 0: return std::sqrt(Real(3) / 2) * -2.;
+        case 1: return std::sqrt(Real(5) / 2) * x * -2.;
+        case 2: return std::sqrt(Real(7) / 2) * (5 * x * x - 1) / -2.;
+        case 3: return std::sqrt(Real(9) / 2) * (7 * x * x - 3) * x / -2.;
+        case 4: return std::sqrt(Real(11) / 2) * ((21 * x * x - 14) * x * x + 1) / -4.;
     }
     throw std::runtime_error("Lobatto degree out of range");
 }
-double lobatto_d(int k, double x) {
+Real lobatto_d(int k, Real x) {
     switch (k) {
         case 0: return 0;
-        case 1: return -2 * std::sqrt(5. / 2.);
-        case 2: return -2 * std::sqrt(7. / 2.) * (2.5 * x);
-        case 3: return -2 * std::sqrt(9. / 2.) * (5.25 * x * x - .75);
+        case 1: return -2 * std::sqrt(Real(5) / 2);
+        case 2: return -2 * std::sqrt(Real(7) / 2) * (2.5 * x);
+        case 3: return -2 * std::sqrt(Real(9) / 2) * (5.25 * x * x - .75);
     }
     throw std::runtime_error("Lobatto derivative degree out of range");
 }
@@ -204,17 +213,17 @@ struct Factor {
     int s = 0;       // linear slope sign
     int k = 0;       // Lobatto degree
     int m = 1;       // mirroring
-    double val(double t) const {
+    Real val(Real t) const {
         return bubble ? (1 - t) * (1 + t) * lobatto(k, m * t) : 1 + s * t;
     }
-    double der(double t) const {
+    Real der(Real t) const {
         return bubble ? -2 * t * lobatto(k, m * t) +
                             (1 - t) * (1 + t) * lobatto_d(k, m * t) * m
-                      : double(s);
+                      : Real(s);
     }
 };
 struct TensorFn {
-    double scale = 1;
+    Real scale = 1;
     std::array<Factor, 3> f;
 };
 // A simplex basis function: product of barycentrics and Lobatto kernels of
@@ -413,7 +422,7 @@ Basis make_basis(int shape, int p) {
 }
 
 // barycentric coordinates and their gradients (helperFunctions.cpp:110-164)
-void barycentric(int dim, const double* x, double* lam, double (*dlam)[3]) {
+void barycentric(int dim, const Real* x, Real* lam, Real (*dlam)[3]) {
     lam[0] = 1;
     for (int d = 0; d < dim; ++d) lam[0] -= x[d];
     for (int d = 0; d < 3; ++d) dlam[0][d] = d < dim ? -1 : 0;
@@ -423,19 +432,19 @@ void barycentric(int dim, const double* x, double* lam, double (*dlam)[3]) {
     }
 }
 
-void eval_basis(const Basis& b, const double* x, double* val, double* grad) {
+void eval_basis(const Basis& b, const Real* x, Real* val, Real* grad) {
     int dim = shape_dim(b.shape);
     if (!is_simplex(b.shape)) {
         for (int i = 0; i < b.size(); ++i) {
             const TensorFn& fn = b.tensor[i];
-            double v[3], dv[3];
+            Real v[3], dv[3];
             for (int d = 0; d < 3; ++d) {
                 v[d] = d < dim ? fn.f[d].val(x[d]) : 1.0;
                 dv[d] = d < dim ? fn.f[d].der(x[d]) : 0.0;
             }
             val[i] = fn.scale * v[0] * v[1] * v[2];
             for (int d = 0; d < dim; ++d) {
-                double g = fn.scale * dv[d];
+                Real g = fn.scale * dv[d];
                 for (int e = 0; e < dim; ++e)
                     if (e != d) g *= v[e];
                 grad[i * dim + d] = g;
@@ -443,29 +452,29 @@ void eval_basis(const Basis& b, const double* x, double* val, double* grad) {
         }
         return;
     }
-    double lam[4], dlam[4][3];
+    Real lam[4], dlam[4][3];
     barycentric(dim, x, lam, dlam);
     for (int i = 0; i < b.size(); ++i) {
         const SimplexFn& fn = b.simplex[i];
         // factors: lambdas then kernels; product rule
-        std::vector<double> fv, fd[3];
+        std::vector<Real> fv, fd[3];
         for (int l : fn.lam) {
             fv.push_back(lam[l]);
             for (int d = 0; d < 3; ++d) fd[d].push_back(dlam[l][d]);
         }
         for (auto& kn : fn.kern) {
-            double s = lam[kn[1]] - lam[kn[2]];
+            Real s = lam[kn[1]] - lam[kn[2]];
             fv.push_back(lobatto(kn[0], s));
-            double ld = lobatto_d(kn[0], s);
+            Real ld = lobatto_d(kn[0], s);
             for (int d = 0; d < 3; ++d) fd[d].push_back(ld * (dlam[kn[1]][d] - dlam[kn[2]][d]));
         }
-        double v = 1;
-        for (double f : fv) v *= f;
+        Real v = 1;
+        for (Real f : fv) v *= f;
         val[i] = v;
         for (int d = 0; d < dim; ++d) {
-            double g = 0;
+            Real g = 0;
             for (size_t a = 0; a < fv.size(); ++a) {
-                double t = fd[d][a];
+                Real t = fd[d][a];
                 for (size_t c = 0; c < fv.size(); ++c)
                     if (c != a) t *= fv[c];
                 g += t;
@@ -479,13 +488,13 @@ void eval_basis(const Basis& b, const double* x, double* val, double* grad) {
 // Codim-1 maps face reference point -> element reference point and the outward
 // normal sign (Mappings/MappingToRef{LineToSquare,SquareToCube,
 // TriangleToTetrahedron}.cpp).
-void face_map(int shape, int face, const double* pf, double* pe, double* jac /*[dim][dim-1]*/,
-              double* sign) {
+void face_map(int shape, int face, const Real* pf, Real* pe, Real* jac /*[dim][dim-1]*/,
+              Real* sign) {
     switch (shape) {
         case SQUARE: {
-            static const double S[4] = {-1, 1, -1, 1};
-            double t = pf[0];
-            double J[2] = {0, 0};
+            static const Real S[4] = {-1, 1, -1, 1};
+            Real t = pf[0];
+            Real J[2] = {0, 0};
             switch (face) {
                 case 0: pe[0] = t; pe[1] = -1; J[0] = 1; break;
                 case 1: pe[0] = -1; pe[1] = t; J[1] = 1; break;
@@ -497,9 +506,9 @@ void face_map(int shape, int face, const double* pf, double* pe, double* jac /*[
             return;
         }
         case CUBE: {
-            static const double S[6] = {-1, 1, -1, 1, -1, 1};
-            double a = pf[0], b = pf[1];
-            double J[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+            static const Real S[6] = {-1, 1, -1, 1, -1, 1};
+            Real a = pf[0], b = pf[1];
+            Real J[3][2] = {{0, 0}, {0, 0}, {0, 0}};
             switch (face) {
                 case 0: pe[0] = a; pe[1] = b; pe[2] = -1; J[0][0] = 1; J[1][1] = 1; break;
                 case 1: pe[0] = a; pe[1] = -1; pe[2] = b; J[0][0] = 1; J[2][1] = 1; break;
@@ -518,9 +527,9 @@ void face_map(int shape, int face, const double* pf, double* pe, double* jac /*[
             // MappingToRefTriangleToTetrahedron0..3: the face's nodes, in the
             // order of TET_FACE_NODES, are the images of triangle nodes
             // (0,0),(1,0),(0,1); the map is affine.
-            static const double V[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+            static const Real V[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
             const int* n = TET_FACE_NODES[face];
-            double a = pf[0], b = pf[1];
+            Real a = pf[0], b = pf[1];
             for (int d = 0; d < 3; ++d) {
                 pe[d] = V[n[0]][d] + a * (V[n[1]][d] - V[n[0]][d]) + b * (V[n[2]][d] - V[n[0]][d]);
                 if (jac) {
@@ -537,13 +546,13 @@ void face_map(int shape, int face, const double* pf, double* pe, double* jac /*[
 
 // Codim-0 orientation maps of the face shape (MappingToRefLineToLine,
 // MappingToRefSquareToSquare0..7, MappingToRefTriangleToTriangle0..5).
-void orientation_map(int fshape, int idx, const double* pin, double* pout) {
+void orientation_map(int fshape, int idx, const Real* pin, Real* pout) {
     switch (fshape) {
         case LINE:
             pout[0] = idx == 0 ? pin[0] : -pin[0];
             return;
         case SQUARE: {
-            double x = pin[0], y = pin[1];
+            Real x = pin[0], y = pin[1];
             switch (idx) {
                 case 0: pout[0] = x; pout[1] = y; return;
                 case 1: pout[0] = -y; pout[1] = x; return;
@@ -557,7 +566,7 @@ void orientation_map(int fshape, int idx, const double* pin, double* pout) {
             break;
         }
         case TRIANGLE: {
-            double x = pin[0], y = pin[1];
+            Real x = pin[0], y = pin[1];
             switch (idx) {
                 case 0: pout[0] = x; pout[1] = y; return;
                 case 1: pout[0] = y; pout[1] = x; return;
@@ -625,7 +634,7 @@ int n_faces(int shape) {
 // Hypercubes: multilinear map through the vertices in reference node order
 // (MappingToPhysHypercubeLinear.cpp). Simplices: affine map
 // x = X0 + sum_d (X_{d+1} - X0) xi_d (MappingToPhysSimplexLinear_Impl.h).
-void elem_map(int shape, const double* X /*[nv][dim]*/, const double* xi, double* x, double* J /*[dim][dim]*/) {
+void elem_map(int shape, const Real* X /*[nv][dim]*/, const Real* xi, Real* x, Real* J /*[dim][dim]*/) {
     int dim = shape_dim(shape);
     int nv = shape_nodes(shape);
     for (int r = 0; r < dim; ++r) {
@@ -636,7 +645,7 @@ void elem_map(int shape, const double* X /*[nv][dim]*/, const double* xi, double
         for (int r = 0; r < dim; ++r) {
             x[r] = X[r];
             for (int c = 0; c < dim; ++c) {
-                double e = X[(c + 1) * dim + r] - X[r];
+                Real e = X[(c + 1) * dim + r] - X[r];
                 x[r] += e * xi[c];
                 J[r * dim + c] = e;
             }
@@ -644,11 +653,11 @@ void elem_map(int shape, const double* X /*[nv][dim]*/, const double* xi, double
         return;
     }
     for (int v = 0; v < nv; ++v) {
-        double N = 1;
-        double dN[3];
+        Real N = 1;
+        Real dN[3];
         for (int d = 0; d < dim; ++d) N *= (1 + node_sign(v, d) * xi[d]) / 2;
         for (int c = 0; c < dim; ++c) {
-            double g = node_sign(v, c) / 2.0;
+            Real g = node_sign(v, c) / 2.0;
             for (int d = 0; d < dim; ++d)
                 if (d != c) g *= (1 + node_sign(v, d) * xi[d]) / 2;
             dN[c] = g;
@@ -660,7 +669,7 @@ void elem_map(int shape, const double* X /*[nv][dim]*/, const double* xi, double
     }
 }
 
-double det(int dim, const double* J) {
+Real det(int dim, const Real* J) {
     switch (dim) {
         case 1: return J[0];
         case 2: return J[0] * J[3] - J[1] * J[2];
@@ -672,7 +681,7 @@ double det(int dim, const double* J) {
 }
 
 // grad_phys = J^{-T} grad_ref by solving J^T g = gref (H1ConformingTransformation.h:64-69)
-void solve_small(int n, double* A /*row-major n*n*/, double* b) {
+void solve_small(int n, Real* A /*row-major n*n*/, Real* b) {
     // Gaussian elimination with partial pivoting
     for (int k = 0; k < n; ++k) {
         int piv = k;
@@ -683,13 +692,13 @@ void solve_small(int n, double* A /*row-major n*n*/, double* b) {
             std::swap(b[k], b[piv]);
         }
         for (int i = k + 1; i < n; ++i) {
-            double f = A[i * n + k] / A[k * n + k];
+            Real f = A[i * n + k] / A[k * n + k];
             for (int j = k; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
             b[i] -= f * b[k];
         }
     }
     for (int k = n - 1; k >= 0; --k) {
-        double s = b[k];
+        Real s = b[k];
         for (int j = k + 1; j < n; ++j) s -= A[k * n + j] * b[j];
         b[k] = s / A[k * n + k];
     }
@@ -698,23 +707,23 @@ void solve_small(int n, double* A /*row-major n*n*/, double* b) {
 // ------------------------------------------------------------ dense LU solve
 // MiddleSizeMatrix::solve -> dgesv on a copy (MiddleSizeMatrix.cpp:723-745):
 // LU with partial pivoting, then forward/back substitution. A is column-major.
-void lu_solve_copy(int n, const std::vector<double>& Acm, double* b) {
-    std::vector<double> A(Acm);
+void lu_solve_copy(int n, const std::vector<Real>& Acm, Real* b) {
+    std::vector<Real> A(Acm);
     std::vector<int> piv(n);
     for (int k = 0; k < n; ++k) {
         int p = k;
-        double best = std::fabs(A[k + k * n]);
+        Real best = std::fabs(A[k + k * n]);
         for (int i = k + 1; i < n; ++i) {
-            double v = std::fabs(A[i + k * n]);
+            Real v = std::fabs(A[i + k * n]);
             if (v > best) { best = v; p = i; }
         }
         piv[k] = p;
         if (p != k)
             for (int j = 0; j < n; ++j) std::swap(A[k + j * n], A[p + j * n]);
-        double inv = 1.0 / A[k + k * n];
+        Real inv = 1.0 / A[k + k * n];
         for (int i = k + 1; i < n; ++i) A[i + k * n] *= inv;
         for (int j = k + 1; j < n; ++j) {
-            double akj = A[k + j * n];
+            Real akj = A[k + j * n];
             if (akj != 0.0)
                 for (int i = k + 1; i < n; ++i) A[i + j * n] -= A[i + k * n] * akj;
         }
@@ -722,12 +731,12 @@ void lu_solve_copy(int n, const std::vector<double>& Acm, double* b) {
     for (int k = 0; k < n; ++k)
         if (piv[k] != k) std::swap(b[k], b[piv[k]]);
     for (int j = 0; j < n; ++j) {
-        double bj = b[j];
+        Real bj = b[j];
         for (int i = j + 1; i < n; ++i) b[i] -= A[i + j * n] * bj;
     }
     for (int j = n - 1; j >= 0; --j) {
         b[j] /= A[j + j * n];
-        double bj = b[j];
+        Real bj = b[j];
         for (int i = 0; i < j; ++i) b[i] -= A[i + j * n] * bj;
     }
 }
@@ -745,19 +754,19 @@ struct FaceRec {
 struct Problem {
     int shape = 0, dim = 0, nv = 0, nfields = 1;
     int64_t nE = 0;
-    std::vector<double> X;        // [nE][nv][dim]
+    std::vector<Real> X;        // [nE][nv][dim]
     std::vector<int64_t> nodes;   // [nE][nv]
     std::vector<int> order;       // [nE]
     std::vector<int64_t> offset;  // [nE+1] DOF offsets (nB * nfields per element)
     std::vector<FaceRec> faces;
     std::map<int, Basis> bases;   // by p
     int op = OP_NONE;
-    double a[3] = {0, 0, 0};
-    std::vector<double> cinv;     // acoustic: per element
+    Real a[3] = {0, 0, 0};
+    std::vector<Real> cinv;     // acoustic: per element
     // assembled (HpgemAPILinear) data
-    std::vector<std::vector<double>> M;      // per element, column-major (nB*nf)^2
-    std::vector<std::vector<double>> S;      // per element stiffness, column-major
-    std::vector<std::array<std::vector<double>, 4>> F;  // per face: LL, LR, RL, RR column-major
+    std::vector<std::vector<Real>> M;      // per element, column-major (nB*nf)^2
+    std::vector<std::vector<Real>> S;      // per element stiffness, column-major
+    std::vector<std::array<std::vector<Real>, 4>> F;  // per face: LL, LR, RL, RR column-major
     bool assembled = false;
 
     const Basis& basis(int p) {
@@ -766,7 +775,7 @@ struct Problem {
         return it->second;
     }
     int nB(int64_t e) { return basis(order[e]).size(); }
-    const double* Xe(int64_t e) const { return &X[size_t(e) * nv * dim]; }
+    const Real* Xe(int64_t e) const { return &X[size_t(e) * nv * dim]; }
 };
 
 // Face search of MeshManipulator::readMesh (MeshManipulator_Impl.h:1860-1950):
@@ -836,24 +845,24 @@ void build_faces(Problem& P) {
 
 // Evaluation context at one face quadrature point (PhysicalFace).
 struct FacePoint {
-    double w;                 // quadrature weight * |n|
-    double nunit[3];          // unit normal, left -> right
-    double x[3];              // physical point (left side)
-    std::vector<double> phiL, phiR;
+    Real w;                 // quadrature weight * |n|
+    Real nunit[3];          // unit normal, left -> right
+    Real x[3];              // physical point (left side)
+    std::vector<Real> phiL, phiR;
 };
 
-void face_point(Problem& P, const FaceRec& f, const double* pf, double wq, FacePoint& out) {
+void face_point(Problem& P, const FaceRec& f, const Real* pf, Real wq, FacePoint& out) {
     int dim = P.dim;
-    double peL[3], jf[6], sgn;
+    Real peL[3], jf[6], sgn;
     face_map(P.shape, f.fL, pf, peL, jf, &sgn);
-    double x[3], J[9];
+    Real x[3], J[9];
     elem_map(P.shape, P.Xe(f.L), peL, x, J);
     // j3 = J_elem * J_face  (dim x dim-1)
-    double j3[6] = {0, 0, 0, 0, 0, 0};
+    Real j3[6] = {0, 0, 0, 0, 0, 0};
     for (int r = 0; r < dim; ++r)
         for (int c = 0; c < dim - 1; ++c)
             for (int k = 0; k < dim; ++k) j3[r * (dim - 1) + c] += J[r * dim + k] * jf[k * (dim - 1) + c];
-    double n[3] = {0, 0, 0};
+    Real n[3] = {0, 0, 0};
     if (dim == 2) {
         n[0] = -j3[1 * 1 + 0];
         n[1] = j3[0];
@@ -864,9 +873,9 @@ void face_point(Problem& P, const FaceRec& f, const double* pf, double wq, FaceP
         n[1] = m(0, 1) * m(2, 0) - m(0, 0) * m(2, 1);
         n[2] = m(0, 0) * m(1, 1) - m(1, 0) * m(0, 1);
     }
-    double dJ = det(dim, J);
-    double s = (dJ > 0 ? 1.0 : -1.0) * sgn;
-    double nn = 0;
+    Real dJ = det(dim, J);
+    Real s = (dJ > 0 ? 1.0 : -1.0) * sgn;
+    Real nn = 0;
     for (int d = 0; d < dim; ++d) {
         n[d] *= s;
         nn += n[d] * n[d];
@@ -879,16 +888,16 @@ void face_point(Problem& P, const FaceRec& f, const double* pf, double wq, FaceP
     }
     const Basis& bL = P.basis(P.order[f.L]);
     out.phiL.assign(bL.size(), 0);
-    std::vector<double> g(size_t(bL.size()) * dim);
+    std::vector<Real> g(size_t(bL.size()) * dim);
     eval_basis(bL, peL, out.phiL.data(), g.data());
     out.phiR.clear();
     if (f.internal) {
-        double pf2[2], peR[3];
+        Real pf2[2], peR[3];
         orientation_map(face_shape(P.shape), f.orient, pf, pf2);
         face_map(P.shape, f.fR, pf2, peR, nullptr, nullptr);
         const Basis& bR = P.basis(P.order[f.R]);
         out.phiR.assign(bR.size(), 0);
-        std::vector<double> gR(size_t(bR.size()) * dim);
+        std::vector<Real> gR(size_t(bR.size()) * dim);
         eval_basis(bR, peR, out.phiR.data(), gR.data());
     }
 }
@@ -904,25 +913,25 @@ int face_rule_order(Problem& P, const FaceRec& f) {
 
 // Element-point evaluation context (PhysicalElement).
 struct ElemPoint {
-    double w;  // weight * |det J|
-    double x[3];
-    std::vector<double> phi, dphi;  // dphi physical [nB][dim]
+    Real w;  // weight * |det J|
+    Real x[3];
+    std::vector<Real> phi, dphi;  // dphi physical [nB][dim]
 };
 
-void elem_point(Problem& P, int64_t e, const double* xi, double wq, ElemPoint& out) {
+void elem_point(Problem& P, int64_t e, const Real* xi, Real wq, ElemPoint& out) {
     int dim = P.dim;
-    double J[9];
+    Real J[9];
     elem_map(P.shape, P.Xe(e), xi, out.x, J);
-    double dJ = det(dim, J);
+    Real dJ = det(dim, J);
     out.w = wq * std::fabs(dJ);
     const Basis& b = P.basis(P.order[e]);
     int n = b.size();
     out.phi.assign(n, 0);
-    std::vector<double> gref(size_t(n) * dim);
+    std::vector<Real> gref(size_t(n) * dim);
     eval_basis(b, xi, out.phi.data(), gref.data());
     out.dphi.assign(size_t(n) * dim, 0);
     for (int i = 0; i < n; ++i) {
-        double A[9], rhs[3];
+        Real A[9], rhs[3];
         for (int r = 0; r < dim; ++r)
             for (int c = 0; c < dim; ++c) A[r * dim + c] = J[c * dim + r];  // J^T
         for (int d = 0; d < dim; ++d) rhs[d] = gref[i * dim + d];
@@ -943,8 +952,8 @@ void assemble(Problem& P) {
     for (int64_t e = 0; e < P.nE; ++e) {
         int nB = P.nB(e);
         int N = nB * nfld;
-        std::vector<double>& M = P.M[e];
-        std::vector<double>& S = P.S[e];
+        std::vector<Real>& M = P.M[e];
+        std::vector<Real>& S = P.S[e];
         M.assign(size_t(N) * N, 0);
         S.assign(size_t(N) * N, 0);
         Rule r = get_rule(P.shape, 2 * P.order[e] + 1);
@@ -952,7 +961,7 @@ void assemble(Problem& P) {
         for (int q = 0; q < r.n(); ++q) {
             elem_point(P, e, &r.pts[q * dim], r.w[q], ep);
             for (int v = 0; v < nfld; ++v) {
-                double mscale = 1.0;
+                Real mscale = 1.0;
                 if (P.op == OP_ACOUSTIC && v > 0) mscale = P.cinv[e];
                 for (int j = 0; j < nB; ++j)
                     for (int i = 0; i < nB; ++i)
@@ -962,7 +971,7 @@ void assemble(Problem& P) {
                 // result(j,i) = phi_i * (a . grad phi_j): row j = test, column i = trial
                 for (int i = 0; i < nB; ++i)
                     for (int j = 0; j < nB; ++j) {
-                        double adg = 0;
+                        Real adg = 0;
                         for (int d = 0; d < dim; ++d) adg += P.a[d] * ep.dphi[j * dim + d];
                         S[j + size_t(i) * N] += ep.w * ep.phi[i] * adg;
                     }
@@ -992,22 +1001,22 @@ void assemble(Problem& P) {
         for (int q = 0; q < r.n(); ++q) {
             face_point(P, f, &r.pts[q * (dim - 1)], r.w[q], fp);
             if (P.op == OP_ADVECTION) {
-                double A = 0;
+                Real A = 0;
                 for (int d = 0; d < dim; ++d) A += P.a[d] * fp.nunit[d];
                 int nL = int(fp.phiL.size()), nR = int(fp.phiR.size());
                 // integrandVal(j, i) = -(a . n_j phi_j) phi_i for trial i on the
                 // upwind side; both sides halved when a.n vanishes
                 // (080AdvectionLinear_SelfTest.cpp:125-170).
-                auto add = [&](int bj, int rows, int j, int bi, int i, double v) {
+                auto add = [&](int bj, int rows, int j, int bi, int i, Real v) {
                     blk[bj * 2 + bi][j + size_t(i) * rows] += fp.w * v;
                 };
                 for (int sj = 0; sj < (f.internal ? 2 : 1); ++sj) {
-                    const std::vector<double>& pj = sj == 0 ? fp.phiL : fp.phiR;
-                    double sgnj = sj == 0 ? 1.0 : -1.0;
+                    const std::vector<Real>& pj = sj == 0 ? fp.phiL : fp.phiR;
+                    Real sgnj = sj == 0 ? 1.0 : -1.0;
                     int rows = sj == 0 ? nL : nR;
                     for (int si = 0; si < (f.internal ? 2 : 1); ++si) {
-                        const std::vector<double>& pi = si == 0 ? fp.phiL : fp.phiR;
-                        double factor;
+                        const std::vector<Real>& pi = si == 0 ? fp.phiL : fp.phiR;
+                        Real factor;
                         if (A > 1e-12 && si == 0) factor = 1.0;
                         else if (A < -1e-12 && si == 1) factor = 1.0;
                         else if (std::fabs(A) < 1e-12) factor = 0.5;
@@ -1026,8 +1035,8 @@ void assemble(Problem& P) {
                 // faces carries -phi_i (n . s) for v and nothing for s.
                 int nBe_L = int(fp.phiL.size());
                 int nBe_R = int(fp.phiR.size());
-                const double* n = fp.nunit;
-                auto addb = [&](int blkid, int rows, int row, int col, double v) {
+                const Real* n = fp.nunit;
+                auto addb = [&](int blkid, int rows, int row, int col, Real v) {
                     blk[blkid][row + size_t(col) * rows] += fp.w * v;
                 };
                 if (!f.internal) {
@@ -1040,18 +1049,18 @@ void assemble(Problem& P) {
                 } else {
                     // test side t (0 = L, 1 = R), trial side u
                     for (int t = 0; t < 2; ++t) {
-                        const std::vector<double>& pt = t == 0 ? fp.phiL : fp.phiR;
+                        const std::vector<Real>& pt = t == 0 ? fp.phiL : fp.phiR;
                         int nt = t == 0 ? nBe_L : nBe_R;
                         for (int u = 0; u < 2; ++u) {
-                            const std::vector<double>& pu = u == 0 ? fp.phiL : fp.phiR;
+                            const std::vector<Real>& pu = u == 0 ? fp.phiL : fp.phiR;
                             int nu = u == 0 ? nBe_L : nBe_R;
-                            double su = u == 0 ? 1.0 : -1.0;  // normal sign of the trial side
+                            Real su = u == 0 ? 1.0 : -1.0;  // normal sign of the trial side
                             int blkid = t * 2 + u;
                             int rows = nt * nfld;
                             for (int i = 0; i < nt; ++i)
                                 for (int j = 0; j < nu; ++j)
                                     for (int d = 0; d < dim; ++d) {
-                                        double v = -0.5 * pt[i] * su * n[d] * pu[j];
+                                        Real v = -0.5 * pt[i] * su * n[d] * pu[j];
                                         addb(blkid, rows, 0 * nt + i, (1 + d) * nu + j, v);
                                         addb(blkid, rows, (1 + d) * nt + i, 0 * nu + j, v);
                                     }
@@ -1065,18 +1074,18 @@ void assemble(Problem& P) {
 }
 
 // getLinearCombinationOfVectors + multiplyStiffnessMatrices + mass solve
-void time_derivative(Problem& P, int nin, const double* const* vecs, const double* coef, double* out) {
+void time_derivative(Problem& P, int nin, const Real* const* vecs, const Real* coef, Real* out) {
     if (!P.assembled) assemble(P);
     int64_t ndof = P.offset[P.nE];
-    std::vector<double> w(ndof);
+    std::vector<Real> w(ndof);
     // first vector scaled, then axpy for the rest (HpgemAPISimplified_Impl.h:599-620)
     for (int64_t i = 0; i < ndof; ++i) w[i] = vecs[0][i] * coef[0];
     for (int k = 1; k < nin; ++k)
         for (int64_t i = 0; i < ndof; ++i) w[i] += coef[k] * vecs[k][i];
-    auto gemv_add = [](int rows, int cols, const std::vector<double>& A, const double* x, double* y, bool overwrite) {
-        std::vector<double> t(rows, 0.0);
+    auto gemv_add = [](int rows, int cols, const std::vector<Real>& A, const Real* x, Real* y, bool overwrite) {
+        std::vector<Real> t(rows, 0.0);
         for (int c = 0; c < cols; ++c) {
-            double xc = x[c];
+            Real xc = x[c];
             for (int r = 0; r < rows; ++r) t[r] += A[r + size_t(c) * rows] * xc;
         }
         for (int r = 0; r < rows; ++r) y[r] = overwrite ? t[r] : y[r] + t[r];
@@ -1105,18 +1114,18 @@ void time_derivative(Problem& P, int nin, const double* const* vecs, const doubl
 }
 
 // RK tableau: classic RK4 (RK4Methods.cpp:45-63)
-const double RK4_A[4][3] = {{0, 0, 0}, {0.5, 0, 0}, {0.0, 0.5, 0}, {0.0, 0.0, 1.0}};
-const double RK4_B[4] = {1.0 / 6, 1.0 / 3, 1.0 / 3, 1.0 / 6};
+const Real RK4_A[4][3] = {{0, 0, 0}, {0.5, 0, 0}, {0.0, 0.5, 0}, {0.0, 0.0, 1.0}};
+const Real RK4_B[4] = {Real(1) / 6, Real(1) / 3, Real(1) / 3, Real(1) / 6};
 
 // One step of an explicit tableau with s stages (a row-major s x s, strictly
 // lower), exactly as computeOneTimeStep: stage inputs u + dt sum_j a_ij k_j,
 // then u += dt b_j k_j one stage at a time.
-void rk_step(Problem& P, double* u, double dt, int s, const double* a, const double* b) {
+void rk_step(Problem& P, Real* u, Real dt, int s, const Real* a, const Real* b) {
     int64_t ndof = P.offset[P.nE];
-    std::vector<std::vector<double>> k(s, std::vector<double>(ndof));
+    std::vector<std::vector<Real>> k(s, std::vector<Real>(ndof));
     for (int i = 0; i < s; ++i) {
-        std::vector<const double*> vecs{u};
-        std::vector<double> coef{1.0};
+        std::vector<const Real*> vecs{u};
+        std::vector<Real> coef{1.0};
         for (int j = 0; j < i; ++j) {
             vecs.push_back(k[j].data());
             coef.push_back(dt * a[i * s + j]);
@@ -1124,17 +1133,17 @@ void rk_step(Problem& P, double* u, double dt, int s, const double* a, const dou
         time_derivative(P, int(vecs.size()), vecs.data(), coef.data(), k[i].data());
     }
     for (int j = 0; j < s; ++j) {
-        double c = dt * b[j];
+        Real c = dt * b[j];
         for (int64_t i = 0; i < ndof; ++i) u[i] += c * k[j][i];
     }
 }
 
-void rk4_step(Problem& P, double* u, double dt) {
+void rk4_step(Problem& P, Real* u, Real dt) {
     int64_t ndof = P.offset[P.nE];
-    std::vector<std::vector<double>> k(4, std::vector<double>(ndof));
+    std::vector<std::vector<Real>> k(4, std::vector<Real>(ndof));
     for (int s = 0; s < 4; ++s) {
-        std::vector<const double*> vecs{u};
-        std::vector<double> coef{1.0};
+        std::vector<const Real*> vecs{u};
+        std::vector<Real> coef{1.0};
         for (int j = 0; j < s; ++j) {
             vecs.push_back(k[j].data());
             coef.push_back(dt * RK4_A[s][j]);
@@ -1142,7 +1151,7 @@ void rk4_step(Problem& P, double* u, double dt) {
         time_derivative(P, int(vecs.size()), vecs.data(), coef.data(), k[s].data());
     }
     for (int s = 0; s < 4; ++s) {
-        double c = dt * RK4_B[s];
+        Real c = dt * RK4_B[s];
         for (int64_t i = 0; i < ndof; ++i) u[i] += c * k[s][i];
     }
 }
@@ -1150,14 +1159,14 @@ void rk4_step(Problem& P, double* u, double dt) {
 // Initial conditions: sum of sine modes  f(x) = sum_m A_m sin(2 pi k_m . x)
 struct Modes {
     int n = 0;
-    const double* amp = nullptr;
-    const double* k = nullptr;  // [n][3]
-    double eval(const double* x, int dim, const double* shift) const {
-        double s = 0;
+    const Real* amp = nullptr;
+    const Real* k = nullptr;  // [n][3]
+    Real eval(const Real* x, int dim, const Real* shift) const {
+        Real s = 0;
         for (int m = 0; m < n; ++m) {
-            double arg = 0;
+            Real arg = 0;
             for (int d = 0; d < dim; ++d) arg += k[m * 3 + d] * (x[d] - (shift ? shift[d] : 0.0));
-            s += amp[m] * std::sin(2 * M_PI * arg);
+            s += amp[m] * std::sin(2 * kPi * arg);
         }
         return s;
     }
@@ -1165,24 +1174,24 @@ struct Modes {
 
 // integrateInitialSolution + solveMassMatrixEquations (HpgemAPISimplified_Impl.h:227-291,751-758);
 // every field receives the same scalar function scaled by field_scale[v].
-void project(Problem& P, const Modes& m, const double* field_scale, double* u) {
+void project(Problem& P, const Modes& m, const Real* field_scale, Real* u) {
     if (!P.assembled) assemble(P);
     int dim = P.dim;
     for (int64_t e = 0; e < P.nE; ++e) {
         int nB = P.nB(e);
         int N = nB * P.nfields;
-        std::vector<double> r(N, 0.0);
+        std::vector<Real> r(N, 0.0);
         Rule rule = get_rule(P.shape, 2 * P.order[e] + 1);
         ElemPoint ep;
         for (int q = 0; q < rule.n(); ++q) {
             elem_point(P, e, &rule.pts[q * dim], rule.w[q], ep);
-            double f = m.eval(ep.x, dim, nullptr);
+            Real f = m.eval(ep.x, dim, nullptr);
             for (int v = 0; v < P.nfields; ++v)
                 for (int i = 0; i < nB; ++i) r[v * nB + i] += ep.w * ep.phi[i] * f * field_scale[v];
         }
         // Mass matrix without the acoustic weighting: the initial solution is a
         // plain L2 projection of each field.
-        std::vector<double> M(size_t(N) * N, 0.0);
+        std::vector<Real> M(size_t(N) * N, 0.0);
         for (int q = 0; q < rule.n(); ++q) {
             elem_point(P, e, &rule.pts[q * dim], rule.w[q], ep);
             for (int v = 0; v < P.nfields; ++v)
@@ -1197,10 +1206,10 @@ void project(Problem& P, const Modes& m, const double* field_scale, double* u) {
 
 // computeTotalError with rule order 2p+2 (HpgemAPISimplified_Impl.h:339-401),
 // error against the shifted initial modes u0(x - a t) for field 0.
-double l2_error(Problem& P, const double* u, const Modes& m, double t) {
+Real l2_error(Problem& P, const Real* u, const Modes& m, Real t) {
     int dim = P.dim;
-    double total = 0;
-    double shift[3];
+    Real total = 0;
+    Real shift[3];
     for (int d = 0; d < 3; ++d) shift[d] = P.a[d] * t;
     for (int64_t e = 0; e < P.nE; ++e) {
         int nB = P.nB(e);
@@ -1208,9 +1217,9 @@ double l2_error(Problem& P, const double* u, const Modes& m, double t) {
         ElemPoint ep;
         for (int q = 0; q < rule.n(); ++q) {
             elem_point(P, e, &rule.pts[q * dim], rule.w[q], ep);
-            double num = 0;
+            Real num = 0;
             for (int j = 0; j < nB; ++j) num += ep.phi[j] * u[P.offset[e] + j];
-            double err = m.eval(ep.x, dim, shift) - num;
+            Real err = m.eval(ep.x, dim, shift) - num;
             total += ep.w * err * err;
         }
     }
@@ -1221,11 +1230,11 @@ double l2_error(Problem& P, const double* u, const Modes& m, double t) {
 // StructuredReader (applications/Preprocessor/lib/hpgem.cpp:100-235).
 struct MeshOut {
     int shape = 0, dim = 0, nv = 0;
-    std::vector<double> X;
+    std::vector<Real> X;
     std::vector<int64_t> nodes;
 };
 
-MeshOut structured_mesh(int dim, const int* n, const double* lo, const double* hi, const int* periodic,
+MeshOut structured_mesh(int dim, const int* n, const Real* lo, const Real* hi, const int* periodic,
                         int simplices) {
     MeshOut out;
     out.dim = dim;
@@ -1250,12 +1259,12 @@ MeshOut structured_mesh(int dim, const int* n, const double* lo, const double* h
     for (int64_t el = 0; el < total; ++el) {
         // hypercube vertices: bit d of the local index = +1 step along d
         std::vector<int64_t> vnode(nvc);
-        std::vector<double> vx(size_t(nvc) * dim);
+        std::vector<Real> vx(size_t(nvc) * dim);
         for (int j = 0; j < nvc; ++j) {
             int64_t node = 0;
             for (int d = 0; d < dim; ++d) {
                 int idx = li[d] + ((j >> d) & 1);
-                double xc = lo[d] + idx * (hi[d] - lo[d]) / n[d];
+                Real xc = lo[d] + idx * (hi[d] - lo[d]) / n[d];
                 if (idx == nn[d]) idx = 0;  // periodic wrap onto the copy coordinate
                 node += idx * mult[d];
                 vx[j * dim + d] = xc;
@@ -1347,16 +1356,33 @@ int orc_basis_size(int shape, int p) {
 int orc_basis_eval(int shape, int p, const double* pt, double* vals, double* grads) {
     ORC_TRY({
         Basis b = make_basis(shape, p);
-        eval_basis(b, pt, vals, grads);
+        const int dim = shape_dim(shape);
+        std::vector<Real> x(pt, pt + dim), v(b.size()), g(size_t(b.size()) * dim);
+        eval_basis(b, x.data(), v.data(), g.data());
+        std::copy(v.begin(), v.end(), vals);
+        std::copy(g.begin(), g.end(), grads);
     })
 }
 
 int orc_face_map(int shape, int face, const double* pin, double* pout, double* sign) {
-    ORC_TRY({ face_map(shape, face, pin, pout, nullptr, sign); })
+    ORC_TRY({
+        const int dim = shape_dim(shape);
+        Real pi[3] = {0, 0, 0}, po[3] = {0, 0, 0}, sg = 0;
+        for (int d = 0; d + 1 < dim; ++d) pi[d] = pin[d];
+        face_map(shape, face, pi, po, nullptr, &sg);
+        for (int d = 0; d < dim; ++d) pout[d] = double(po[d]);
+        *sign = double(sg);
+    })
 }
 
 int orc_orientation_map(int fshape, int idx, const double* pin, double* pout) {
-    ORC_TRY({ orientation_map(fshape, idx, pin, pout); })
+    ORC_TRY({
+        const int dim = shape_dim(fshape);
+        Real pi[3] = {0, 0, 0}, po[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) pi[d] = pin[d];
+        orientation_map(fshape, idx, pi, po);
+        for (int d = 0; d < dim; ++d) pout[d] = double(po[d]);
+    })
 }
 
 int orc_orientation_index(int fshape, int n, const int64_t* l1, const int64_t* l2) {
@@ -1372,7 +1398,8 @@ int orc_orientation_index(int fshape, int n, const int64_t* l1, const int64_t* l
 int orc_structured_mesh(int dim, const int* n, const double* lo, const double* hi, const int* periodic,
                         int simplices, int64_t* n_elem, int* nv, int* shape, double* X, int64_t* nodes) {
     ORC_TRY({
-        MeshOut m = structured_mesh(dim, n, lo, hi, periodic, simplices);
+        std::vector<Real> rlo(lo, lo + dim), rhi(hi, hi + dim);
+        MeshOut m = structured_mesh(dim, n, rlo.data(), rhi.data(), periodic, simplices);
         *nv = m.nv;
         *shape = m.shape;
         *n_elem = int64_t(m.nodes.size() / m.nv);
@@ -1455,42 +1482,65 @@ int orc_set_acoustic(void* h, const double* cinv) {
 int orc_assemble(void* h) { ORC_TRY({ assemble(*static_cast<Problem*>(h)); }) }
 
 int orc_time_derivative(void* h, int n_in, const double* const* vecs, const double* coef, double* out) {
-    ORC_TRY({ time_derivative(*static_cast<Problem*>(h), n_in, vecs, coef, out); })
+    ORC_TRY({
+        Problem* P = static_cast<Problem*>(h);
+        const size_t n = size_t(P->offset[P->nE]);
+        std::vector<std::vector<Real>> in(n_in);
+        std::vector<const Real*> ptrs(n_in);
+        std::vector<Real> c(coef, coef + n_in), o(n);
+        for (int i = 0; i < n_in; ++i) {
+            in[i].assign(vecs[i], vecs[i] + n);
+            ptrs[i] = in[i].data();
+        }
+        time_derivative(*P, n_in, ptrs.data(), c.data(), o.data());
+        std::copy(o.begin(), o.end(), out);
+    })
 }
 
 int orc_rk4(void* h, double* u, double dt, int n_steps) {
     ORC_TRY({
         Problem* P = static_cast<Problem*>(h);
-        for (int s = 0; s < n_steps; ++s) rk4_step(*P, u, dt);
+        std::vector<Real> ur(u, u + P->offset[P->nE]);
+        for (int s = 0; s < n_steps; ++s) rk4_step(*P, ur.data(), Real(dt));
+        std::copy(ur.begin(), ur.end(), u);
     })
 }
 
 int orc_rk_steps(void* h, double* u, double dt, int n_steps, int stages, const double* a, const double* b) {
     ORC_TRY({
         Problem* P = static_cast<Problem*>(h);
-        for (int s = 0; s < n_steps; ++s) rk_step(*P, u, dt, stages, a, b);
+        std::vector<Real> ur(u, u + P->offset[P->nE]);
+        std::vector<Real> ar(a, a + stages * stages), br(b, b + stages);
+        for (int s = 0; s < n_steps; ++s) rk_step(*P, ur.data(), Real(dt), stages, ar.data(), br.data());
+        std::copy(ur.begin(), ur.end(), u);
     })
 }
 
 int orc_project_modes(void* h, int n_modes, const double* amp, const double* k, const double* field_scale,
                       double* u) {
     ORC_TRY({
+        Problem* P = static_cast<Problem*>(h);
+        std::vector<Real> ar(amp, amp + n_modes), kr(k, k + 3 * n_modes), fs(field_scale, field_scale + P->nfields);
+        std::vector<Real> ur(size_t(P->offset[P->nE]));
         Modes m;
         m.n = n_modes;
-        m.amp = amp;
-        m.k = k;
-        project(*static_cast<Problem*>(h), m, field_scale, u);
+        m.amp = ar.data();
+        m.k = kr.data();
+        project(*P, m, fs.data(), ur.data());
+        std::copy(ur.begin(), ur.end(), u);
     })
 }
 
 int orc_l2_error_modes(void* h, const double* u, int n_modes, const double* amp, const double* k, double t,
                        double* err) {
     ORC_TRY({
+        Problem* P = static_cast<Problem*>(h);
+        std::vector<Real> ar(amp, amp + n_modes), kr(k, k + 3 * n_modes), ur(u, u + P->offset[P->nE]);
         Modes m;
         m.n = n_modes;
-        m.amp = amp;
-        m.k = k;
-        *err = l2_error(*static_cast<Problem*>(h), u, m, t);
+        m.amp = ar.data();
+        m.k = kr.data();
+        *err = double(l2_error(*P, ur.data(), m, Real(t)));
     })
 }
 

@@ -12,6 +12,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+LIB_PATH_LD = os.path.join(ROOT, "oracle", "liboracle_ld.so")
 FIXTURE = os.path.join(ROOT, "tests", "golden", "ref_tables.json")
 
 LINE, SQUARE, TRIANGLE, CUBE, TETRAHEDRON = 1, 2, 3, 4, 5
@@ -19,7 +20,7 @@ SHAPE_IDS = {"line": LINE, "square": SQUARE, "triangle": TRIANGLE, "cube": CUBE,
              "tetrahedron": TETRAHEDRON}
 DIM = {LINE: 1, SQUARE: 2, TRIANGLE: 2, CUBE: 3, TETRAHEDRON: 3}
 
-_lib = None
+_libs = {}
 _D = ctypes.POINTER(ctypes.c_double)
 _I64 = ctypes.POINTER(ctypes.c_int64)
 _I32 = ctypes.POINTER(ctypes.c_int)
@@ -29,14 +30,16 @@ def _dp(a):
     return a.ctypes.data_as(_D)
 
 
-def load():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
+def load(extended=False):
+    """The double-precision oracle, or (extended=True) the same algorithm in
+    long double."""
+    if extended in _libs:
+        return _libs[extended]
+    path = LIB_PATH_LD if extended else LIB_PATH
+    if not os.path.exists(path):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True,
                        stdout=subprocess.DEVNULL)
-    lib = ctypes.CDLL(LIB_PATH)
+    lib = ctypes.CDLL(path)
     lib.orc_last_error.restype = ctypes.c_char_p
     lib.orc_problem_create.restype = ctypes.c_void_p
     lib.orc_problem_create.argtypes = [ctypes.c_int, ctypes.c_int64, _D, _I64, _I32, ctypes.c_int]
@@ -55,14 +58,14 @@ def load():
     lib.orc_project_modes.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D]
     lib.orc_l2_error_modes.argtypes = [ctypes.c_void_p, _D, ctypes.c_int, _D, _D, ctypes.c_double, _D]
     lib.orc_mass_matrix.argtypes = [ctypes.c_void_p, ctypes.c_int64, _D]
-    _lib = lib
+    _libs[extended] = lib
     _load_simplex_rules(lib)
     return lib
 
 
-def check(rc):
+def check(rc, lib=None):
     if rc != 0:
-        raise RuntimeError("oracle: " + load().orc_last_error().decode())
+        raise RuntimeError("oracle: " + (lib or load()).orc_last_error().decode())
 
 
 def fixture_tables():
@@ -159,8 +162,8 @@ def structured_mesh(n, lo=None, hi=None, periodic=None, simplices=False):
 class Problem:
     """Mesh + DG space + operator in the oracle, state in reference layout."""
 
-    def __init__(self, shape, X, nodes, orders, n_fields=1):
-        self.lib = load()
+    def __init__(self, shape, X, nodes, orders, n_fields=1, extended=False):
+        self.lib = load(extended)
         self.shape = shape
         self.dim = DIM[shape]
         self.X = np.ascontiguousarray(X, dtype=np.float64)
