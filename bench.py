@@ -155,6 +155,33 @@ def run_gpu(steps, warmup):
     return results, dict(n_dof=n_dof, n_elem=n_elem, n_faces=n_faces, dt=dt)
 
 
+def run_profile_workload(steps):
+    """Kernels only, for counter runs: `steps` FP64 RHS evaluations, `steps`
+    mixed RHS evaluations and `steps` plain copies of one FP64 state vector
+    (2,097,152 doubles = 16 MiB read + 16 MiB written), the copies being the
+    reference for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    from hpgem_amd import HpgemAPIDevice, Mesh, RK4
+
+    mesh = Mesh.structured([N_CELLS] * 3)
+    amp, k = workload_modes()
+    dt = 0.1 / N_CELLS / float(np.linalg.norm(VELOCITY))
+    for prec in ("fp64", "mixed"):
+        api = HpgemAPIDevice(mesh, P, velocity=VELOCITY, precision=prec, tableau=RK4, n_extra_vectors=2)
+        api.setInitialSolutionModes(amp, k)
+        if prec == "mixed":
+            for v in (1, 2):
+                api.setStorage(v, "mixed")
+        api.computeTimeDerivative([0], [1.0], 1)
+        for _ in range(steps):
+            api.computeTimeDerivative([0, 1], [1.0, 0.5 * dt], 2)
+        if prec == "fp64":
+            for _ in range(steps):
+                api.copyVector(5, 0)
+        api.synchronize()
+        api.close()
+    print(json.dumps({"profile_workload": "done", "steps": steps}))
+
+
 def run_cpu_baseline(target_seconds=15.0):
     """FP64 oracle (HpgemAPILinear restatement: dense element/face mat-vecs and
     a per-element LU mass solve per call) on a bounded sample: the same
@@ -193,6 +220,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--profile-workload", action="store_true",
+                    help="launch only the RHS and copy kernels (for rocprofv3 counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -200,6 +229,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.profile_workload:
+        run_profile_workload(args.steps)
+        return
     gpu, info = run_gpu(args.steps, args.warmup)
     cpu = None if args.no_cpu else run_cpu_baseline(args.cpu_seconds)
 

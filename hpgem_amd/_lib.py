@@ -49,6 +49,7 @@ SIGNATURES = [
                                      ctypes.c_double, ctypes.c_int]),
     ("hgdg_project_modes", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, _D]),
     ("hgdg_state_diff", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, _D]),
+    ("hgdg_copy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ("hgdg_sync", ctypes.c_int, [ctypes.c_void_p]),
     ("hgdg_get_stream", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     ("hgdg_basis_size", ctypes.c_int, [ctypes.c_int, ctypes.c_int]),

@@ -154,6 +154,9 @@ class HpgemAPIDevice:
         n = int(round((end_time - start_time) / dt))
         return self.computeOneTimeStep(start_time, dt, n)
 
+    def copyVector(self, dst, src):
+        check(self.lib.hgdg_copy(self.h, dst, src))
+
     def synchronize(self):
         check(self.lib.hgdg_sync(self.h))
 

@@ -105,6 +105,7 @@ struct Tensor {
 template <typename T>
 struct Acc {
     T s;
+    __device__ Acc() : s(T(0)) {}
     __device__ explicit Acc(T v) : s(v) {}
     __device__ void add(T v) { s += v; }
     __device__ T value() const { return s; }
@@ -112,6 +113,7 @@ struct Acc {
 template <>
 struct Acc<float> {
     float s, c;
+    __device__ Acc() : s(0.f), c(0.f) {}
     __device__ explicit Acc(float v) : s(v), c(0.f) {}
     __device__ void add(float v) {
         float t = s + v;
@@ -122,70 +124,189 @@ struct Acc<float> {
     __device__ float value() const { return s + c; }
 };
 
+// One thread owns one x-pencil of an element (the N1 coefficients with fixed
+// y/z indices): the x-contraction and the x-face terms stay in registers, the
+// y/z contractions read the other pencils of the element from LDS, and the
+// y/z inflow traces arrive as whole pencils of the neighbour (vector loads).
+template <int DIM, int N1>
+struct Pencil {
+    static constexpr int NP = DIM == 3 ? N1 * N1 : N1;  // pencils per element
+    static constexpr int NB = NP * N1;                   // coefficients per element
+    static constexpr int EPB = kBlock / NP;              // elements per block
+};
+
+// x[a] = vector i at [base, base + N1), as double
+template <int N1>
+__device__ __forceinline__ void load_pencil(const RhsParams& p, int i, int64_t base, double (&x)[N1]) {
+    if ((p.in_f32 >> i) & 1u) {
+        const float* f = static_cast<const float*>(p.in[i]) + base;
+        if constexpr (N1 % 4 == 0) {
+#pragma unroll
+            for (int a = 0; a < N1; a += 4) {
+                const float4 v = *reinterpret_cast<const float4*>(f + a);
+                x[a] = v.x; x[a + 1] = v.y; x[a + 2] = v.z; x[a + 3] = v.w;
+            }
+        } else if constexpr (N1 % 2 == 0) {
+#pragma unroll
+            for (int a = 0; a < N1; a += 2) {
+                const float2 v = *reinterpret_cast<const float2*>(f + a);
+                x[a] = v.x; x[a + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < N1; ++a) x[a] = f[a];
+        }
+    } else {
+        const double* d = static_cast<const double*>(p.in[i]) + base;
+        if constexpr (N1 % 2 == 0) {
+#pragma unroll
+            for (int a = 0; a < N1; a += 2) {
+                const double2 v = *reinterpret_cast<const double2*>(d + a);
+                x[a] = v.x; x[a + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < N1; ++a) x[a] = d[a];
+        }
+    }
+}
+
+// Stage input pencil sum_i coef_i vec_i (getLinearCombinationOfVectors).
+template <int N1>
+__device__ __forceinline__ void combine_pencil(const RhsParams& p, int64_t base, double (&w)[N1]) {
+    double x[N1];
+    load_pencil<N1>(p, 0, base, x);
+#pragma unroll
+    for (int a = 0; a < N1; ++a) w[a] = p.coef[0] * x[a];
+#pragma unroll
+    for (int i = 1; i < kMaxIn; ++i) {
+        if (i >= p.n_in) break;
+        load_pencil<N1>(p, i, base, x);
+#pragma unroll
+        for (int a = 0; a < N1; ++a) w[a] += p.coef[i] * x[a];
+    }
+}
+
+__device__ __forceinline__ double face_jump(uint32_t m, double own, double other) {
+    if (m == MODE_NONE) return -own;
+    if (m == MODE_HALF_OWN) return -0.5 * own;
+    return (m == MODE_NBR) ? other - own : 0.5 * (other - own);
+}
+
 template <int DIM, int N1, typename T>
 __global__ __launch_bounds__(kBlock) void tensor_advection_rhs(const RhsParams p) {
-    using S = Tensor<DIM, N1>;
-    __shared__ double s_w[S::EPB * S::NB];  // FP64 stage input
+    using S = Pencil<DIM, N1>;
+    __shared__ double s_w[S::EPB * S::NB];  // FP64 stage input of the block's elements
     __shared__ T s_D[N1 * N1];
     __shared__ T s_Mi[2 * N1];  // s_Mi[s * N1 + a] = Mi(a, s)
     const int tid = threadIdx.x;
     if (tid < N1 * N1) s_D[tid] = T(p.Dr[tid]);
     if (tid < 2 * N1) s_Mi[tid] = T(p.Mi[(tid % N1) * N1 + tid / N1]);
-    const int le = tid / S::L;
-    const int dof = tid % S::L;
+    const int le = tid / S::NP;
+    const int pc = tid % S::NP;
     const int64_t e = int64_t(blockIdx.x) * S::EPB + le;
-    const bool active = (e < p.n_elem) && (dof < S::NB);
-    if (active) s_w[le * S::NB + dof] = combine(p, e * S::NB + dof);
+    const bool active = (le < S::EPB) && (e < p.n_elem);
+    const int ib = pc % N1;                    // y index of this pencil
+    const int ic = DIM == 3 ? pc / N1 : 0;     // z index of this pencil
+    double* sw = s_w + le * S::NB;
+    double w[N1];
+    if (active) {
+        combine_pencil<N1>(p, e * S::NB + pc * N1, w);
+#pragma unroll
+        for (int a = 0; a < N1; ++a) sw[pc * N1 + a] = w[a];
+    }
     __syncthreads();
     if (!active) return;
 
-    const int idx[3] = {dof % N1, (dof / N1) % N1, DIM == 3 ? dof / (N1 * N1) : 0};
-    constexpr int stride[3] = {1, N1, N1 * N1};
-    const double* w = s_w + le * S::NB;
     T ah[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) ah[d] = T(p.ahat[d * p.n_elem + e]);
 
     // volume (strong form): -sum_d ahat_d (Dr along d) w
-    T vol = T(0);
+    T vol[N1];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        const int row = dof - idx[d] * stride[d];
+    for (int a = 0; a < N1; ++a) {
         T s = T(0);
 #pragma unroll
-        for (int j = 0; j < N1; ++j) s += s_D[idx[d] * N1 + j] * T(w[row + j * stride[d]]);
-        vol -= ah[d] * s;
+        for (int j = 0; j < N1; ++j) s += s_D[a * N1 + j] * T(w[j]);
+        vol[a] = -ah[0] * s;
     }
-    Acc<T> acc(vol);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+        const T cy = -ah[1] * s_D[ib * N1 + j];
+        const double* row = sw + (j + N1 * ic) * N1;  // pencil (j, ic)
+#pragma unroll
+        for (int a = 0; a < N1; ++a) vol[a] += cy * T(row[a]);
+    }
+    if constexpr (DIM == 3) {
+#pragma unroll
+        for (int j = 0; j < N1; ++j) {
+            const T cz = -ah[2] * s_D[ic * N1 + j];
+            const double* row = sw + (ib + N1 * j) * N1;  // pencil (ib, j)
+#pragma unroll
+            for (int a = 0; a < N1; ++a) vol[a] += cz * T(row[a]);
+        }
+    }
+    Acc<T> acc[N1] = {};
+#pragma unroll
+    for (int a = 0; a < N1; ++a) acc[a] = Acc<T>(vol[a]);
 
     // faces: -(2s-1) ahat_d Mi(idx_d, s) (T_{d,s} - own_{d,s}); jumps in FP64
     const uint32_t md = p.mode[e];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        const int row = dof - idx[d] * stride[d];
+    for (int s = 0; s < 2; ++s) {  // x faces: one trace value per pencil
+        const uint32_t m = (md >> (3 * s)) & 7u;
+        if (m == MODE_OWN) continue;
+        double other = 0.0;
+        if (m == MODE_NBR || m == MODE_AVG) {
+            const int64_t nb = p.nbr[s * p.n_elem + e];
+            double v[1];
+            const int64_t idx = nb * S::NB + pc * N1 + (1 - s);
+            v[0] = p.coef[0] * load_in(p, 0, idx);
+            for (int i = 1; i < kMaxIn; ++i) {
+                if (i >= p.n_in) break;
+                v[0] += p.coef[i] * load_in(p, i, idx);
+            }
+            other = v[0];
+        }
+        const T jump = T(face_jump(m, w[s], other));
+        const T sc = (s ? T(-1) : T(1)) * ah[0] * jump;
+#pragma unroll
+        for (int a = 0; a < N1; ++a) acc[a].add(sc * s_Mi[s * N1 + a]);
+    }
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) {  // y (and z) faces: a whole pencil of traces
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const uint32_t m = (md >> (3 * (2 * d + s))) & 7u;
+            const int f = 2 * d + s;
+            const uint32_t m = (md >> (3 * f)) & 7u;
             if (m == MODE_OWN) continue;
-            const double own = w[row + s * stride[d]];
-            double jump;
-            if (m == MODE_NONE) {
-                jump = -own;
-            } else if (m == MODE_HALF_OWN) {
-                jump = -0.5 * own;
+            const int own_pc = d == 1 ? s + N1 * ic : ib + N1 * s;
+            const int nbr_pc = d == 1 ? (1 - s) + N1 * ic : ib + N1 * (1 - s);
+            double other[N1];
+            if (m == MODE_NBR || m == MODE_AVG) {
+                const int64_t nb = p.nbr[f * p.n_elem + e];
+                combine_pencil<N1>(p, nb * S::NB + nbr_pc * N1, other);
             } else {
-                const int64_t nb = p.nbr[(2 * d + s) * p.n_elem + e];
-                const double other = combine(p, nb * S::NB + row + (1 - s) * stride[d]);
-                jump = (m == MODE_NBR) ? other - own : 0.5 * (other - own);
+#pragma unroll
+                for (int a = 0; a < N1; ++a) other[a] = 0.0;
             }
-            const T sgn = s ? T(-1) : T(1);
-            acc.add(sgn * ah[d] * s_Mi[s * N1 + idx[d]] * T(jump));
+            const int my = d == 1 ? ib : ic;
+            const T sc = (s ? T(-1) : T(1)) * ah[d] * s_Mi[s * N1 + my];
+#pragma unroll
+            for (int a = 0; a < N1; ++a) acc[a].add(sc * T(face_jump(m, sw[own_pc * N1 + a], other[a])));
         }
     }
-    const T r = acc.value();
-    const int64_t o = e * S::NB + dof;
-    if (p.out_f32) static_cast<float*>(p.out)[o] = float(r);
-    else static_cast<double*>(p.out)[o] = double(r);
+    const int64_t o = e * S::NB + pc * N1;
+    if (p.out_f32) {
+        float* out = static_cast<float*>(p.out) + o;
+#pragma unroll
+        for (int a = 0; a < N1; ++a) out[a] = float(acc[a].value());
+    } else {
+        double* out = static_cast<double*>(p.out) + o;
+#pragma unroll
+        for (int a = 0; a < N1; ++a) out[a] = double(acc[a].value());
+    }
 }
 
 // u[i] += sum_j c_j k_j[i], terms applied one after another in FP64
@@ -211,6 +332,13 @@ __global__ __launch_bounds__(kBlock) void multi_axpy(const AxpyParams p) {
         y += p.c[j] * x;
     }
     p.y[i] = y;
+}
+
+// dst = src, 16 bytes per thread (plain streaming copy)
+__global__ __launch_bounds__(kBlock) void copy_words(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                     int64_t n16) {
+    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n16) dst[i] = src[i];
 }
 
 // Per-element mass-norm of x - y: |J| dx^T (M1 x M1 x M1) dx, and max |dx|;
@@ -660,7 +788,7 @@ void launch_rhs(hgdg_ctx* c, const int* in_ids, const double* coeffs, int n_in, 
     const bool mixed = c->op.precision == HGDG_PREC_MIXED_FP32;
     dispatch_tensor(c->dim, c->n1, [&](auto D, auto N) {
         constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
-        using S = Tensor<DIMc, N1c>;
+        using S = Pencil<DIMc, N1c>;
         const int64_t grid = (c->n_elem + S::EPB - 1) / S::EPB;
         if (mixed)
             hipLaunchKernelGGL((tensor_advection_rhs<DIMc, N1c, float>), dim3(unsigned(grid)), dim3(kBlock), 0,
@@ -1047,6 +1175,30 @@ int hgdg_project_modes(hgdg_ctx* ctx, int vec, int n_modes, const double* amp, c
         require(n_modes >= 0, HGDG_ERR_ARG, "negative mode count");
         HGDG_HIP(hipSetDevice(ctx->device));
         hgdg::project_modes(ctx, vec, n_modes, amp, k);
+    });
+}
+
+int hgdg_copy(hgdg_ctx* ctx, int dst, int src) {
+    return run([&] {
+        require(ctx != nullptr, HGDG_ERR_ARG, "ctx is NULL");
+        require(dst != src, HGDG_ERR_ARG, "copy onto itself");
+        HGDG_HIP(hipSetDevice(ctx->device));
+        hgdg::Vec& d = hgdg::vec_at(ctx, dst);
+        hgdg::Vec& sv = hgdg::vec_at(ctx, src);
+        require(d.is_f32 == sv.is_f32, HGDG_ERR_ARG, "copy between vectors of different storage");
+        const int64_t bytes = ctx->n_dof() * (sv.is_f32 ? 4 : 8);
+        const int64_t n16 = bytes / 16;
+        const int64_t tail = bytes - n16 * 16;
+        if (n16 > 0) {
+            const int64_t grid = (n16 + hgdg::kBlock - 1) / hgdg::kBlock;
+            hipLaunchKernelGGL(hgdg::copy_words, dim3(unsigned(grid)), dim3(hgdg::kBlock), 0, ctx->stream,
+                               static_cast<const uint4*>(sv.ptr()), static_cast<uint4*>(d.ptr()), n16);
+            HGDG_HIP(hipGetLastError());
+        }
+        if (tail > 0)
+            HGDG_HIP(hipMemcpyAsync(static_cast<char*>(d.ptr()) + n16 * 16,
+                                    static_cast<const char*>(sv.ptr()) + n16 * 16, size_t(tail),
+                                    hipMemcpyDeviceToDevice, ctx->stream));
     });
 }
 

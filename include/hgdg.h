@@ -153,6 +153,11 @@ int hgdg_project_modes(hgdg_ctx* ctx, int vec, int n_modes, const double* amp, c
  * abs coefficient difference (FP64, on the device). */
 int hgdg_state_diff(hgdg_ctx* ctx, int x, int y, double* l2, double* max_abs);
 
+/* vec[dst] = vec[src] (same storage), on the device.
+ * Replaces HpgemAPIBase::copyTimeIntegrationToTimeLevelData-style vector copies
+ * (kernel/Base/HpgemAPIBase_Impl.h:158-176). */
+int hgdg_copy(hgdg_ctx* ctx, int dst, int src);
+
 /* Block until all queued work on the context's stream is done. */
 int hgdg_sync(hgdg_ctx* ctx);
 
