@@ -1,0 +1,148 @@
+// Shared pieces of the hgdg library: error type, HIP checks, stage-input
+// loads, compensated accumulation, device buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hgdg.h"
+
+namespace hgdg {
+
+extern thread_local std::string g_error;
+
+struct Failure : std::runtime_error {
+    int code;
+    Failure(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HGDG_HIP(call)                                                                        \
+    do {                                                                                      \
+        hipError_t err_ = (call);                                                             \
+        if (err_ != hipSuccess)                                                               \
+            throw ::hgdg::Failure(HGDG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(err_)); \
+    } while (0)
+
+constexpr int kMaxIn = 8;
+constexpr int kBlock = 256;
+
+
+
+// Inputs and output of one stage evaluation: out = f(sum_i coef_i in_i).
+struct StageIO {
+    const void* in[kMaxIn];
+    double coef[kMaxIn];
+    uint32_t in_f32;  // bit i set: input i is stored in FP32
+    int32_t n_in;
+    void* out;
+    int32_t out_f32;
+};
+
+__device__ __forceinline__ double load_in(const StageIO& p, int i, int64_t idx) {
+    return ((p.in_f32 >> i) & 1u) ? double(static_cast<const float*>(p.in[i])[idx])
+                                  : static_cast<const double*>(p.in[i])[idx];
+}
+
+// Stage input sum_i coef_i vec_i, first term scaled then accumulated in
+// order (HpgemAPISimplified::getLinearCombinationOfVectors).
+__device__ __forceinline__ double combine(const StageIO& p, int64_t idx) {
+    double s = p.coef[0] * load_in(p, 0, idx);
+#pragma unroll
+    for (int i = 1; i < kMaxIn; ++i)
+        if (i < p.n_in) s += p.coef[i] * load_in(p, i, idx);
+    return s;
+}
+
+// Residual accumulation: FP64 plainly; FP32 with a compensated (TwoSum)
+// running sum for the face contributions.
+template <typename T>
+struct Acc {
+    T s;
+    __device__ Acc() : s(T(0)) {}
+    __device__ explicit Acc(T v) : s(v) {}
+    __device__ void add(T v) { s += v; }
+    __device__ T value() const { return s; }
+};
+template <>
+struct Acc<float> {
+    float s, c;
+    __device__ Acc() : s(0.f), c(0.f) {}
+    __device__ explicit Acc(float v) : s(v), c(0.f) {}
+    __device__ void add(float v) {
+        float t = s + v;
+        float bp = t - s;
+        c += (s - (t - bp)) + (v - bp);
+        s = t;
+    }
+    __device__ float value() const { return s + c; }
+};
+
+void require(bool ok, int code, const std::string& msg);
+
+template <typename T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), n(o.n) {
+        o.ptr = nullptr;
+        o.n = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            reset();
+            ptr = o.ptr;
+            n = o.n;
+            o.ptr = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        reset();
+        if (count == 0) return;
+        hipError_t err = hipMalloc(&ptr, count * sizeof(T));
+        if (err != hipSuccess) {
+            ptr = nullptr;
+            throw Failure(HGDG_ERR_MEMORY, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                               " bytes failed: " + hipGetErrorString(err));
+        }
+        n = count;
+    }
+    void upload(const std::vector<T>& h, hipStream_t s) {
+        alloc(h.size());
+        if (!h.empty()) HGDG_HIP(hipMemcpyAsync(ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+struct Vec {
+    std::unique_ptr<DevBuf<double>> f64;
+    std::unique_ptr<DevBuf<float>> f32;
+    bool is_f32 = false;
+    void* ptr() const { return is_f32 ? static_cast<void*>(f32->ptr) : static_cast<void*>(f64->ptr); }
+};
+
+struct GraphKey {
+    int solution;
+    std::vector<int> aux;
+    std::vector<double> a, b;
+    double dt;
+    bool operator==(const GraphKey& o) const {
+        return solution == o.solution && aux == o.aux && a == o.a && b == o.b && dt == o.dt;
+    }
+};
+
+}  // namespace hgdg

@@ -1,0 +1,104 @@
+// The library context (hgdg_ctx) and the functions the element families
+// implement: tensor elements (squares, cubes) in tensor.hip, tetrahedra in
+// simplex.hip, shared state/time stepping and the C ABI in hgdg.hip.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+#include "host_tables.hpp"
+#include "simplex_tables.hpp"
+
+namespace hgdg {
+
+
+// Squares and cubes of one order p; state in tensor order per element.
+struct TensorMesh {
+    int p = 0, n1 = 0, nb = 0;
+    std::vector<int> perm;          // hpGEM basis index -> tensor index
+    std::vector<double> Jinv;       // [e][dim][dim]
+    std::vector<double> X0, E;      // [e][dim] first vertex, [e][dim][dim] edge vectors
+    std::vector<double> absdet;     // [e]
+    std::vector<int32_t> nbr;       // [2 dim][e]
+    TensorOps ops;
+    DevBuf<int32_t> d_nbr;
+    DevBuf<double> d_absdet, d_ops;  // d_ops: Dr, Mi, M1 (n1 x n1 each)
+    DevBuf<double> d_geo;            // advection: ahat [dim][e]; acoustic: J^{-1} [dim*dim][e], then c_inv [e]
+    DevBuf<uint32_t> d_mode;         // advection: upwind mode, 3 bits per face
+};
+
+// Tetrahedra with per-element order 1..4, grouped by order on the device.
+struct SimplexMesh {
+    int64_t group_start[5] = {0, 0, 0, 0, 0};    // device elements of order g+1: [start[g], start[g+1])
+    int64_t group_dof[5] = {0, 0, 0, 0, 0};      // first DOF of each group
+    std::vector<int64_t> dev_of;                 // mesh element -> device element
+    std::vector<int64_t> mesh_of;                // device element -> mesh element
+    std::vector<int32_t> order;                  // per device element
+    std::vector<int64_t> dof_off;                // per device element: first DOF
+    std::vector<SimplexOps> ops;                 // per order (index p-1), built on demand
+    DevBuf<double> d_tables;                     // Dr, proj, lift of every order
+    DevBuf<int32_t> d_tab_index;                 // offsets and column lists into d_tables
+    DevBuf<double> d_geo;                        // J^{-1} [9][E], then c_inv [E], sigma [4][E], normal [3][4][E]
+    DevBuf<int32_t> d_face;                      // [4][E]: neighbour device element or -1, then packed info [4][E]
+    DevBuf<double> d_absdet;                     // [E]
+    std::vector<double> absdet;
+    std::vector<double> jinv;                    // [E][9]
+    std::vector<double> X;                       // mesh order [E][4][3]
+    std::vector<int32_t> nbr_dev;                // [4][E]
+    std::vector<int32_t> info;                   // [4][E]
+    std::vector<double> sigma, normal;           // [4][E], [3][4][E]
+};
+
+}  // namespace hgdg
+
+struct hgdg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool has_mesh = false;
+    bool has_op = false;
+    int shape = 0, dim = 0, n_fields = 1;
+    int64_t n_elem = 0, n_faces = 0, n_internal = 0, dof = 0;
+    hgdg_op_desc op{};
+    std::vector<double> cinv;  // mesh order
+    hgdg::TensorMesh tm;
+    hgdg::SimplexMesh sm;
+    std::vector<hgdg::Vec> vecs;
+    hgdg::DevBuf<double> d_partial;
+    bool graph_valid = false;
+    hgdg::GraphKey graph_key{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+
+    bool simplex() const { return shape == HGDG_SHAPE_TETRAHEDRON; }
+    int64_t n_dof() const { return dof; }
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_valid = false;
+    }
+};
+
+namespace hgdg {
+
+Vec& vec_at(hgdg_ctx* c, int id);
+
+// tensor.hip
+void tensor_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m);
+void tensor_operator_set(hgdg_ctx* c, const hgdg_op_desc* op);
+void tensor_rhs(hgdg_ctx* c, const StageIO& io);
+void tensor_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* l2, double* max_abs);
+void tensor_project_modes(hgdg_ctx* c, int n_modes, const double* amp, const double* k, std::vector<double>& dev);
+void tensor_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev);
+void tensor_from_device(hgdg_ctx* c, const std::vector<double>& dev, double* host);
+
+// simplex.hip
+void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m);
+void simplex_operator_set(hgdg_ctx* c, const hgdg_op_desc* op);
+void simplex_rhs(hgdg_ctx* c, const StageIO& io);
+void simplex_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* l2, double* max_abs);
+void simplex_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev);
+void simplex_from_device(hgdg_ctx* c, const std::vector<double>& dev, double* host);
+
+}  // namespace hgdg

@@ -1,0 +1,601 @@
+// Tetrahedral elements with per-element order p = 1..4 (hp meshes): the
+// first-order acoustic wave system of hpGEM's AcousticWaveLinear
+// (applications/ExampleMultipleVariableProblem/AcousticWaveLinear_Impl.h:
+// 102-283): fields v and s_1..s_3, mass blocks M (v) and c_inv M (s),
+//   r_v   = int phi grad.s  - 1/2 int_f phi n_L.(s_L - s_R)   (internal faces)
+//   r_s_d = int phi d_d v   - 1/2 int_f phi n_L,d (v_L - v_R)
+// and on reflecting walls r_v gets - int_f phi n.s and r_s nothing.
+//
+// On an affine tetrahedron the inverse-mass apply turns the volume terms into
+// exact reference derivative matrices Dr_e (d/dxi_e in P_p) combined with
+// J^{-1}, and each face term into a lifting of the flux's face-polynomial
+// coefficients: face traces are projected (exactly: everything is a
+// polynomial of degree <= 4) onto an orthonormal basis of P_4 on the reference
+// triangle, the flux is formed coefficient by coefficient, and
+// Lift = M_ref^{-1} P^T scaled by area/volume brings it back. Both sides of a
+// face are evaluated in the left element's face coordinates, the right side
+// through hpGEM's orientation map (FaceGeometry.h:372-393). Elements are
+// grouped by order on the device; one launch covers all groups, each block
+// working on elements of one order.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <string>
+#include <unordered_map>
+
+#include "context.hpp"
+
+namespace hgdg {
+
+namespace {
+
+// int32 table index per order (p-1): Dr, Mref, then proj and lift offsets for
+// (face, orientation), then (ncol, cols offset) per face; columns follow.
+constexpr int kTixDr = 0;
+constexpr int kTixM = 1;
+constexpr int kTixProj = 2;           // + f * 6 + o
+constexpr int kTixLift = 2 + 24;      // + f * 6 + o
+constexpr int kTixNcol = 2 + 48;      // + f
+constexpr int kTixCols = 2 + 52;      // + f
+constexpr int kTixStride = 2 + 56;
+
+// packed per-face info
+__host__ __device__ constexpr int info_lf(int v) { return v & 3; }
+__host__ __device__ constexpr int info_oown(int v) { return (v >> 2) & 7; }
+__host__ __device__ constexpr int info_onbr(int v) { return (v >> 5) & 7; }
+__host__ __device__ constexpr int info_pn(int v) { return (v >> 8) & 7; }
+constexpr int info_pack(int lf, int oown, int onbr, int pn) { return lf | (oown << 2) | (onbr << 5) | (pn << 8); }
+
+}  // namespace
+
+struct SimplexParams {
+    StageIO io;
+    int64_t n_elem;
+    int64_t group_start[5];
+    int64_t group_dof[5];
+    int32_t group_block[5];
+    const double* jinv;    // [9][E]: (J^{-1})_{e,d} at [(3e + d) E + elem]
+    const double* cinv;    // [E]
+    const double* sigma;   // [4][E]: face area ratio / |det J|
+    const double* normal;  // [3][4][E]: outward unit normal
+    const int32_t* nbr;    // [4][E]: neighbour device element, -1 on walls
+    const int32_t* info;   // [4][E]
+    const double* tab;
+    const int32_t* tix;
+};
+
+__host__ __device__ constexpr int simplex_nb(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
+__host__ __device__ constexpr int simplex_tpe(int p) { return simplex_nb(p) > 16 ? simplex_nb(p) : 16; }
+__device__ constexpr int tri_dim(int p) { return (p + 1) * (p + 2) / 2; }
+
+template <int P>
+struct SimplexShape {
+    static constexpr int NB = simplex_nb(P);
+    static constexpr int TPE = simplex_tpe(P);  // threads per element (>= face basis size)
+    static constexpr int EPB = kBlock / TPE;
+    // LDS carve-up (doubles)
+    static constexpr int U = 0;                         // [EPB][4][NB] stage input
+    static constexpr int N = U + EPB * 4 * NB;          // [EPB][4][kFaceBasis] neighbour trace columns
+    static constexpr int G = N + EPB * 4 * kFaceBasis;  // [EPB][3][NB] J^{-T}-rotated s
+    static constexpr int J = G + EPB * 3 * NB;          // [EPB][2][kFaceBasis] face flux coefficients
+    static constexpr int TOTAL = J + EPB * 2 * kFaceBasis;
+};
+
+constexpr int lds_doubles() {
+    int m = 0;
+    m = SimplexShape<1>::TOTAL > m ? SimplexShape<1>::TOTAL : m;
+    m = SimplexShape<2>::TOTAL > m ? SimplexShape<2>::TOTAL : m;
+    m = SimplexShape<3>::TOTAL > m ? SimplexShape<3>::TOTAL : m;
+    m = SimplexShape<4>::TOTAL > m ? SimplexShape<4>::TOTAL : m;
+    return m;
+}
+
+template <int P>
+__device__ void simplex_acoustic_block(const SimplexParams& p, int blk, double* lds) {
+    using S = SimplexShape<P>;
+    constexpr int NB = S::NB;
+    const int tid = threadIdx.x;
+    const int le = tid / S::TPE;
+    const int i = tid % S::TPE;
+    const int64_t e = p.group_start[P - 1] + int64_t(blk) * S::EPB + le;
+    const bool elem_ok = (le < S::EPB) && (e < p.group_start[P]);
+    const bool row_ok = elem_ok && (i < NB);
+    const int64_t E = p.n_elem;
+    const int64_t base = p.group_dof[P - 1] + (e - p.group_start[P - 1]) * 4 * NB;
+    double* su = lds + S::U + le * 4 * NB;
+    double* sn = lds + S::N + le * 4 * kFaceBasis;
+    double* sg = lds + S::G + le * 3 * NB;
+    double* sJ = lds + S::J + le * 2 * kFaceBasis;
+    const int32_t* tx = p.tix + (P - 1) * kTixStride;
+
+    // stage input of the element, all four fields
+    if (row_ok)
+        for (int f = 0; f < 4; ++f) su[f * NB + i] = combine(p.io, base + f * NB + i);
+    __syncthreads();
+
+    double Ji[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double cinv = 1.0;
+    if (elem_ok) {
+        for (int q = 0; q < 9; ++q) Ji[q] = p.jinv[q * E + e];
+        cinv = p.cinv[e];
+    }
+    // g_e = sum_d (J^{-1})_{e,d} s_d: the reference-direction flux of s
+    if (row_ok)
+        for (int r = 0; r < 3; ++r)
+            sg[r * NB + i] = Ji[r * 3 + 0] * su[1 * NB + i] + Ji[r * 3 + 1] * su[2 * NB + i] + Ji[r * 3 + 2] * su[3 * NB + i];
+    __syncthreads();
+
+    // volume: k_v = sum_e Dr_e g_e ; k_s_d = (1/c_inv) sum_e (J^{-1})_{e,d} Dr_e v
+    double kv = 0, ks[3] = {0, 0, 0};
+    if (row_ok) {
+        const double* Dr = p.tab + tx[kTixDr];
+        double t[3];
+        for (int r = 0; r < 3; ++r) {
+            const double* row = Dr + (r * NB + i) * NB;
+            double a = 0, b = 0;
+            for (int j = 0; j < NB; ++j) {
+                a += row[j] * sg[r * NB + j];
+                b += row[j] * su[j];
+            }
+            kv += a;
+            t[r] = b;
+        }
+        for (int d = 0; d < 3; ++d) ks[d] = (Ji[0 * 3 + d] * t[0] + Ji[1 * 3 + d] * t[1] + Ji[2 * 3 + d] * t[2]) / cinv;
+    }
+
+    // faces
+    for (int f = 0; f < 4; ++f) {
+        int64_t nb = -1;
+        int inf = 0;
+        if (elem_ok) {
+            nb = p.nbr[f * E + e];
+            inf = p.info[f * E + e];
+        }
+        const int pn = info_pn(inf);
+        const int lfn = info_lf(inf);
+        // (a) neighbour trace columns into LDS
+        if (elem_ok && nb >= 0) {
+            const int32_t* txn = p.tix + (pn - 1) * kTixStride;
+            const int ncn = txn[kTixNcol + lfn];
+            const int32_t* colsn = p.tix + txn[kTixCols + lfn];
+            const int nbn = simplex_nb(pn);
+            const int64_t nbase = p.group_dof[pn - 1] + (nb - p.group_start[pn - 1]) * 4 * nbn;
+            for (int it = i; it < 4 * ncn; it += S::TPE) {
+                const int fld = it / ncn, col = it % ncn;
+                sn[fld * kFaceBasis + col] = combine(p.io, nbase + fld * nbn + colsn[col]);
+            }
+        }
+        __syncthreads();
+        // (b) face-basis coefficients of both traces and of the flux
+        const int nF = tri_dim(nb >= 0 && pn > P ? pn : P);
+        double n[3] = {0, 0, 0};
+        double sig = 0;
+        if (elem_ok) {
+            for (int d = 0; d < 3; ++d) n[d] = p.normal[(d * 4 + f) * E + e];
+            sig = p.sigma[f * E + e];
+        }
+        if (elem_ok && i < kFaceBasis) {
+            double Jv = 0, Js = 0;
+            if (i < nF) {
+                const int nco = tx[kTixNcol + f];
+                const int32_t* colso = p.tix + tx[kTixCols + f];
+                const double* Po = p.tab + tx[kTixProj + f * 6 + info_oown(inf)] + i * nco;
+                double ov = 0, os[3] = {0, 0, 0};
+                for (int c = 0; c < nco; ++c) {
+                    const int col = colso[c];
+                    ov += Po[c] * su[col];
+                    for (int d = 0; d < 3; ++d) os[d] += Po[c] * su[(1 + d) * NB + col];
+                }
+                if (nb >= 0) {
+                    const int32_t* txn = p.tix + (pn - 1) * kTixStride;
+                    const int ncn = txn[kTixNcol + lfn];
+                    const double* Pn = p.tab + txn[kTixProj + lfn * 6 + info_onbr(inf)] + i * ncn;
+                    double nv = 0, ns[3] = {0, 0, 0};
+                    for (int c = 0; c < ncn; ++c) {
+                        nv += Pn[c] * sn[c];
+                        for (int d = 0; d < 3; ++d) ns[d] += Pn[c] * sn[(1 + d) * kFaceBasis + c];
+                    }
+                    Jv = -0.5 * (n[0] * (os[0] - ns[0]) + n[1] * (os[1] - ns[1]) + n[2] * (os[2] - ns[2]));
+                    Js = -0.5 * (ov - nv);
+                } else {
+                    Jv = -(n[0] * os[0] + n[1] * os[1] + n[2] * os[2]);
+                    Js = 0;
+                }
+            }
+            sJ[i] = Jv;
+            sJ[kFaceBasis + i] = Js;
+        }
+        __syncthreads();
+        // (c) lift back: k += sigma M_ref^{-1} P^T J
+        if (row_ok) {
+            const double* L = p.tab + tx[kTixLift + f * 6 + info_oown(inf)] + i * kFaceBasis;
+            double lv = 0, ls = 0;
+            for (int m = 0; m < nF; ++m) {
+                lv += L[m] * sJ[m];
+                ls += L[m] * sJ[kFaceBasis + m];
+            }
+            kv += sig * lv;
+            const double c = sig * ls / cinv;
+            for (int d = 0; d < 3; ++d) ks[d] += c * n[d];
+        }
+        __syncthreads();
+    }
+
+    if (row_ok) {
+        const double vals[4] = {kv, ks[0], ks[1], ks[2]};
+        for (int f = 0; f < 4; ++f) {
+            const int64_t o = base + f * NB + i;
+            if (p.io.out_f32) static_cast<float*>(p.io.out)[o] = float(vals[f]);
+            else static_cast<double*>(p.io.out)[o] = vals[f];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void simplex_acoustic_rhs(const SimplexParams p) {
+    __shared__ double lds[lds_doubles()];
+    const int b = blockIdx.x;
+    if (b < p.group_block[1]) simplex_acoustic_block<1>(p, b - p.group_block[0], lds);
+    else if (b < p.group_block[2]) simplex_acoustic_block<2>(p, b - p.group_block[1], lds);
+    else if (b < p.group_block[3]) simplex_acoustic_block<3>(p, b - p.group_block[2], lds);
+    else simplex_acoustic_block<4>(p, b - p.group_block[3], lds);
+}
+
+// Per-element |det J| sum_f dx_f^T M_ref dx_f (plain L2 of all fields) and
+// max |dx|, one partial per block.
+struct SimplexDiffParams {
+    int64_t n_elem;
+    int64_t group_start[5];
+    int64_t group_dof[5];
+    int32_t group_block[5];
+    const void* x;
+    const void* y;
+    int32_t x_f32, y_f32;
+    const double* absdet;
+    const double* tab;
+    const int32_t* tix;
+    double* partial;
+};
+
+template <int P>
+__device__ void simplex_diff_block(const SimplexDiffParams& p, int blk, double* sd, double* red) {
+    using S = SimplexShape<P>;
+    constexpr int NB = S::NB;
+    const int tid = threadIdx.x;
+    const int le = tid / S::TPE;
+    const int i = tid % S::TPE;
+    const int64_t e = p.group_start[P - 1] + int64_t(blk) * S::EPB + le;
+    const bool row_ok = (le < S::EPB) && (e < p.group_start[P]) && (i < NB);
+    const int64_t base = p.group_dof[P - 1] + (e - p.group_start[P - 1]) * 4 * NB;
+    double* d = sd + le * 4 * NB;
+    double v = 0, m = 0;
+    if (row_ok)
+        for (int f = 0; f < 4; ++f) {
+            const int64_t k = base + f * NB + i;
+            const double xv = p.x_f32 ? double(static_cast<const float*>(p.x)[k]) : static_cast<const double*>(p.x)[k];
+            const double yv = p.y_f32 ? double(static_cast<const float*>(p.y)[k]) : static_cast<const double*>(p.y)[k];
+            d[f * NB + i] = xv - yv;
+            m = fmax(m, fabs(xv - yv));
+        }
+    __syncthreads();
+    if (row_ok) {
+        const double* M = p.tab + p.tix[(P - 1) * kTixStride + kTixM] + i * NB;
+        for (int f = 0; f < 4; ++f) {
+            double s = 0;
+            for (int j = 0; j < NB; ++j) s += M[j] * d[f * NB + j];
+            v += s * d[f * NB + i];
+        }
+        v *= p.absdet[e];
+    }
+    red[tid] = v;
+    red[kBlock + tid] = m;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            red[tid] += red[tid + w];
+            red[kBlock + tid] = fmax(red[kBlock + tid], red[kBlock + tid + w]);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void simplex_diff_norm(const SimplexDiffParams p) {
+    __shared__ double sd[SimplexShape<4>::EPB * 4 * SimplexShape<4>::NB > SimplexShape<2>::EPB * 4 * SimplexShape<2>::NB
+                             ? SimplexShape<4>::EPB * 4 * SimplexShape<4>::NB
+                             : SimplexShape<2>::EPB * 4 * SimplexShape<2>::NB];
+    __shared__ double red[2 * kBlock];
+    const int b = blockIdx.x;
+    if (b < p.group_block[1]) simplex_diff_block<1>(p, b - p.group_block[0], sd, red);
+    else if (b < p.group_block[2]) simplex_diff_block<2>(p, b - p.group_block[1], sd, red);
+    else if (b < p.group_block[3]) simplex_diff_block<3>(p, b - p.group_block[2], sd, red);
+    else simplex_diff_block<4>(p, b - p.group_block[3], sd, red);
+    if (threadIdx.x == 0) {
+        p.partial[2 * b] = red[0];
+        p.partial[2 * b + 1] = red[kBlock];
+    }
+}
+
+// ---------------------------------------------------------------- host side
+namespace {
+
+int epb_of(int p) { return kBlock / simplex_tpe(p); }
+
+void group_blocks(const SimplexMesh& sm, int32_t* gb) {
+    gb[0] = 0;
+    for (int g = 0; g < 4; ++g) {
+        const int64_t n = sm.group_start[g + 1] - sm.group_start[g];
+        gb[g + 1] = gb[g] + int32_t((n + epb_of(g + 1) - 1) / epb_of(g + 1));
+    }
+}
+
+}  // namespace
+
+void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
+    SimplexMesh& sm = c->sm;
+    const int64_t ne = m->n_elements;
+    std::vector<int32_t> order(ne);
+    for (int64_t e = 0; e < ne; ++e) {
+        order[e] = m->order ? m->order[e] : m->uniform_order;
+        require(order[e] >= 1 && order[e] <= kMaxSimplexOrder, HGDG_ERR_UNSUPPORTED,
+                "tetrahedral orders must be in 1..4");
+    }
+    // group by order, mesh order inside a group
+    int64_t counts[4] = {0, 0, 0, 0};
+    for (int64_t e = 0; e < ne; ++e) ++counts[order[e] - 1];
+    sm.group_start[0] = 0;
+    for (int g = 0; g < 4; ++g) sm.group_start[g + 1] = sm.group_start[g] + counts[g];
+    sm.dev_of.assign(ne, 0);
+    sm.mesh_of.assign(ne, 0);
+    sm.order.assign(ne, 0);
+    int64_t fill[4] = {sm.group_start[0], sm.group_start[1], sm.group_start[2], sm.group_start[3]};
+    for (int64_t e = 0; e < ne; ++e) {
+        const int64_t d = fill[order[e] - 1]++;
+        sm.dev_of[e] = d;
+        sm.mesh_of[d] = e;
+        sm.order[d] = order[e];
+    }
+    constexpr int nf = 4;  // fields of the acoustic system
+    sm.group_dof[0] = 0;
+    for (int g = 0; g < 4; ++g) sm.group_dof[g + 1] = sm.group_dof[g] + counts[g] * nf * tet_nb(g + 1);
+    sm.dof_off.assign(ne, 0);
+    for (int64_t d = 0; d < ne; ++d) {
+        const int g = sm.order[d] - 1;
+        sm.dof_off[d] = sm.group_dof[g] + (d - sm.group_start[g]) * nf * tet_nb(g + 1);
+    }
+    c->n_fields = nf;
+    c->dof = sm.group_dof[4];
+
+    // geometry (device order): J column k = X_{k+1} - X_0
+    sm.jinv.assign(size_t(ne) * 9, 0.0);
+    sm.absdet.assign(ne, 0.0);
+    std::vector<double> jinv_soa(9 * size_t(ne)), detJ(ne);
+    std::vector<std::array<double, 9>> Jm(ne);
+    for (int64_t d = 0; d < ne; ++d) {
+        const double* X = m->coords + size_t(sm.mesh_of[d]) * 12;
+        double J[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int k = 0; k < 3; ++k) J[r][k] = X[(k + 1) * 3 + r] - X[r];
+        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        require(det != 0.0 && std::isfinite(det), HGDG_ERR_ARG, "degenerate tetrahedron");
+        double inv[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int k = 0; k < 3; ++k) {
+                const int r1 = (k + 1) % 3, r2 = (k + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+                inv[r][k] = (J[r1][c1] * J[r2][c2] - J[r1][c2] * J[r2][c1]) / det;
+            }
+        for (int r = 0; r < 3; ++r)
+            for (int k = 0; k < 3; ++k) {
+                jinv_soa[(r * 3 + k) * size_t(ne) + d] = inv[r][k];
+                Jm[d][r * 3 + k] = J[r][k];
+            }
+        detJ[d] = det;
+        sm.absdet[d] = std::fabs(det);
+    }
+
+    // faces: sorted node triples; the lower mesh index is hpGEM's left element
+    sm.nbr_dev.assign(4 * size_t(ne), -1);
+    sm.info.assign(4 * size_t(ne), 0);
+    sm.sigma.assign(4 * size_t(ne), 0.0);
+    sm.normal.assign(12 * size_t(ne), 0.0);
+    struct Open { int64_t e; int f; };
+    std::unordered_map<std::string, Open> open;
+    open.reserve(size_t(ne) * 2);
+    int64_t n_internal = 0, n_total = 0;
+    static const double V[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int64_t e = 0; e < ne; ++e) {
+        const int64_t* nid = m->node_ids + size_t(e) * 4;
+        const int64_t d = sm.dev_of[e];
+        for (int f = 0; f < 4; ++f) {
+            // outward normal: sign(det J) (J c1) x (J c2), c = face edges in
+            // reference coordinates (FaceGeometry.h:412-459)
+            const int* fn = kTetFaceNodes[f];
+            double a[3], b[3];
+            for (int r = 0; r < 3; ++r) {
+                a[r] = 0;
+                b[r] = 0;
+                for (int k = 0; k < 3; ++k) {
+                    a[r] += Jm[d][r * 3 + k] * (V[fn[1]][k] - V[fn[0]][k]);
+                    b[r] += Jm[d][r * 3 + k] * (V[fn[2]][k] - V[fn[0]][k]);
+                }
+            }
+            double nrm[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+            const double sg = detJ[d] > 0 ? 1.0 : -1.0;
+            const double len = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+            for (int r = 0; r < 3; ++r) sm.normal[(size_t(r) * 4 + f) * ne + d] = sg * nrm[r] / len;
+            sm.sigma[size_t(f) * ne + d] = len / sm.absdet[d];
+            sm.info[size_t(f) * ne + d] = info_pack(0, 0, 0, 0);
+
+            std::array<int64_t, 3> key{nid[fn[0]], nid[fn[1]], nid[fn[2]]};
+            std::sort(key.begin(), key.end());
+            std::string k(reinterpret_cast<const char*>(key.data()), sizeof(key));
+            auto it = open.find(k);
+            if (it == open.end()) {
+                open.emplace(std::move(k), Open{e, f});
+                continue;
+            }
+            const int64_t L = it->second.e;
+            const int fL = it->second.f;
+            require(L != e, HGDG_ERR_ARG, "tetrahedron meets itself across a face");
+            const int64_t* lid = m->node_ids + size_t(L) * 4;
+            long long ln[3], rn[3];
+            for (int q = 0; q < 3; ++q) {
+                ln[q] = lid[kTetFaceNodes[fL][q]];
+                rn[q] = nid[fn[q]];
+            }
+            const int o = tri_orientation_index(ln, rn);
+            const int64_t dL = sm.dev_of[L];
+            require(sm.nbr_dev[size_t(fL) * ne + dL] < 0, HGDG_ERR_ARG, "a face is bounded by more than two elements");
+            sm.nbr_dev[size_t(fL) * ne + dL] = int32_t(d);
+            sm.nbr_dev[size_t(f) * ne + d] = int32_t(dL);
+            sm.info[size_t(fL) * ne + dL] = info_pack(f, 0, o, sm.order[d]);
+            sm.info[size_t(f) * ne + d] = info_pack(fL, o, 0, sm.order[dL]);
+            open.erase(it);
+            ++n_internal;
+            ++n_total;
+        }
+    }
+    n_total += int64_t(open.size());
+    c->n_faces = n_total;
+    c->n_internal = n_internal;
+
+    // reference tables of every order
+    sm.ops.clear();
+    for (int q = 1; q <= kMaxSimplexOrder; ++q) sm.ops.push_back(simplex_ops(q));
+    std::vector<double> tab;
+    std::vector<int32_t> tix(size_t(kTixStride) * kMaxSimplexOrder, 0);
+    std::vector<int32_t> cols;
+    for (int q = 1; q <= kMaxSimplexOrder; ++q) {
+        const SimplexOps& ops = sm.ops[q - 1];
+        int32_t* t = tix.data() + (q - 1) * kTixStride;
+        t[kTixDr] = int32_t(tab.size());
+        tab.insert(tab.end(), ops.Dr.begin(), ops.Dr.end());
+        t[kTixM] = int32_t(tab.size());
+        tab.insert(tab.end(), ops.Mref.begin(), ops.Mref.end());
+        for (int f = 0; f < 4; ++f)
+            for (int o = 0; o < 6; ++o) {
+                t[kTixProj + f * 6 + o] = int32_t(tab.size());
+                tab.insert(tab.end(), ops.proj[f][o].begin(), ops.proj[f][o].end());
+                t[kTixLift + f * 6 + o] = int32_t(tab.size());
+                tab.insert(tab.end(), ops.lift[f][o].begin(), ops.lift[f][o].end());
+            }
+        for (int f = 0; f < 4; ++f) {
+            t[kTixNcol + f] = int32_t(ops.cols[f].size());
+            t[kTixCols + f] = int32_t(cols.size());  // relative, fixed below
+            cols.insert(cols.end(), ops.cols[f].begin(), ops.cols[f].end());
+        }
+    }
+    const int32_t col_base = int32_t(tix.size());
+    for (int q = 0; q < kMaxSimplexOrder; ++q)
+        for (int f = 0; f < 4; ++f) tix[q * kTixStride + kTixCols + f] += col_base;
+    tix.insert(tix.end(), cols.begin(), cols.end());
+
+    sm.d_tables.upload(tab, c->stream);
+    sm.d_tab_index.upload(tix, c->stream);
+    sm.d_absdet.upload(sm.absdet, c->stream);
+    std::vector<double> geo(jinv_soa);
+    geo.resize(geo.size() + size_t(ne), 1.0);  // c_inv, set with the operator
+    geo.insert(geo.end(), sm.sigma.begin(), sm.sigma.end());
+    geo.insert(geo.end(), sm.normal.begin(), sm.normal.end());
+    sm.d_geo.upload(geo, c->stream);
+    std::vector<int32_t> face(sm.nbr_dev);
+    face.insert(face.end(), sm.info.begin(), sm.info.end());
+    sm.d_face.upload(face, c->stream);
+    HGDG_HIP(hipStreamSynchronize(c->stream));
+}
+
+void simplex_operator_set(hgdg_ctx* c, const hgdg_op_desc* op) {
+    require(op->kind == HGDG_OP_ACOUSTIC_CENTRAL, HGDG_ERR_UNSUPPORTED,
+            "tetrahedral meshes run the acoustic operator");
+    require(op->precision == HGDG_PREC_FP64, HGDG_ERR_UNSUPPORTED,
+            "the tetrahedral kernels run in FP64 in this build");
+    SimplexMesh& sm = c->sm;
+    const int64_t ne = c->n_elem;
+    std::vector<double> cinv(ne);
+    for (int64_t d = 0; d < ne; ++d) cinv[d] = op->c_inv[sm.mesh_of[d]];
+    HGDG_HIP(hipMemcpyAsync(sm.d_geo.ptr + 9 * ne, cinv.data(), size_t(ne) * sizeof(double), hipMemcpyHostToDevice,
+                            c->stream));
+    HGDG_HIP(hipStreamSynchronize(c->stream));
+}
+
+void simplex_rhs(hgdg_ctx* c, const StageIO& io) {
+    SimplexMesh& sm = c->sm;
+    const int64_t ne = c->n_elem;
+    SimplexParams p{};
+    p.io = io;
+    p.n_elem = ne;
+    for (int g = 0; g < 5; ++g) {
+        p.group_start[g] = sm.group_start[g];
+        p.group_dof[g] = sm.group_dof[g];
+    }
+    group_blocks(sm, p.group_block);
+    p.jinv = sm.d_geo.ptr;
+    p.cinv = sm.d_geo.ptr + 9 * ne;
+    p.sigma = sm.d_geo.ptr + 10 * ne;
+    p.normal = sm.d_geo.ptr + 14 * ne;
+    p.nbr = sm.d_face.ptr;
+    p.info = sm.d_face.ptr + 4 * ne;
+    p.tab = sm.d_tables.ptr;
+    p.tix = sm.d_tab_index.ptr;
+    const int grid = p.group_block[4];
+    if (grid > 0) hipLaunchKernelGGL(simplex_acoustic_rhs, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
+    HGDG_HIP(hipGetLastError());
+}
+
+void simplex_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, double* max_abs) {
+    SimplexMesh& sm = c->sm;
+    SimplexDiffParams p{};
+    p.n_elem = c->n_elem;
+    for (int g = 0; g < 5; ++g) {
+        p.group_start[g] = sm.group_start[g];
+        p.group_dof[g] = sm.group_dof[g];
+    }
+    group_blocks(sm, p.group_block);
+    p.x = vx.ptr();
+    p.y = vy.ptr();
+    p.x_f32 = vx.is_f32;
+    p.y_f32 = vy.is_f32;
+    p.absdet = sm.d_absdet.ptr;
+    p.tab = sm.d_tables.ptr;
+    p.tix = sm.d_tab_index.ptr;
+    const int grid = p.group_block[4];
+    if (c->d_partial.n < size_t(2 * grid)) c->d_partial.alloc(size_t(2 * grid));
+    p.partial = c->d_partial.ptr;
+    hipLaunchKernelGGL(simplex_diff_norm, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
+    HGDG_HIP(hipGetLastError());
+    std::vector<double> h(size_t(2 * grid));
+    HGDG_HIP(hipMemcpyAsync(h.data(), c->d_partial.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HGDG_HIP(hipStreamSynchronize(c->stream));
+    double s = 0, mx = 0;
+    for (int b = 0; b < grid; ++b) {
+        s += h[2 * b];
+        mx = std::max(mx, h[2 * b + 1]);
+    }
+    *l2 = std::sqrt(std::max(s, 0.0));
+    *max_abs = mx;
+}
+
+void simplex_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev) {
+    const SimplexMesh& sm = c->sm;
+    dev.assign(size_t(c->n_dof()), 0.0);
+    int64_t off = 0;
+    for (int64_t e = 0; e < c->n_elem; ++e) {
+        const int64_t d = sm.dev_of[e];
+        const int64_t n = int64_t(c->n_fields) * tet_nb(sm.order[d]);
+        std::copy(host + off, host + off + n, dev.begin() + sm.dof_off[d]);
+        off += n;
+    }
+}
+
+void simplex_from_device(hgdg_ctx* c, const std::vector<double>& dev, double* host) {
+    const SimplexMesh& sm = c->sm;
+    int64_t off = 0;
+    for (int64_t e = 0; e < c->n_elem; ++e) {
+        const int64_t d = sm.dev_of[e];
+        const int64_t n = int64_t(c->n_fields) * tet_nb(sm.order[d]);
+        std::copy(dev.begin() + sm.dof_off[d], dev.begin() + sm.dof_off[d] + n, host + off);
+        off += n;
+    }
+}
+
+}  // namespace hgdg

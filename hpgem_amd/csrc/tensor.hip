@@ -1,0 +1,646 @@
+// hgdg: MI355X (gfx950) explicit DG operator behind a C ABI.
+//
+// Tensor elements (squares, cubes). On an affine element with constant
+// advection velocity a, hpGEM's stiffness, face and mass integrals
+// (kernel/Base/HpgemAPILinear_Impl.h:131-139,250-288 with the integrands of
+// tests/self/Basic/080AdvectionLinear_SelfTest.cpp:108-170) factor into 1D
+// operators: with the basis psi_a(x) psi_b(y) psi_c(z), M = |J| M1 x M1 x M1,
+// and after the block-diagonal inverse mass apply the weak form
+//
+//   k = sum_d ahat_d (D along d) w - sum_{faces (d,s)} (2s-1) ahat_d Mi(:,s) (x) T_{d,s}
+//
+// (ahat = J^{-1} a, D = M1^{-1} G1, Mi = M1^{-1}, T_{d,s} the upwind trace
+// coefficients on face x_d = 2s-1) is evaluated in its integrated-by-parts
+// ("lifting") form, which is the same number up to rounding:
+//
+//   k = -sum_d ahat_d (Dr along d) w - sum_{faces} (2s-1) ahat_d Mi(:,s) (x) (T_{d,s} - own_{d,s})
+//
+// with Dr = M1^{-1} K the exact 1D derivative matrix (K(a,b) = int psi_a psi_b')
+// and own_{d,s} the element's own trace. Outflow faces drop out, and the face
+// terms only carry jumps, which are formed in FP64 from the FP64 stage input:
+// no large terms cancel, so the FP32 variant keeps ~1e-7 relative accuracy.
+// The 1D operators are built once on the host in long double by exact Gauss
+// quadrature; each element needs its stage input, `dim` numbers of geometry
+// and the traces of its inflow neighbours. One thread owns one coefficient;
+// an element's stage input sits in LDS for the 1D contractions.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <string>
+#include <unordered_map>
+
+#include "context.hpp"
+
+namespace hgdg {
+
+// Per-face upwind modes, 3 bits per tensor face f = 2d + s.
+enum : uint32_t { MODE_NONE = 0, MODE_OWN = 1, MODE_NBR = 2, MODE_AVG = 3, MODE_HALF_OWN = 4 };
+
+struct RhsParams {
+    int64_t n_elem;
+    StageIO io;
+    const double* ahat;    // [dim][n_elem]
+    const int32_t* nbr;    // [2 dim][n_elem]
+    const uint32_t* mode;  // [n_elem]
+    const double* Dr;      // [n1][n1] exact derivative matrix
+    const double* Mi;      // [n1][n1] inverse 1D mass
+};
+
+template <int DIM, int N1>
+struct Tensor {
+    static constexpr int NB = N1 * N1 * (DIM == 3 ? N1 : 1);
+    static constexpr int L = NB <= 4 ? 4 : NB <= 8 ? 8 : NB <= 16 ? 16 : NB <= 32 ? 32 : NB <= 64 ? 64 : NB <= 128 ? 128 : 256;
+    static constexpr int EPB = kBlock / L;
+};
+
+// One thread owns one x-pencil of an element (the N1 coefficients with fixed
+// y/z indices): the x-contraction and the x-face terms stay in registers, the
+// y/z contractions read the other pencils of the element from LDS, and the
+// y/z inflow traces arrive as whole pencils of the neighbour (vector loads).
+template <int DIM, int N1>
+struct Pencil {
+    static constexpr int NP = DIM == 3 ? N1 * N1 : N1;  // pencils per element
+    static constexpr int NB = NP * N1;                   // coefficients per element
+    static constexpr int EPB = kBlock / NP;              // elements per block
+};
+
+// x[a] = vector i at [base, base + N1), as double
+template <int N1>
+__device__ __forceinline__ void load_pencil(const StageIO& p, int i, int64_t base, double (&x)[N1]) {
+    if ((p.in_f32 >> i) & 1u) {
+        const float* f = static_cast<const float*>(p.in[i]) + base;
+        if constexpr (N1 % 4 == 0) {
+#pragma unroll
+            for (int a = 0; a < N1; a += 4) {
+                const float4 v = *reinterpret_cast<const float4*>(f + a);
+                x[a] = v.x; x[a + 1] = v.y; x[a + 2] = v.z; x[a + 3] = v.w;
+            }
+        } else if constexpr (N1 % 2 == 0) {
+#pragma unroll
+            for (int a = 0; a < N1; a += 2) {
+                const float2 v = *reinterpret_cast<const float2*>(f + a);
+                x[a] = v.x; x[a + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < N1; ++a) x[a] = f[a];
+        }
+    } else {
+        const double* d = static_cast<const double*>(p.in[i]) + base;
+        if constexpr (N1 % 2 == 0) {
+#pragma unroll
+            for (int a = 0; a < N1; a += 2) {
+                const double2 v = *reinterpret_cast<const double2*>(d + a);
+                x[a] = v.x; x[a + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < N1; ++a) x[a] = d[a];
+        }
+    }
+}
+
+// Stage input pencil sum_i coef_i vec_i (getLinearCombinationOfVectors).
+template <int N1>
+__device__ __forceinline__ void combine_pencil(const StageIO& p, int64_t base, double (&w)[N1]) {
+    double x[N1];
+    load_pencil<N1>(p, 0, base, x);
+#pragma unroll
+    for (int a = 0; a < N1; ++a) w[a] = p.coef[0] * x[a];
+#pragma unroll
+    for (int i = 1; i < kMaxIn; ++i) {
+        if (i >= p.n_in) break;
+        load_pencil<N1>(p, i, base, x);
+#pragma unroll
+        for (int a = 0; a < N1; ++a) w[a] += p.coef[i] * x[a];
+    }
+}
+
+__device__ __forceinline__ double face_jump(uint32_t m, double own, double other) {
+    if (m == MODE_NONE) return -own;
+    if (m == MODE_HALF_OWN) return -0.5 * own;
+    return (m == MODE_NBR) ? other - own : 0.5 * (other - own);
+}
+
+template <int DIM, int N1, typename T>
+__global__ __launch_bounds__(kBlock) void tensor_advection_rhs(const RhsParams p) {
+    using S = Pencil<DIM, N1>;
+    __shared__ double s_w[S::EPB * S::NB];  // FP64 stage input of the block's elements
+    __shared__ T s_D[N1 * N1];
+    __shared__ T s_Mi[2 * N1];  // s_Mi[s * N1 + a] = Mi(a, s)
+    const int tid = threadIdx.x;
+    if (tid < N1 * N1) s_D[tid] = T(p.Dr[tid]);
+    if (tid < 2 * N1) s_Mi[tid] = T(p.Mi[(tid % N1) * N1 + tid / N1]);
+    const int le = tid / S::NP;
+    const int pc = tid % S::NP;
+    const int64_t e = int64_t(blockIdx.x) * S::EPB + le;
+    const bool active = (le < S::EPB) && (e < p.n_elem);
+    const int ib = pc % N1;                    // y index of this pencil
+    const int ic = DIM == 3 ? pc / N1 : 0;     // z index of this pencil
+    double* sw = s_w + le * S::NB;
+    double w[N1];
+    if (active) {
+        combine_pencil<N1>(p.io, e * S::NB + pc * N1, w);
+#pragma unroll
+        for (int a = 0; a < N1; ++a) sw[pc * N1 + a] = w[a];
+    }
+    __syncthreads();
+    if (!active) return;
+
+    T ah[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) ah[d] = T(p.ahat[d * p.n_elem + e]);
+
+    // volume (strong form): -sum_d ahat_d (Dr along d) w
+    T vol[N1];
+#pragma unroll
+    for (int a = 0; a < N1; ++a) {
+        T s = T(0);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) s += s_D[a * N1 + j] * T(w[j]);
+        vol[a] = -ah[0] * s;
+    }
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+        const T cy = -ah[1] * s_D[ib * N1 + j];
+        const double* row = sw + (j + N1 * ic) * N1;  // pencil (j, ic)
+#pragma unroll
+        for (int a = 0; a < N1; ++a) vol[a] += cy * T(row[a]);
+    }
+    if constexpr (DIM == 3) {
+#pragma unroll
+        for (int j = 0; j < N1; ++j) {
+            const T cz = -ah[2] * s_D[ic * N1 + j];
+            const double* row = sw + (ib + N1 * j) * N1;  // pencil (ib, j)
+#pragma unroll
+            for (int a = 0; a < N1; ++a) vol[a] += cz * T(row[a]);
+        }
+    }
+    Acc<T> acc[N1] = {};
+#pragma unroll
+    for (int a = 0; a < N1; ++a) acc[a] = Acc<T>(vol[a]);
+
+    // faces: -(2s-1) ahat_d Mi(idx_d, s) (T_{d,s} - own_{d,s}); jumps in FP64
+    const uint32_t md = p.mode[e];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {  // x faces: one trace value per pencil
+        const uint32_t m = (md >> (3 * s)) & 7u;
+        if (m == MODE_OWN) continue;
+        double other = 0.0;
+        if (m == MODE_NBR || m == MODE_AVG) {
+            const int64_t nb = p.nbr[s * p.n_elem + e];
+            double v[1];
+            const int64_t idx = nb * S::NB + pc * N1 + (1 - s);
+            v[0] = p.io.coef[0] * load_in(p.io, 0, idx);
+            for (int i = 1; i < kMaxIn; ++i) {
+                if (i >= p.io.n_in) break;
+                v[0] += p.io.coef[i] * load_in(p.io, i, idx);
+            }
+            other = v[0];
+        }
+        const T jump = T(face_jump(m, w[s], other));
+        const T sc = (s ? T(-1) : T(1)) * ah[0] * jump;
+#pragma unroll
+        for (int a = 0; a < N1; ++a) acc[a].add(sc * s_Mi[s * N1 + a]);
+    }
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) {  // y (and z) faces: a whole pencil of traces
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int f = 2 * d + s;
+            const uint32_t m = (md >> (3 * f)) & 7u;
+            if (m == MODE_OWN) continue;
+            const int own_pc = d == 1 ? s + N1 * ic : ib + N1 * s;
+            const int nbr_pc = d == 1 ? (1 - s) + N1 * ic : ib + N1 * (1 - s);
+            double other[N1];
+            if (m == MODE_NBR || m == MODE_AVG) {
+                const int64_t nb = p.nbr[f * p.n_elem + e];
+                combine_pencil<N1>(p.io, nb * S::NB + nbr_pc * N1, other);
+            } else {
+#pragma unroll
+                for (int a = 0; a < N1; ++a) other[a] = 0.0;
+            }
+            const int my = d == 1 ? ib : ic;
+            const T sc = (s ? T(-1) : T(1)) * ah[d] * s_Mi[s * N1 + my];
+#pragma unroll
+            for (int a = 0; a < N1; ++a) acc[a].add(sc * T(face_jump(m, sw[own_pc * N1 + a], other[a])));
+        }
+    }
+    const int64_t o = e * S::NB + pc * N1;
+    if (p.io.out_f32) {
+        float* out = static_cast<float*>(p.io.out) + o;
+#pragma unroll
+        for (int a = 0; a < N1; ++a) out[a] = float(acc[a].value());
+    } else {
+        double* out = static_cast<double*>(p.io.out) + o;
+#pragma unroll
+        for (int a = 0; a < N1; ++a) out[a] = double(acc[a].value());
+    }
+}
+
+// Per-element mass-norm of x - y: |J| dx^T (M1 x M1 x M1) dx, and max |dx|;
+// one partial per block.
+struct DiffParams {
+    int64_t n_elem;
+    const void* x;
+    const void* y;
+    int32_t x_f32, y_f32;
+    const double* absdet;  // [n_elem]
+    const double* M1;      // [n1][n1]
+    double* partial;       // [grid][2]
+};
+
+template <int DIM, int N1>
+__global__ __launch_bounds__(kBlock) void tensor_diff_norm(const DiffParams p) {
+    using S = Tensor<DIM, N1>;
+    __shared__ double s_a[S::EPB * S::NB];
+    __shared__ double s_b[S::EPB * S::NB];
+    __shared__ double s_M[N1 * N1];
+    __shared__ double s_red[2][kBlock];
+    const int tid = threadIdx.x;
+    if (tid < N1 * N1) s_M[tid] = p.M1[tid];
+    const int le = tid / S::L;
+    const int dof = tid % S::L;
+    const int64_t e = int64_t(blockIdx.x) * S::EPB + le;
+    const bool active = (e < p.n_elem) && (dof < S::NB);
+    double dx = 0;
+    if (active) {
+        const int64_t i = e * S::NB + dof;
+        const double xv = p.x_f32 ? double(static_cast<const float*>(p.x)[i]) : static_cast<const double*>(p.x)[i];
+        const double yv = p.y_f32 ? double(static_cast<const float*>(p.y)[i]) : static_cast<const double*>(p.y)[i];
+        dx = xv - yv;
+        s_a[le * S::NB + dof] = dx;
+    }
+    __syncthreads();
+    const int idx[3] = {dof % N1, (dof / N1) % N1, DIM == 3 ? dof / (N1 * N1) : 0};
+    constexpr int stride[3] = {1, N1, N1 * N1};
+    double* src = s_a;
+    double* dst = s_b;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        if (active) {
+            const int row = dof - idx[d] * stride[d];
+            double s = 0;
+            for (int j = 0; j < N1; ++j) s += s_M[idx[d] * N1 + j] * src[le * S::NB + row + j * stride[d]];
+            dst[le * S::NB + dof] = s;
+        }
+        __syncthreads();
+        double* t = src;
+        src = dst;
+        dst = t;
+    }
+    double v = 0, m = 0;
+    if (active) {
+        v = dx * src[le * S::NB + dof] * p.absdet[e];
+        m = fabs(dx);
+    }
+    s_red[0][tid] = v;
+    s_red[1][tid] = m;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            s_red[0][tid] += s_red[0][tid + w];
+            s_red[1][tid] = fmax(s_red[1][tid], s_red[1][tid + w]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        p.partial[2 * blockIdx.x] = s_red[0][0];
+        p.partial[2 * blockIdx.x + 1] = s_red[1][0];
+    }
+}
+
+template <typename F>
+void dispatch_tensor(int dim, int n1, F&& f) {
+#define HGDG_CASE(D, N) \
+    if (dim == D && n1 == N) return f(std::integral_constant<int, D>{}, std::integral_constant<int, N>{});
+    HGDG_CASE(2, 2) HGDG_CASE(2, 3) HGDG_CASE(2, 4) HGDG_CASE(2, 5) HGDG_CASE(2, 6)
+    HGDG_CASE(3, 2) HGDG_CASE(3, 3) HGDG_CASE(3, 4) HGDG_CASE(3, 5) HGDG_CASE(3, 6)
+#undef HGDG_CASE
+    throw Failure(HGDG_ERR_UNSUPPORTED, "no tensor kernel for this dimension and order");
+}
+
+// ---------------------------------------------------------------- host side
+namespace {
+const double* Dr_ptr(hgdg_ctx* c) { return c->tm.d_ops.ptr; }
+const double* Mi_ptr(hgdg_ctx* c) { return c->tm.d_ops.ptr + c->tm.n1 * c->tm.n1; }
+const double* M1_ptr(hgdg_ctx* c) { return c->tm.d_ops.ptr + 2 * c->tm.n1 * c->tm.n1; }
+}  // namespace
+
+void tensor_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
+    const int dim = shape_dim(m->shape);
+    const int64_t ne = m->n_elements;
+    int p = m->order ? m->order[0] : m->uniform_order;
+    if (m->order)
+        for (int64_t e = 0; e < ne; ++e)
+            require(m->order[e] == p, HGDG_ERR_UNSUPPORTED,
+                    "tensor elements need one polynomial order per mesh");
+    require(p >= 1 && p <= 5, HGDG_ERR_UNSUPPORTED, "polynomial order must be in 1..5");
+    const int nv = shape_vertices(m->shape);
+
+    c->tm.p = p;
+    c->tm.n1 = p + 1;
+    c->tm.nb = dim == 3 ? c->tm.n1 * c->tm.n1 * c->tm.n1 : c->tm.n1 * c->tm.n1;
+    c->tm.perm = tensor_permutation(dim, p);
+
+    // geometry: affine hypercubes only; J column d = (X_{2^d} - X_0) / 2
+    c->tm.Jinv.assign(size_t(ne) * dim * dim, 0.0);
+    c->tm.absdet.assign(ne, 0.0);
+    c->tm.X0.assign(size_t(ne) * dim, 0.0);
+    c->tm.E.assign(size_t(ne) * dim * dim, 0.0);
+    for (int64_t e = 0; e < ne; ++e) {
+        const double* X = m->coords + size_t(e) * nv * dim;
+        double E[3][3] = {{0}};
+        double scale = 0;
+        for (int d = 0; d < dim; ++d)
+            for (int r = 0; r < dim; ++r) {
+                E[d][r] = X[(1 << d) * dim + r] - X[r];
+                scale = std::max(scale, std::fabs(E[d][r]));
+            }
+        for (int d = 0; d < dim; ++d) {
+            c->tm.X0[size_t(e) * dim + d] = X[d];
+            for (int r = 0; r < dim; ++r) c->tm.E[(size_t(e) * dim + d) * dim + r] = E[d][r];
+        }
+        for (int v = 0; v < nv; ++v)
+            for (int r = 0; r < dim; ++r) {
+                double pred = X[r];
+                for (int d = 0; d < dim; ++d)
+                    if ((v >> d) & 1) pred += E[d][r];
+                require(std::fabs(pred - X[v * dim + r]) <= 1e-12 * scale, HGDG_ERR_UNSUPPORTED,
+                        "element " + std::to_string(e) + " is not affine (parallelogram/parallelepiped)");
+            }
+        double J[3][3] = {{0}};  // J[r][c] = d x_r / d xi_c
+        for (int r = 0; r < dim; ++r)
+            for (int col = 0; col < dim; ++col) J[r][col] = E[col][r] / 2;
+        double det, inv[3][3];
+        if (dim == 2) {
+            det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+            inv[0][0] = J[1][1] / det;
+            inv[0][1] = -J[0][1] / det;
+            inv[1][0] = -J[1][0] / det;
+            inv[1][1] = J[0][0] / det;
+        } else {
+            det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                  J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+            for (int r = 0; r < 3; ++r)
+                for (int col = 0; col < 3; ++col) {
+                    const int r1 = (col + 1) % 3, r2 = (col + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+                    inv[r][col] = (J[r1][c1] * J[r2][c2] - J[r1][c2] * J[r2][c1]) / det;
+                }
+        }
+        require(det != 0.0 && std::isfinite(det), HGDG_ERR_ARG, "degenerate element " + std::to_string(e));
+        c->tm.absdet[e] = std::fabs(det);
+        for (int r = 0; r < dim; ++r)
+            for (int col = 0; col < dim; ++col) c->tm.Jinv[(size_t(e) * dim + r) * dim + col] = inv[r][col];
+    }
+
+    // faces: match sorted node tuples; the lower element index is hpGEM's
+    // left element (faces are created while elements are visited in order).
+    const int nfaces = 2 * dim;
+    const int fverts = 1 << (dim - 1);
+    c->tm.nbr.assign(size_t(nfaces) * ne, -1);
+    std::unordered_map<std::string, std::pair<int64_t, int>> open;
+    open.reserve(size_t(ne) * nfaces);
+    int64_t n_internal = 0, n_total = 0;
+    auto face_vertex = [&](int f, int q) {
+        // local vertex of tensor face f = 2d + s: bit d = s, the other bits from q in order
+        const int d = f / 2, s = f % 2;
+        int v = 0, bit = 0;
+        for (int k = 0; k < dim; ++k) {
+            if (k == d) {
+                v |= s << k;
+            } else {
+                v |= ((q >> bit) & 1) << k;
+                ++bit;
+            }
+        }
+        return v;
+    };
+    for (int64_t e = 0; e < ne; ++e) {
+        const int64_t* nid = m->node_ids + size_t(e) * nv;
+        for (int f = 0; f < nfaces; ++f) {
+            std::array<int64_t, 4> key{};
+            for (int q = 0; q < fverts; ++q) key[q] = nid[face_vertex(f, q)];
+            std::sort(key.begin(), key.begin() + fverts);
+            std::string k(reinterpret_cast<const char*>(key.data()), sizeof(int64_t) * fverts);
+            auto it = open.find(k);
+            if (it == open.end()) {
+                open.emplace(std::move(k), std::make_pair(e, f));
+                continue;
+            }
+            const int64_t o = it->second.first;
+            const int fo = it->second.second;
+            require(o != e, HGDG_ERR_UNSUPPORTED, "element " + std::to_string(e) + " meets itself across a face");
+            require(c->tm.nbr[size_t(fo) * ne + o] < 0, HGDG_ERR_ARG, "a face is bounded by more than two elements");
+            // tensor kernels need the neighbour's opposite face in the same
+            // orientation: my face (d, s) against its (d, 1-s), vertex by vertex
+            require(fo / 2 == f / 2 && fo % 2 != f % 2, HGDG_ERR_UNSUPPORTED,
+                    "neighbouring elements are not aligned (different local face directions)");
+            const int64_t* onid = m->node_ids + size_t(o) * nv;
+            for (int q = 0; q < fverts; ++q)
+                require(onid[face_vertex(fo, q)] == nid[face_vertex(f, q)], HGDG_ERR_UNSUPPORTED,
+                        "neighbouring elements are not aligned (rotated face)");
+            c->tm.nbr[size_t(f) * ne + e] = int32_t(o);
+            c->tm.nbr[size_t(fo) * ne + o] = int32_t(e);
+            open.erase(it);
+            ++n_internal;
+            ++n_total;
+        }
+    }
+    n_total += int64_t(open.size());
+    c->n_faces = n_total;
+    c->n_internal = n_internal;
+    c->n_fields = 1;
+    c->dof = ne * c->tm.nb;
+
+    c->tm.ops = tensor_ops(p);
+    std::vector<double> opsv;
+    opsv.insert(opsv.end(), c->tm.ops.Dr.begin(), c->tm.ops.Dr.end());
+    opsv.insert(opsv.end(), c->tm.ops.Mi.begin(), c->tm.ops.Mi.end());
+    opsv.insert(opsv.end(), c->tm.ops.M1.begin(), c->tm.ops.M1.end());
+    c->tm.d_ops.upload(opsv, c->stream);
+    c->tm.d_nbr.upload(c->tm.nbr, c->stream);
+    c->tm.d_absdet.upload(c->tm.absdet, c->stream);
+    HGDG_HIP(hipStreamSynchronize(c->stream));
+}
+
+void tensor_operator_set(hgdg_ctx* c, const hgdg_op_desc* op) {
+    require(op->kind == HGDG_OP_ADVECTION_UPWIND, HGDG_ERR_UNSUPPORTED,
+            "the acoustic operator is not available on tensor elements");
+    const int dim = c->dim;
+    const int64_t ne = c->n_elem;
+    std::vector<double> ahat(size_t(dim) * ne);
+    std::vector<double> rownorm(size_t(dim) * ne);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int d = 0; d < dim; ++d) {
+            double s = 0, n2 = 0;
+            for (int k = 0; k < dim; ++k) {
+                const double v = c->tm.Jinv[(size_t(e) * dim + d) * dim + k];
+                s += v * op->velocity[k];
+                n2 += v * v;
+            }
+            ahat[size_t(d) * ne + e] = s;
+            rownorm[size_t(d) * ne + e] = std::sqrt(n2);
+        }
+    // a . n_unit on face (d, s) of element e: outward normal (2s-1) J^{-T} e_d
+    auto a_dot_n = [&](int64_t e, int f) {
+        const int d = f / 2, s = f % 2;
+        return (2 * s - 1) * ahat[size_t(d) * ne + e] / rownorm[size_t(d) * ne + e];
+    };
+    std::vector<uint32_t> mode(ne, 0);
+    for (int64_t e = 0; e < ne; ++e) {
+        uint32_t word = 0;
+        for (int f = 0; f < 2 * dim; ++f) {
+            const int32_t o = c->tm.nbr[size_t(f) * ne + e];
+            uint32_t m;
+            if (o < 0) {
+                // boundary: only the left (own) trial functions exist
+                const double A = a_dot_n(e, f);
+                m = A > 1e-12 ? MODE_OWN : (A < -1e-12 ? MODE_NONE : MODE_HALF_OWN);
+            } else {
+                // the left element's normal decides the upwind side
+                const bool left = e < o;
+                const double A = left ? a_dot_n(e, f) : a_dot_n(o, f ^ 1);
+                if (A > 1e-12) m = left ? MODE_OWN : MODE_NBR;
+                else if (A < -1e-12) m = left ? MODE_NBR : MODE_OWN;
+                else m = MODE_AVG;
+            }
+            word |= m << (3 * f);
+        }
+        mode[e] = word;
+    }
+    c->tm.d_geo.upload(ahat, c->stream);
+    c->tm.d_mode.upload(mode, c->stream);
+    HGDG_HIP(hipStreamSynchronize(c->stream));
+}
+
+void tensor_rhs(hgdg_ctx* c, const StageIO& io) {
+    RhsParams p{};
+    p.n_elem = c->n_elem;
+    p.io = io;
+    p.ahat = c->tm.d_geo.ptr;
+    p.nbr = c->tm.d_nbr.ptr;
+    p.mode = c->tm.d_mode.ptr;
+    p.Dr = Dr_ptr(c);
+    p.Mi = Mi_ptr(c);
+    const bool mixed = c->op.precision == HGDG_PREC_MIXED_FP32;
+    dispatch_tensor(c->dim, c->tm.n1, [&](auto D, auto N) {
+        constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
+        using S = Pencil<DIMc, N1c>;
+        const int64_t grid = (c->n_elem + S::EPB - 1) / S::EPB;
+        if (mixed)
+            hipLaunchKernelGGL((tensor_advection_rhs<DIMc, N1c, float>), dim3(unsigned(grid)), dim3(kBlock), 0,
+                               c->stream, p);
+        else
+            hipLaunchKernelGGL((tensor_advection_rhs<DIMc, N1c, double>), dim3(unsigned(grid)), dim3(kBlock), 0,
+                               c->stream, p);
+    });
+    HGDG_HIP(hipGetLastError());
+}
+
+void tensor_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, double* max_abs) {
+    DiffParams p{};
+    p.n_elem = c->n_elem;
+    p.x = vx.ptr();
+    p.y = vy.ptr();
+    p.x_f32 = vx.is_f32;
+    p.y_f32 = vy.is_f32;
+    p.absdet = c->tm.d_absdet.ptr;
+    p.M1 = M1_ptr(c);
+    int64_t grid = 0;
+    dispatch_tensor(c->dim, c->tm.n1, [&](auto D, auto N) {
+        constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
+        using S = Tensor<DIMc, N1c>;
+        grid = (c->n_elem + S::EPB - 1) / S::EPB;
+        if (c->d_partial.n < size_t(2 * grid)) c->d_partial.alloc(size_t(2 * grid));
+        p.partial = c->d_partial.ptr;
+        hipLaunchKernelGGL((tensor_diff_norm<DIMc, N1c>), dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
+    });
+    HGDG_HIP(hipGetLastError());
+    std::vector<double> h(size_t(2 * grid));
+    HGDG_HIP(hipMemcpyAsync(h.data(), c->d_partial.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HGDG_HIP(hipStreamSynchronize(c->stream));
+    double s = 0, m = 0;
+    for (int64_t i = 0; i < grid; ++i) {
+        s += h[2 * i];
+        m = std::max(m, h[2 * i + 1]);
+    }
+    *l2 = std::sqrt(std::max(s, 0.0));
+    *max_abs = m;
+}
+
+// L2 projection of f(x) = sum_m amp_m sin(2 pi k_m . x) onto the DG space
+// (HpgemAPISimplified::setInitialSolution = integrateInitialSolution +
+// solveMassMatrixEquations, HpgemAPISimplified_Impl.h:227-291,751-758), with
+// the element rule of order 2p+1 (Element.cpp:89-98): on an affine tensor
+// element the coefficients are (Mi B^T W)^{(x) dim} applied to f at the
+// Gauss points, B(q, a) = psi_a(x_q). Result in device layout.
+void tensor_project_modes(hgdg_ctx* c, int n_modes, const double* amp, const double* kv, std::vector<double>& out) {
+    const int n1 = c->tm.n1, dim = c->dim, nb = c->tm.nb;
+    std::vector<double> xq, wq;
+    gauss_legendre(n1, xq, wq);
+    // P(a, q) = sum_b Mi(a, b) psi_b(x_q) w_q
+    std::vector<double> P(size_t(n1) * n1, 0.0);
+    for (int a = 0; a < n1; ++a)
+        for (int q = 0; q < n1; ++q) {
+            double s = 0;
+            for (int b = 0; b < n1; ++b) s += c->tm.ops.Mi[a * n1 + b] * psi(b, xq[q]);
+            P[a * n1 + q] = s * wq[q];
+        }
+    const int nq = nb;  // n1^dim points
+    out.assign(size_t(c->n_dof()), 0.0);
+    std::vector<double> F(nq), T1(nq), T2(nq);
+    for (int64_t e = 0; e < c->n_elem; ++e) {
+        const double* X0 = c->tm.X0.data() + size_t(e) * dim;
+        const double* E = c->tm.E.data() + size_t(e) * dim * dim;
+        for (int q = 0; q < nq; ++q) {
+            int r = q;
+            double x[3] = {0, 0, 0};
+            for (int d = 0; d < dim; ++d) x[d] = X0[d];
+            for (int d = 0; d < dim; ++d) {
+                const double t = (xq[r % n1] + 1) / 2;
+                r /= n1;
+                for (int k = 0; k < dim; ++k) x[k] += t * E[d * dim + k];
+            }
+            double f = 0;
+            for (int m = 0; m < n_modes; ++m) {
+                double arg = 0;
+                for (int d = 0; d < dim; ++d) arg += kv[m * 3 + d] * x[d];
+                f += amp[m] * std::sin(2 * M_PI * arg);
+            }
+            F[q] = f;
+        }
+        // apply P along each direction (sum factorisation)
+        std::vector<double>* src = &F;
+        std::vector<double>* dst = &T1;
+        int stride = 1;
+        for (int d = 0; d < dim; ++d) {
+            for (int i = 0; i < nq; ++i) {
+                const int id = (i / stride) % n1;
+                const int base = i - id * stride;
+                double s = 0;
+                for (int q = 0; q < n1; ++q) s += P[id * n1 + q] * (*src)[base + q * stride];
+                (*dst)[i] = s;
+            }
+            src = dst;
+            dst = (dst == &T1) ? &T2 : &T1;
+            stride *= n1;
+        }
+        std::copy(src->begin(), src->end(), out.begin() + size_t(e) * nb);
+    }
+}
+
+void tensor_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev) {
+    const int nb = c->tm.nb;
+    dev.assign(size_t(c->n_dof()), 0.0);
+    for (int64_t e = 0; e < c->n_elem; ++e)
+        for (int i = 0; i < nb; ++i) dev[size_t(e) * nb + c->tm.perm[i]] = host[size_t(e) * nb + i];
+}
+
+void tensor_from_device(hgdg_ctx* c, const std::vector<double>& dev, double* host) {
+    const int nb = c->tm.nb;
+    for (int64_t e = 0; e < c->n_elem; ++e)
+        for (int i = 0; i < nb; ++i) host[size_t(e) * nb + i] = dev[size_t(e) * nb + c->tm.perm[i]];
+}
+
+}  // namespace hgdg

@@ -44,6 +44,16 @@ def test_tensor_basis_matches_oracle_order_and_values(shape, dim, p):
         np.testing.assert_allclose(v_lib, v_orc, rtol=0, atol=1e-14)
 
 
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_tet_basis_matches_oracle_order_and_values(p):
+    rng = np.random.default_rng(10 + p)
+    for _ in range(6):
+        x = rng.dirichlet([1, 1, 1, 1])[:3]
+        v_lib = hpgem_amd.basis_eval(orc.TETRAHEDRON, p, x)
+        v_orc, _ = orc.basis_eval(orc.TETRAHEDRON, p, x)
+        np.testing.assert_allclose(v_lib, v_orc, rtol=0, atol=1e-14)
+
+
 @pytest.mark.parametrize("n,periodic,simplices", [
     ([4, 3], [1, 1], False), ([3, 4, 5], [1, 0, 1], False), ([2, 3, 2], [0, 0, 0], True),
     ([3, 3, 3], [1, 1, 1], True), ([16, 16], [1, 1], False)])
