@@ -48,6 +48,7 @@ SIGNATURES = [
     ("hgdg_rk_steps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _INT, ctypes.c_int, _D, _D,
                                      ctypes.c_double, ctypes.c_int]),
     ("hgdg_project_modes", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, _D]),
+    ("hgdg_project_field_modes", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D]),
     ("hgdg_state_diff", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, _D]),
     ("hgdg_copy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ("hgdg_sync", ctypes.c_int, [ctypes.c_void_p]),

@@ -124,13 +124,14 @@ class HpgemAPIDevice:
     def setStorage(self, vec, precision):
         check(self.lib.hgdg_state_set_storage(self.h, vec, PRECISIONS[precision]))
 
-    def setInitialSolutionModes(self, amp, k, vec=None):
-        """L2 projection of sum_m amp_m sin(2 pi k_m . x) (setInitialSolution)."""
+    def setInitialSolutionModes(self, amp, k, vec=None, field=-1):
+        """L2 projection of sum_m amp_m sin(2 pi k_m . x) (setInitialSolution),
+        into every field (field=-1) or into one field."""
         amp = np.ascontiguousarray(amp, dtype=np.float64)
         kk = np.zeros((len(amp), 3))
         kk[:, : np.asarray(k).shape[1]] = k
-        check(self.lib.hgdg_project_modes(self.h, self.solutionVectorId_ if vec is None else vec, len(amp),
-                                          dptr(amp), dptr(kk)))
+        check(self.lib.hgdg_project_field_modes(self.h, self.solutionVectorId_ if vec is None else vec, field,
+                                                len(amp), dptr(amp), dptr(kk)))
 
     # ---- the hot path ------------------------------------------------------
     def computeTimeDerivative(self, inputVectorIds, coefficientsInputVectors, resultVectorId, time=0.0):

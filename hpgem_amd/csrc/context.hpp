@@ -89,7 +89,8 @@ void tensor_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m);
 void tensor_operator_set(hgdg_ctx* c, const hgdg_op_desc* op);
 void tensor_rhs(hgdg_ctx* c, const StageIO& io);
 void tensor_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* l2, double* max_abs);
-void tensor_project_modes(hgdg_ctx* c, int n_modes, const double* amp, const double* k, std::vector<double>& dev);
+void tensor_project_modes(hgdg_ctx* c, int field, int n_modes, const double* amp, const double* k,
+                          std::vector<double>& dev);
 void tensor_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev);
 void tensor_from_device(hgdg_ctx* c, const std::vector<double>& dev, double* host);
 

@@ -417,15 +417,21 @@ int hgdg_state_diff(hgdg_ctx* ctx, int x, int y, double* l2, double* max_abs) {
 }
 
 int hgdg_project_modes(hgdg_ctx* ctx, int vec, int n_modes, const double* amp, const double* k) {
+    return hgdg_project_field_modes(ctx, vec, -1, n_modes, amp, k);
+}
+
+int hgdg_project_field_modes(hgdg_ctx* ctx, int vec, int field, int n_modes, const double* amp, const double* k) {
     return run([&] {
         require(ctx != nullptr && (n_modes == 0 || (amp && k)), HGDG_ERR_ARG, "NULL argument");
         require(n_modes >= 0, HGDG_ERR_ARG, "negative mode count");
         HGDG_HIP(hipSetDevice(ctx->device));
         require(ctx->has_mesh, HGDG_ERR_STATE, "no mesh");
         require(!ctx->simplex(), HGDG_ERR_UNSUPPORTED, "sine-mode projection is available on tensor elements");
+        require(field >= -1 && field < ctx->n_fields, HGDG_ERR_ARG, "field out of range");
         hgdg::Vec& v = hgdg::vec_at(ctx, vec);
         std::vector<double> dev;
-        hgdg::tensor_project_modes(ctx, n_modes, amp, k, dev);
+        if (field >= 0) hgdg::download_vector(ctx, v, dev);
+        hgdg::tensor_project_modes(ctx, field, n_modes, amp, k, dev);
         hgdg::upload_vector(ctx, v, dev);
     });
 }

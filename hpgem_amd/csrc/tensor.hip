@@ -238,6 +238,208 @@ __global__ __launch_bounds__(kBlock) void tensor_advection_rhs(const RhsParams p
     }
 }
 
+// Acoustic wave system on tensor elements (AcousticWaveLinear_Impl.h:102-283):
+// fields v, s_1..s_dim stored field-major per element. With g_r = sum_d
+// (J^{-1})_{r,d} s_d,
+//   k_v   = sum_r (Dr along r) g_r + sum_faces Mi(:,s) (x) C_v
+//   k_s_d = (1/c_inv) [ sum_r (J^{-1})_{r,d} (Dr along r) v + sum_faces Mi(:,s) (x) C_d ]
+// where, on face (d_f, s) with w_k = (2s-1) (J^{-1})_{d_f,k} (= outward normal
+// times face area over element volume), C_v = -1/2 w.(s_own - s_nbr) and
+// C_d = -1/2 w_d (v_own - v_nbr) on internal faces, C_v = -w.s_own and C_d = 0
+// on reflecting walls. Jumps are formed in FP64.
+struct AcousticParams {
+    int64_t n_elem;
+    StageIO io;
+    const double* jinv;   // [dim*dim][n_elem]: (J^{-1})_{r,d} at [(r dim + d) n_elem + e]
+    const double* cinv;   // [n_elem]
+    const int32_t* nbr;   // [2 dim][n_elem], -1 on walls
+    const double* Dr;
+    const double* Mi;
+};
+
+template <int DIM, int N1, typename T>
+__global__ __launch_bounds__(kBlock) void tensor_acoustic_rhs(const AcousticParams p) {
+    using S = Pencil<DIM, N1>;
+    constexpr int NF = DIM + 1;
+    constexpr int NB = S::NB;
+    __shared__ double s_w[S::EPB * NF * NB];
+    __shared__ T s_D[N1 * N1];
+    __shared__ T s_Mi[2 * N1];
+    const int tid = threadIdx.x;
+    if (tid < N1 * N1) s_D[tid] = T(p.Dr[tid]);
+    if (tid < 2 * N1) s_Mi[tid] = T(p.Mi[(tid % N1) * N1 + tid / N1]);
+    const int le = tid / S::NP;
+    const int pc = tid % S::NP;
+    const int64_t e = int64_t(blockIdx.x) * S::EPB + le;
+    const bool active = (le < S::EPB) && (e < p.n_elem);
+    const int ib = pc % N1;
+    const int ic = DIM == 3 ? pc / N1 : 0;
+    double* sw = s_w + le * NF * NB;
+    const int64_t base = e * NF * NB;
+    double w[NF][N1];
+    if (active) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            combine_pencil<N1>(p.io, base + f * NB + pc * N1, w[f]);
+#pragma unroll
+            for (int a = 0; a < N1; ++a) sw[f * NB + pc * N1 + a] = w[f][a];
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+
+    double Ji[DIM][DIM];
+#pragma unroll
+    for (int r = 0; r < DIM; ++r)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) Ji[r][d] = p.jinv[(r * DIM + d) * p.n_elem + e];
+    const double cinv = p.cinv[e];
+
+    // g_r of a pencil from its s values
+    auto g_of = [&](const double* sv_base, int r, int a) {
+        double g = 0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) g += Ji[r][d] * sv_base[(1 + d) * NB + a];
+        return g;
+    };
+    T kv[N1], t[DIM][N1];
+#pragma unroll
+    for (int a = 0; a < N1; ++a) {
+        kv[a] = T(0);
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) t[r][a] = T(0);
+    }
+    // x direction (own pencil, registers)
+#pragma unroll
+    for (int a = 0; a < N1; ++a) {
+        T sg = T(0), sv = T(0);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) {
+            double g0 = 0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) g0 += Ji[0][d] * w[1 + d][j];
+            sg += s_D[a * N1 + j] * T(g0);
+            sv += s_D[a * N1 + j] * T(w[0][j]);
+        }
+        kv[a] += sg;
+        t[0][a] += sv;
+    }
+    // y (and z) directions from the element's other pencils
+#pragma unroll
+    for (int r = 1; r < DIM; ++r) {
+        const int my = r == 1 ? ib : ic;
+#pragma unroll
+        for (int j = 0; j < N1; ++j) {
+            const int opc = r == 1 ? j + N1 * ic : ib + N1 * j;
+            const double* ob = sw + opc * N1;
+            const T dr = s_D[my * N1 + j];
+#pragma unroll
+            for (int a = 0; a < N1; ++a) {
+                kv[a] += dr * T(g_of(ob, r, a));
+                t[r][a] += dr * T(ob[a]);
+            }
+        }
+    }
+    Acc<T> av[N1], as[DIM][N1];
+#pragma unroll
+    for (int a = 0; a < N1; ++a) {
+        av[a] = Acc<T>(kv[a]);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            T v = T(0);
+#pragma unroll
+            for (int r = 0; r < DIM; ++r) v += T(Ji[r][d]) * t[r][a];
+            as[d][a] = Acc<T>(v);
+        }
+    }
+
+    // faces
+#pragma unroll
+    for (int df = 0; df < DIM; ++df) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int f = 2 * df + s;
+            const int64_t nb = p.nbr[f * p.n_elem + e];
+            double wk[DIM];
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) wk[k] = (2 * s - 1) * Ji[df][k];
+            if (df == 0) {
+                // one trace value per field for this pencil
+                double own[NF], oth[NF];
+#pragma unroll
+                for (int q = 0; q < NF; ++q) own[q] = w[q][s];
+                if (nb >= 0) {
+#pragma unroll
+                    for (int q = 0; q < NF; ++q) oth[q] = combine(p.io, nb * NF * NB + q * NB + pc * N1 + (1 - s));
+                }
+                double cv = 0, cd = 0;
+                if (nb >= 0) {
+#pragma unroll
+                    for (int k = 0; k < DIM; ++k) cv += wk[k] * (own[1 + k] - oth[1 + k]);
+                    cv *= -0.5;
+                    cd = -0.5 * (own[0] - oth[0]);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < DIM; ++k) cv -= wk[k] * own[1 + k];
+                }
+#pragma unroll
+                for (int a = 0; a < N1; ++a) {
+                    av[a].add(s_Mi[s * N1 + a] * T(cv));
+                    if (nb >= 0)
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) as[d][a].add(s_Mi[s * N1 + a] * T(wk[d] * cd));
+                }
+            } else {
+                const int own_pc = df == 1 ? s + N1 * ic : ib + N1 * s;
+                const int nbr_pc = df == 1 ? (1 - s) + N1 * ic : ib + N1 * (1 - s);
+                const int my = df == 1 ? ib : ic;
+                double oth[NF][N1];
+                if (nb >= 0) {
+#pragma unroll
+                    for (int q = 0; q < NF; ++q) combine_pencil<N1>(p.io, nb * NF * NB + q * NB + nbr_pc * N1, oth[q]);
+                }
+                const T m = s_Mi[s * N1 + my];
+#pragma unroll
+                for (int a = 0; a < N1; ++a) {
+                    double cv = 0, cd = 0;
+                    if (nb >= 0) {
+#pragma unroll
+                        for (int k = 0; k < DIM; ++k) cv += wk[k] * (sw[(1 + k) * NB + own_pc * N1 + a] - oth[1 + k][a]);
+                        cv *= -0.5;
+                        cd = -0.5 * (sw[own_pc * N1 + a] - oth[0][a]);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < DIM; ++k) cv -= wk[k] * sw[(1 + k) * NB + own_pc * N1 + a];
+                    }
+                    av[a].add(m * T(cv));
+                    if (nb >= 0)
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) as[d][a].add(m * T(wk[d] * cd));
+                }
+            }
+        }
+    }
+    const int64_t o = base + pc * N1;
+    const T ci = T(1.0 / cinv);
+    if (p.io.out_f32) {
+        float* out = static_cast<float*>(p.io.out);
+#pragma unroll
+        for (int a = 0; a < N1; ++a) {
+            out[o + a] = float(av[a].value());
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) out[o + (1 + d) * NB + a] = float(ci * as[d][a].value());
+        }
+    } else {
+        double* out = static_cast<double*>(p.io.out);
+#pragma unroll
+        for (int a = 0; a < N1; ++a) {
+            out[o + a] = double(av[a].value());
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) out[o + (1 + d) * NB + a] = double(ci * as[d][a].value());
+        }
+    }
+}
+
 // Per-element mass-norm of x - y: |J| dx^T (M1 x M1 x M1) dx, and max |dx|;
 // one partial per block.
 struct DiffParams {
@@ -245,7 +447,8 @@ struct DiffParams {
     const void* x;
     const void* y;
     int32_t x_f32, y_f32;
-    const double* absdet;  // [n_elem]
+    const double* absdet;  // [n_elem / fields]
+    int32_t fields;        // consecutive blocks of one element
     const double* M1;      // [n1][n1]
     double* partial;       // [grid][2]
 };
@@ -291,7 +494,7 @@ __global__ __launch_bounds__(kBlock) void tensor_diff_norm(const DiffParams p) {
     }
     double v = 0, m = 0;
     if (active) {
-        v = dx * src[le * S::NB + dof] * p.absdet[e];
+        v = dx * src[le * S::NB + dof] * p.absdet[e / p.fields];
         m = fabs(dx);
     }
     s_red[0][tid] = v;
@@ -465,10 +668,24 @@ void tensor_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
 }
 
 void tensor_operator_set(hgdg_ctx* c, const hgdg_op_desc* op) {
-    require(op->kind == HGDG_OP_ADVECTION_UPWIND, HGDG_ERR_UNSUPPORTED,
-            "the acoustic operator is not available on tensor elements");
     const int dim = c->dim;
     const int64_t ne = c->n_elem;
+    if (op->kind == HGDG_OP_ACOUSTIC_CENTRAL) {
+        std::vector<double> geo(size_t(dim) * dim * ne + ne);
+        for (int64_t e = 0; e < ne; ++e) {
+            for (int r = 0; r < dim; ++r)
+                for (int d = 0; d < dim; ++d)
+                    geo[(size_t(r) * dim + d) * ne + e] = c->tm.Jinv[(size_t(e) * dim + r) * dim + d];
+            geo[size_t(dim) * dim * ne + e] = c->cinv[e];
+        }
+        c->tm.d_geo.upload(geo, c->stream);
+        c->n_fields = dim + 1;
+        c->dof = ne * c->tm.nb * c->n_fields;
+        HGDG_HIP(hipStreamSynchronize(c->stream));
+        return;
+    }
+    c->n_fields = 1;
+    c->dof = ne * c->tm.nb;
     std::vector<double> ahat(size_t(dim) * ne);
     std::vector<double> rownorm(size_t(dim) * ne);
     for (int64_t e = 0; e < ne; ++e)
@@ -515,6 +732,30 @@ void tensor_operator_set(hgdg_ctx* c, const hgdg_op_desc* op) {
 }
 
 void tensor_rhs(hgdg_ctx* c, const StageIO& io) {
+    const bool mixed = c->op.precision == HGDG_PREC_MIXED_FP32;
+    if (c->op.kind == HGDG_OP_ACOUSTIC_CENTRAL) {
+        AcousticParams a{};
+        a.n_elem = c->n_elem;
+        a.io = io;
+        a.jinv = c->tm.d_geo.ptr;
+        a.cinv = c->tm.d_geo.ptr + size_t(c->dim) * c->dim * c->n_elem;
+        a.nbr = c->tm.d_nbr.ptr;
+        a.Dr = Dr_ptr(c);
+        a.Mi = Mi_ptr(c);
+        dispatch_tensor(c->dim, c->tm.n1, [&](auto D, auto N) {
+            constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
+            using S = Pencil<DIMc, N1c>;
+            const int64_t grid = (c->n_elem + S::EPB - 1) / S::EPB;
+            if (mixed)
+                hipLaunchKernelGGL((tensor_acoustic_rhs<DIMc, N1c, float>), dim3(unsigned(grid)), dim3(kBlock), 0,
+                                   c->stream, a);
+            else
+                hipLaunchKernelGGL((tensor_acoustic_rhs<DIMc, N1c, double>), dim3(unsigned(grid)), dim3(kBlock), 0,
+                                   c->stream, a);
+        });
+        HGDG_HIP(hipGetLastError());
+        return;
+    }
     RhsParams p{};
     p.n_elem = c->n_elem;
     p.io = io;
@@ -523,7 +764,6 @@ void tensor_rhs(hgdg_ctx* c, const StageIO& io) {
     p.mode = c->tm.d_mode.ptr;
     p.Dr = Dr_ptr(c);
     p.Mi = Mi_ptr(c);
-    const bool mixed = c->op.precision == HGDG_PREC_MIXED_FP32;
     dispatch_tensor(c->dim, c->tm.n1, [&](auto D, auto N) {
         constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
         using S = Pencil<DIMc, N1c>;
@@ -540,7 +780,8 @@ void tensor_rhs(hgdg_ctx* c, const StageIO& io) {
 
 void tensor_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, double* max_abs) {
     DiffParams p{};
-    p.n_elem = c->n_elem;
+    p.n_elem = c->n_elem * c->n_fields;
+    p.fields = c->n_fields;
     p.x = vx.ptr();
     p.y = vy.ptr();
     p.x_f32 = vx.is_f32;
@@ -551,7 +792,7 @@ void tensor_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, do
     dispatch_tensor(c->dim, c->tm.n1, [&](auto D, auto N) {
         constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
         using S = Tensor<DIMc, N1c>;
-        grid = (c->n_elem + S::EPB - 1) / S::EPB;
+        grid = (p.n_elem + S::EPB - 1) / S::EPB;
         if (c->d_partial.n < size_t(2 * grid)) c->d_partial.alloc(size_t(2 * grid));
         p.partial = c->d_partial.ptr;
         hipLaunchKernelGGL((tensor_diff_norm<DIMc, N1c>), dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
@@ -575,7 +816,8 @@ void tensor_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, do
 // the element rule of order 2p+1 (Element.cpp:89-98): on an affine tensor
 // element the coefficients are (Mi B^T W)^{(x) dim} applied to f at the
 // Gauss points, B(q, a) = psi_a(x_q). Result in device layout.
-void tensor_project_modes(hgdg_ctx* c, int n_modes, const double* amp, const double* kv, std::vector<double>& out) {
+void tensor_project_modes(hgdg_ctx* c, int field, int n_modes, const double* amp, const double* kv,
+                          std::vector<double>& out) {
     const int n1 = c->tm.n1, dim = c->dim, nb = c->tm.nb;
     std::vector<double> xq, wq;
     gauss_legendre(n1, xq, wq);
@@ -588,7 +830,8 @@ void tensor_project_modes(hgdg_ctx* c, int n_modes, const double* amp, const dou
             P[a * n1 + q] = s * wq[q];
         }
     const int nq = nb;  // n1^dim points
-    out.assign(size_t(c->n_dof()), 0.0);
+    const int nf = c->n_fields;
+    if (out.size() != size_t(c->n_dof())) out.assign(size_t(c->n_dof()), 0.0);
     std::vector<double> F(nq), T1(nq), T2(nq);
     for (int64_t e = 0; e < c->n_elem; ++e) {
         const double* X0 = c->tm.X0.data() + size_t(e) * dim;
@@ -626,21 +869,24 @@ void tensor_project_modes(hgdg_ctx* c, int n_modes, const double* amp, const dou
             dst = (dst == &T1) ? &T2 : &T1;
             stride *= n1;
         }
-        std::copy(src->begin(), src->end(), out.begin() + size_t(e) * nb);
+        for (int f = 0; f < nf; ++f)
+            if (field < 0 || field == f) std::copy(src->begin(), src->end(), out.begin() + (size_t(e) * nf + f) * nb);
     }
 }
 
 void tensor_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev) {
     const int nb = c->tm.nb;
+    const int64_t blocks = c->n_elem * c->n_fields;  // field-major blocks of nb per element
     dev.assign(size_t(c->n_dof()), 0.0);
-    for (int64_t e = 0; e < c->n_elem; ++e)
-        for (int i = 0; i < nb; ++i) dev[size_t(e) * nb + c->tm.perm[i]] = host[size_t(e) * nb + i];
+    for (int64_t b = 0; b < blocks; ++b)
+        for (int i = 0; i < nb; ++i) dev[size_t(b) * nb + c->tm.perm[i]] = host[size_t(b) * nb + i];
 }
 
 void tensor_from_device(hgdg_ctx* c, const std::vector<double>& dev, double* host) {
     const int nb = c->tm.nb;
-    for (int64_t e = 0; e < c->n_elem; ++e)
-        for (int i = 0; i < nb; ++i) host[size_t(e) * nb + i] = dev[size_t(e) * nb + c->tm.perm[i]];
+    const int64_t blocks = c->n_elem * c->n_fields;
+    for (int64_t b = 0; b < blocks; ++b)
+        for (int i = 0; i < nb; ++i) host[size_t(b) * nb + i] = dev[size_t(b) * nb + c->tm.perm[i]];
 }
 
 }  // namespace hgdg

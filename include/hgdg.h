@@ -149,6 +149,10 @@ int hgdg_rk_steps(hgdg_ctx* ctx, int solution, const int* aux_ids, int stages, c
  * initial conditions. */
 int hgdg_project_modes(hgdg_ctx* ctx, int vec, int n_modes, const double* amp, const double* k);
 
+/* The same projection into one field of a multi-field operator (field = -1:
+ * every field); the other fields of the vector are left as they are. */
+int hgdg_project_field_modes(hgdg_ctx* ctx, int vec, int field, int n_modes, const double* amp, const double* k);
+
 /* L2 norm of (vec[x] - vec[y]) under the element mass matrices, and the max
  * abs coefficient difference (FP64, on the device). */
 int hgdg_state_diff(hgdg_ctx* ctx, int x, int y, double* l2, double* max_abs);

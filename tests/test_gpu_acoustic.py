@@ -103,3 +103,54 @@ def test_acoustic_energy_is_conserved_on_gpu():
         off += nb4
     assert abs(rate) < 1e-12 * scale
     dev.close()
+
+
+TENSOR_CASES = [([4, 4], 2, True), ([4, 3], 3, False), ([3, 3, 3], 1, True), ([3, 3, 3], 2, False),
+                ([3, 4, 3], 3, True), ([3, 3, 3], 4, False)]
+
+
+@pytest.mark.parametrize("n,p,periodic", TENSOR_CASES)
+def test_tensor_acoustic_rhs_matches_oracle(n, p, periodic):
+    """The same acoustic operator on squares and cubes (walls on the
+    non-periodic boxes)."""
+    print(f"[case] tensor acoustic n={n} p={p} periodic={periodic}", flush=True)
+    mesh = Mesh.structured(n, periodic=[int(periodic)] * len(n))
+    rng = np.random.default_rng(p + len(n))
+    cinv = rng.uniform(0.25, 4.0, size=mesh.n_elements)
+    dev = HpgemAPIDevice(mesh, p, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=RK4, n_extra_vectors=2)
+    nf = len(n) + 1
+    ref = orc.Problem(mesh.shape, mesh.coords, mesh.node_ids, p, n_fields=nf)
+    ref.set_acoustic(cinv)
+    ref_ld = orc.Problem(mesh.shape, mesh.coords, mesh.node_ids, p, n_fields=nf, extended=True)
+    ref_ld.set_acoustic(cinv)
+    assert dev.n_dof == ref.n_dof
+    u = rng.standard_normal(dev.n_dof)
+    k = rng.standard_normal(dev.n_dof)
+    dev.setTimeIntegrationVector(0, u)
+    dev.setTimeIntegrationVector(1, k)
+    dev.computeTimeDerivative([0, 1], [1.0, 0.3], 2)
+    assert_parity(dev.getTimeIntegrationVector(2), ref.time_derivative([u, k], [1.0, 0.3]),
+                  ref_ld.time_derivative([u, k], [1.0, 0.3]))
+    dev.close()
+
+
+def test_tensor_acoustic_mixed_within_1e6():
+    mesh = Mesh.structured([6, 6, 6])
+    rng = np.random.default_rng(2)
+    cinv = rng.uniform(0.25, 4.0, size=mesh.n_elements)
+    runs = {}
+    for prec in ("fp64", "mixed"):
+        dev = HpgemAPIDevice(mesh, 3, precision=prec, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=RK4,
+                             n_extra_vectors=2)
+        dev.setInitialSolutionModes([0.7, -0.4], [[1, 2, 1], [2, 1, 3]], field=0)
+        dev.computeOneTimeStep(0.0, 2e-3, 20)
+        runs[prec] = (dev, dev.getTimeIntegrationVector(0))
+    dev64, u64 = runs["fp64"]
+    dev64.setTimeIntegrationVector(5, runs["mixed"][1])
+    l2d, _ = dev64.difference(0, 5)
+    dev64.setTimeIntegrationVector(6, np.zeros_like(u64))
+    l2, _ = dev64.difference(0, 6)
+    print(f"acoustic mixed vs fp64 rel L2 {l2d / l2:.3e}")
+    assert l2d / l2 < 1e-6
+    for d, _ in runs.values():
+        d.close()
