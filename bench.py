@@ -182,6 +182,26 @@ def run_profile_workload(steps):
     print(json.dumps({"profile_workload": "done", "steps": steps}))
 
 
+def run_serving(precision, n_requests, seed=SEED):
+    """The serving stream: n_requests independent solves (2D/3D, advection or
+    acoustic, periodic boxes with side log-uniform in [8, 64], p in 1..4,
+    200 RK4 steps, seeded sine data) arriving with Poisson inter-arrival
+    times of mean 5 ms; continuous batching with HBM-footprint admission."""
+    from hpgem_amd.serving import Server, run_stream, serving_stream
+
+    reqs = serving_stream(n_requests=n_requests, seed=seed)
+    srv = Server(precision=precision, hbm_budget_gb=96.0, max_active=128)
+    t0 = time.perf_counter()
+    run_stream(srv, reqs)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return {"requests": n_requests, "completed": st["completed"], "failed": st["failed"],
+            "latency_p50_s": st["latency_p50_s"], "latency_p99_s": st["latency_p99_s"],
+            "latency_mean_s": st["latency_mean_s"], "gdof_rhs_per_s": st["dof_rhs_per_s"] / 1e9,
+            "wall_s": wall, "peak_hbm_gb": st["peak_bytes"] / 2 ** 30}
+
+
 def run_cpu_baseline(target_seconds=15.0):
     """FP64 oracle (HpgemAPILinear restatement: dense element/face mat-vecs and
     a per-element LU mass solve per call) on a bounded sample: the same
@@ -220,6 +240,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--serving-requests", type=int, default=512,
+                    help="requests in the serving stream (0 skips it)")
     ap.add_argument("--profile-workload", action="store_true",
                     help="launch only the RHS and copy kernels (for rocprofv3 counter runs)")
     args = ap.parse_args()
@@ -233,6 +255,9 @@ def main():
         run_profile_workload(args.steps)
         return
     gpu, info = run_gpu(args.steps, args.warmup)
+    serving = None
+    if args.serving_requests > 0:
+        serving = {prec: run_serving(prec, args.serving_requests) for prec in ("fp64", "mixed")}
     cpu = None if args.no_cpu else run_cpu_baseline(args.cpu_seconds)
 
     mixed_ok = gpu["mixed"]["rel_l2_err_vs_fp64"] <= 1e-6
@@ -262,6 +287,7 @@ def main():
         "cpu_baseline": cpu,
         "fp64": gpu["fp64"],
         "mixed": gpu["mixed"],
+        "serving": serving,
     }
     if cpu:
         out["speedup_vs_cpu"] = {"fp64": gpu["fp64"]["gdofs"] / cpu["value"],

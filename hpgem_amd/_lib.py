@@ -92,3 +92,28 @@ def check(rc):
 
 def dptr(a):
     return a.ctypes.data_as(_D)
+
+
+class SolveDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("kind", ctypes.c_int32), ("order", ctypes.c_int32),
+                ("cells", ctypes.c_int32 * 3), ("velocity", ctypes.c_double * 3), ("wave_speed", ctypes.c_double),
+                ("n_modes", ctypes.c_int32), ("amp", _D), ("k", _D), ("dt", ctypes.c_double),
+                ("n_steps", ctypes.c_int32), ("keep_state", ctypes.c_int32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("failed", ctypes.c_int64),
+                ("active", ctypes.c_int64), ("latency_p50_s", ctypes.c_double), ("latency_p99_s", ctypes.c_double),
+                ("latency_mean_s", ctypes.c_double), ("dof_rhs_per_s", ctypes.c_double), ("wall_s", ctypes.c_double),
+                ("peak_bytes", ctypes.c_int64)]
+
+
+SIGNATURES += [
+    ("hgdg_server_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int64, ctypes.c_int]),
+    ("hgdg_server_submit", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveDesc), ctypes.POINTER(ctypes.c_int64)]),
+    ("hgdg_server_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]),
+    ("hgdg_server_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), _D, _D, _D]),
+    ("hgdg_server_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
+    ("hgdg_server_destroy", ctypes.c_int, [ctypes.c_void_p]),
+]

@@ -25,6 +25,8 @@ struct TensorMesh {
     DevBuf<double> d_absdet, d_ops;  // d_ops: Dr, Mi, M1 (n1 x n1 each)
     DevBuf<double> d_geo;            // advection: ahat [dim][e]; acoustic: J^{-1} [dim*dim][e], then c_inv [e]
     DevBuf<uint32_t> d_mode;         // advection: upwind mode, 3 bits per face
+    bool structured = false;         // periodic unit box built by tensor_structured_upload
+    int32_t cells[3] = {1, 1, 1};
 };
 
 // Tetrahedra with per-element order 1..4, grouped by order on the device.
@@ -86,13 +88,37 @@ Vec& vec_at(hgdg_ctx* c, int id);
 
 // tensor.hip
 void tensor_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m);
+void tensor_structured_upload(hgdg_ctx* c, int dim, const int32_t* n, int p);
+void tensor_project_device(hgdg_ctx* c, Vec& v, int field, int n_modes, const double* amp, const double* k);
 void tensor_operator_set(hgdg_ctx* c, const hgdg_op_desc* op);
 void tensor_rhs(hgdg_ctx* c, const StageIO& io);
 void tensor_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* l2, double* max_abs);
-void tensor_project_modes(hgdg_ctx* c, int field, int n_modes, const double* amp, const double* k,
-                          std::vector<double>& dev);
 void tensor_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev);
 void tensor_from_device(hgdg_ctx* c, const std::vector<double>& dev, double* host);
+
+// batched tensor launches (serving)
+size_t tensor_batch_param_bytes(const hgdg_ctx* c);
+int64_t tensor_batch_blocks(const hgdg_ctx* c);
+void tensor_batch_param(hgdg_ctx* c, const StageIO& io, void* out);
+void tensor_batch_launch(hipStream_t stream, int kind, int dim, int n1, bool mixed, const void* params,
+                         const int64_t* off, int n, int64_t blocks);
+
+// hgdg.hip
+void set_storage(hgdg_ctx* c, Vec& v, bool f32);
+void mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m);
+void operator_set(hgdg_ctx* c, const hgdg_op_desc* op);
+void upload_vector(hgdg_ctx* c, Vec& v, const std::vector<double>& dev);
+void download_vector(hgdg_ctx* c, Vec& v, std::vector<double>& dev);
+void state_diff(hgdg_ctx* c, int x, int y, double* l2, double* max_abs);
+struct AxpyItem {
+    int64_t n;
+    double* y;
+    const void* x[kMaxIn];
+    double c[kMaxIn];
+    uint32_t x_f32;
+    int32_t n_x;
+};
+void axpy_batch_launch(hipStream_t stream, const AxpyItem* items_dev, const int64_t* off, int n, int64_t blocks);
 
 // simplex.hip
 void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m);

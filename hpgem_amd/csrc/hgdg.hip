@@ -33,6 +33,36 @@ struct AxpyParams {
     int32_t n_x;
 };
 
+// Batched RK updates of many problems: block b belongs to item s with
+// off[s] <= b < off[s+1]; each item is a multi_axpy.
+__global__ __launch_bounds__(kBlock) void multi_axpy_batched(const AxpyItem* items, const int64_t* off, int n) {
+    int lo = 0, hi = n - 1;
+    const int64_t b = blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (off[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const AxpyItem& p = items[lo];
+    const int64_t i = (b - off[lo]) * kBlock + threadIdx.x;
+    if (i >= p.n) return;
+    double y = p.y[i];
+#pragma unroll
+    for (int j = 0; j < kMaxIn; ++j) {
+        if (j >= p.n_x) break;
+        const double x = ((p.x_f32 >> j) & 1u) ? double(static_cast<const float*>(p.x[j])[i])
+                                                : static_cast<const double*>(p.x[j])[i];
+        y += p.c[j] * x;
+    }
+    p.y[i] = y;
+}
+
+void axpy_batch_launch(hipStream_t stream, const AxpyItem* items_dev, const int64_t* off, int n, int64_t blocks) {
+    if (n == 0 || blocks == 0) return;
+    hipLaunchKernelGGL(multi_axpy_batched, dim3(unsigned(blocks)), dim3(kBlock), 0, stream, items_dev, off, n);
+    HGDG_HIP(hipGetLastError());
+}
+
 __global__ __launch_bounds__(kBlock) void multi_axpy(const AxpyParams p) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= p.n) return;
@@ -429,10 +459,7 @@ int hgdg_project_field_modes(hgdg_ctx* ctx, int vec, int field, int n_modes, con
         require(!ctx->simplex(), HGDG_ERR_UNSUPPORTED, "sine-mode projection is available on tensor elements");
         require(field >= -1 && field < ctx->n_fields, HGDG_ERR_ARG, "field out of range");
         hgdg::Vec& v = hgdg::vec_at(ctx, vec);
-        std::vector<double> dev;
-        if (field >= 0) hgdg::download_vector(ctx, v, dev);
-        hgdg::tensor_project_modes(ctx, field, n_modes, amp, k, dev);
-        hgdg::upload_vector(ctx, v, dev);
+        hgdg::tensor_project_device(ctx, v, field, n_modes, amp, k);
     });
 }
 

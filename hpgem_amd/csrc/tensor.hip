@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstring>
 #include <string>
 #include <unordered_map>
 
@@ -123,7 +124,7 @@ __device__ __forceinline__ double face_jump(uint32_t m, double own, double other
 }
 
 template <int DIM, int N1, typename T>
-__global__ __launch_bounds__(kBlock) void tensor_advection_rhs(const RhsParams p) {
+__device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk) {
     using S = Pencil<DIM, N1>;
     __shared__ double s_w[S::EPB * S::NB];  // FP64 stage input of the block's elements
     __shared__ T s_D[N1 * N1];
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void tensor_advection_rhs(const RhsParams p
     if (tid < 2 * N1) s_Mi[tid] = T(p.Mi[(tid % N1) * N1 + tid / N1]);
     const int le = tid / S::NP;
     const int pc = tid % S::NP;
-    const int64_t e = int64_t(blockIdx.x) * S::EPB + le;
+    const int64_t e = blk * S::EPB + le;
     const bool active = (le < S::EPB) && (e < p.n_elem);
     const int ib = pc % N1;                    // y index of this pencil
     const int ic = DIM == 3 ? pc / N1 : 0;     // z index of this pencil
@@ -238,6 +239,31 @@ __global__ __launch_bounds__(kBlock) void tensor_advection_rhs(const RhsParams p
     }
 }
 
+template <int DIM, int N1, typename T>
+__global__ __launch_bounds__(kBlock) void tensor_advection_rhs(const RhsParams p) {
+    advection_block<DIM, N1, T>(p, blockIdx.x);
+}
+
+// Block b of a batched launch belongs to problem s with off[s] <= b < off[s+1].
+__device__ __forceinline__ int find_problem(const int64_t* off, int n, int64_t b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (off[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Many independent problems of one kind in one launch: every block runs the
+// exact code of the single-problem kernel on its own problem's data.
+template <int DIM, int N1, typename T>
+__global__ __launch_bounds__(kBlock) void tensor_advection_rhs_batched(const RhsParams* ps, const int64_t* off,
+                                                                       int n) {
+    const int s = find_problem(off, n, blockIdx.x);
+    advection_block<DIM, N1, T>(ps[s], int64_t(blockIdx.x) - off[s]);
+}
+
 // Acoustic wave system on tensor elements (AcousticWaveLinear_Impl.h:102-283):
 // fields v, s_1..s_dim stored field-major per element. With g_r = sum_d
 // (J^{-1})_{r,d} s_d,
@@ -258,7 +284,7 @@ struct AcousticParams {
 };
 
 template <int DIM, int N1, typename T>
-__global__ __launch_bounds__(kBlock) void tensor_acoustic_rhs(const AcousticParams p) {
+__device__ __forceinline__ void acoustic_block(const AcousticParams& p, int64_t blk) {
     using S = Pencil<DIM, N1>;
     constexpr int NF = DIM + 1;
     constexpr int NB = S::NB;
@@ -270,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void tensor_acoustic_rhs(const AcousticPara
     if (tid < 2 * N1) s_Mi[tid] = T(p.Mi[(tid % N1) * N1 + tid / N1]);
     const int le = tid / S::NP;
     const int pc = tid % S::NP;
-    const int64_t e = int64_t(blockIdx.x) * S::EPB + le;
+    const int64_t e = blk * S::EPB + le;
     const bool active = (le < S::EPB) && (e < p.n_elem);
     const int ib = pc % N1;
     const int ic = DIM == 3 ? pc / N1 : 0;
@@ -440,6 +466,135 @@ __global__ __launch_bounds__(kBlock) void tensor_acoustic_rhs(const AcousticPara
     }
 }
 
+template <int DIM, int N1, typename T>
+__global__ __launch_bounds__(kBlock) void tensor_acoustic_rhs(const AcousticParams p) {
+    acoustic_block<DIM, N1, T>(p, blockIdx.x);
+}
+
+template <int DIM, int N1, typename T>
+__global__ __launch_bounds__(kBlock) void tensor_acoustic_rhs_batched(const AcousticParams* ps, const int64_t* off,
+                                                                      int n) {
+    const int s = find_problem(off, n, blockIdx.x);
+    acoustic_block<DIM, N1, T>(ps[s], int64_t(blockIdx.x) - off[s]);
+}
+
+// L2 projection of f(x) = sum_m amp_m sin(2 pi k_m . x) on affine tensor
+// elements (HpgemAPISimplified::setInitialSolution with the element rule of
+// order 2p+1): the coefficients are (Mi B^T W) applied along each direction
+// to f at the n1^dim Gauss points, P(a, q) = sum_b Mi(a, b) psi_b(x_q) w_q.
+// One thread per x-line of quadrature points; the y/z passes go through LDS.
+struct ProjParams {
+    int64_t n_elem;
+    int32_t nf, field;          // fields per element; target field (-1: all)
+    void* out;
+    int32_t out_f32;
+    int32_t n_modes;
+    const double* modes;        // [n_modes][4]: amp, k_x, k_y, k_z
+    const double* X0;           // [dim][n_elem] (generic meshes)
+    const double* E;            // [dim*dim][n_elem] (generic meshes)
+    int32_t structured;
+    int32_t cells[3];
+    const double* P;            // [n1][n1]
+    const double* xq;           // [n1]
+};
+
+template <int DIM, int N1>
+__global__ __launch_bounds__(kBlock) void tensor_project(const ProjParams p) {
+    using S = Pencil<DIM, N1>;
+    __shared__ double s_a[S::EPB * S::NB];
+    __shared__ double s_b[S::EPB * S::NB];
+    const int tid = threadIdx.x;
+    const int le = tid / S::NP;
+    const int pc = tid % S::NP;
+    const int64_t e = int64_t(blockIdx.x) * S::EPB + le;
+    const bool active = (le < S::EPB) && (e < p.n_elem);
+    const int i1 = pc % N1;
+    const int i2 = DIM == 3 ? pc / N1 : 0;
+    double X0[3] = {0, 0, 0}, E[3][3] = {{0}};
+    if (active) {
+        if (p.structured) {
+            int64_t idx[3] = {e % p.cells[0], (e / p.cells[0]) % p.cells[1],
+                              DIM == 3 ? e / (int64_t(p.cells[0]) * p.cells[1]) : 0};
+            for (int d = 0; d < DIM; ++d) {
+                X0[d] = 0.0 + double(idx[d]) * (1.0 - 0.0) / p.cells[d];
+                E[d][d] = (0.0 + double(idx[d] + 1) * (1.0 - 0.0) / p.cells[d]) - X0[d];
+            }
+        } else {
+            for (int d = 0; d < DIM; ++d) {
+                X0[d] = p.X0[d * p.n_elem + e];
+                for (int k = 0; k < DIM; ++k) E[d][k] = p.E[(d * DIM + k) * p.n_elem + e];
+            }
+        }
+    }
+    double* A = s_a + le * S::NB;
+    double* B = s_b + le * S::NB;
+    if (active) {
+        double f[N1];
+#pragma unroll
+        for (int q = 0; q < N1; ++q) {
+            const int qi[3] = {q, i1, i2};
+            double x[3] = {X0[0], X0[1], X0[2]};
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                const double t = (p.xq[qi[d]] + 1) / 2;
+#pragma unroll
+                for (int k = 0; k < DIM; ++k) x[k] += t * E[d][k];
+            }
+            double v = 0;
+            for (int m = 0; m < p.n_modes; ++m) {
+                double arg = 0;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) arg += p.modes[4 * m + 1 + d] * x[d];
+                v += p.modes[4 * m] * sin(2 * M_PI * arg);
+            }
+            f[q] = v;
+        }
+#pragma unroll
+        for (int a = 0; a < N1; ++a) {
+            double t = 0;
+#pragma unroll
+            for (int q = 0; q < N1; ++q) t += p.P[a * N1 + q] * f[q];
+            A[pc * N1 + a] = t;  // (a, q_y, q_z)
+        }
+    }
+    __syncthreads();
+    if (active) {  // y pass: this thread makes (., b = i1, q_z = i2)
+#pragma unroll
+        for (int a = 0; a < N1; ++a) {
+            double t = 0;
+#pragma unroll
+            for (int q = 0; q < N1; ++q) t += p.P[i1 * N1 + q] * A[(q + N1 * i2) * N1 + a];
+            B[pc * N1 + a] = t;
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    double coef[N1];
+    if (DIM == 3) {  // z pass: (., b, c) from (., b, q_z)
+#pragma unroll
+        for (int a = 0; a < N1; ++a) {
+            double t = 0;
+#pragma unroll
+            for (int q = 0; q < N1; ++q) t += p.P[i2 * N1 + q] * B[(i1 + N1 * q) * N1 + a];
+            coef[a] = t;
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < N1; ++a) coef[a] = B[pc * N1 + a];
+    }
+    for (int f = 0; f < p.nf; ++f) {
+        if (p.field >= 0 && f != p.field) continue;
+        const int64_t o = (e * p.nf + f) * S::NB + pc * N1;
+        if (p.out_f32) {
+#pragma unroll
+            for (int a = 0; a < N1; ++a) static_cast<float*>(p.out)[o + a] = float(coef[a]);
+        } else {
+#pragma unroll
+            for (int a = 0; a < N1; ++a) static_cast<double*>(p.out)[o + a] = coef[a];
+        }
+    }
+}
+
 // Per-element mass-norm of x - y: |J| dx^T (M1 x M1 x M1) dx, and max |dx|;
 // one partial per block.
 struct DiffParams {
@@ -530,6 +685,92 @@ const double* Mi_ptr(hgdg_ctx* c) { return c->tm.d_ops.ptr + c->tm.n1 * c->tm.n1
 const double* M1_ptr(hgdg_ctx* c) { return c->tm.d_ops.ptr + 2 * c->tm.n1 * c->tm.n1; }
 }  // namespace
 
+// J (column d = E_d / 2), |det J| and J^{-1} of affine element e from its edge
+// vectors E[d] = X_{2^d} - X_0; shared by the generic and the structured setup
+// so both produce the same numbers.
+void affine_geometry(hgdg_ctx* c, int64_t e, const double (&E)[3][3]) {
+    const int dim = c->dim;
+    double J[3][3] = {{0}};  // J[r][c] = d x_r / d xi_c
+    for (int r = 0; r < dim; ++r)
+        for (int col = 0; col < dim; ++col) J[r][col] = E[col][r] / 2;
+    double det, inv[3][3];
+    if (dim == 2) {
+        det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        inv[0][0] = J[1][1] / det;
+        inv[0][1] = -J[0][1] / det;
+        inv[1][0] = -J[1][0] / det;
+        inv[1][1] = J[0][0] / det;
+    } else {
+        det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+              J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        for (int r = 0; r < 3; ++r)
+            for (int col = 0; col < 3; ++col) {
+                const int r1 = (col + 1) % 3, r2 = (col + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+                inv[r][col] = (J[r1][c1] * J[r2][c2] - J[r1][c2] * J[r2][c1]) / det;
+            }
+    }
+    require(det != 0.0 && std::isfinite(det), HGDG_ERR_ARG, "degenerate element " + std::to_string(e));
+    c->tm.absdet[e] = std::fabs(det);
+    for (int r = 0; r < dim; ++r)
+        for (int col = 0; col < dim; ++col) c->tm.Jinv[(size_t(e) * dim + r) * dim + col] = inv[r][col];
+}
+
+void finish_tensor_upload(hgdg_ctx* c) {
+    c->n_fields = 1;
+    c->dof = c->n_elem * c->tm.nb;
+    c->tm.ops = tensor_ops(c->tm.p);
+    std::vector<double> opsv;
+    opsv.insert(opsv.end(), c->tm.ops.Dr.begin(), c->tm.ops.Dr.end());
+    opsv.insert(opsv.end(), c->tm.ops.Mi.begin(), c->tm.ops.Mi.end());
+    opsv.insert(opsv.end(), c->tm.ops.M1.begin(), c->tm.ops.M1.end());
+    c->tm.d_ops.upload(opsv, c->stream);
+    c->tm.d_nbr.upload(c->tm.nbr, c->stream);
+    c->tm.d_absdet.upload(c->tm.absdet, c->stream);
+    HGDG_HIP(hipStreamSynchronize(c->stream));
+}
+
+// The periodic box [0,1]^dim with n[d] cells, as structured_mesh builds it
+// (x fastest, vertex order x fastest, coordinates lo + i (hi - lo) / n), set
+// up without the generic face search: neighbours are known from the indices.
+void tensor_structured_upload(hgdg_ctx* c, int dim, const int32_t* n, int p) {
+    require(dim == 2 || dim == 3, HGDG_ERR_ARG, "dim must be 2 or 3");
+    require(p >= 1 && p <= 5, HGDG_ERR_UNSUPPORTED, "polynomial order must be in 1..5");
+    int64_t ne = 1;
+    for (int d = 0; d < dim; ++d) {
+        require(n[d] >= 3, HGDG_ERR_ARG, "periodic boxes need at least 3 cells per direction");
+        ne *= n[d];
+    }
+    c->shape = dim == 3 ? SHAPE_CUBE : SHAPE_SQUARE;
+    c->dim = dim;
+    c->n_elem = ne;
+    c->tm.p = p;
+    c->tm.n1 = p + 1;
+    c->tm.nb = dim == 3 ? c->tm.n1 * c->tm.n1 * c->tm.n1 : c->tm.n1 * c->tm.n1;
+    c->tm.perm = tensor_permutation(dim, p);
+    c->tm.structured = true;
+    for (int d = 0; d < 3; ++d) c->tm.cells[d] = d < dim ? n[d] : 1;
+    c->tm.Jinv.assign(size_t(ne) * dim * dim, 0.0);
+    c->tm.absdet.assign(ne, 0.0);
+    c->tm.nbr.assign(size_t(2 * dim) * ne, -1);
+    int64_t stride[3] = {1, n[0], dim == 3 ? int64_t(n[0]) * n[1] : 0};
+    for (int64_t e = 0; e < ne; ++e) {
+        int64_t idx[3] = {e % n[0], (e / n[0]) % n[1], dim == 3 ? e / (int64_t(n[0]) * n[1]) : 0};
+        double E[3][3] = {{0}};
+        for (int d = 0; d < dim; ++d)
+            E[d][d] = (0.0 + double(idx[d] + 1) * (1.0 - 0.0) / n[d]) - (0.0 + double(idx[d]) * (1.0 - 0.0) / n[d]);
+        affine_geometry(c, e, E);
+        for (int d = 0; d < dim; ++d) {
+            const int64_t lo_n = idx[d] == 0 ? e + (n[d] - 1) * stride[d] : e - stride[d];
+            const int64_t hi_n = idx[d] == n[d] - 1 ? e - (n[d] - 1) * stride[d] : e + stride[d];
+            c->tm.nbr[size_t(2 * d) * ne + e] = int32_t(lo_n);
+            c->tm.nbr[size_t(2 * d + 1) * ne + e] = int32_t(hi_n);
+        }
+    }
+    c->n_faces = int64_t(dim) * ne;
+    c->n_internal = c->n_faces;
+    finish_tensor_upload(c);
+}
+
 void tensor_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     const int dim = shape_dim(m->shape);
     const int64_t ne = m->n_elements;
@@ -572,29 +813,7 @@ void tensor_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
                 require(std::fabs(pred - X[v * dim + r]) <= 1e-12 * scale, HGDG_ERR_UNSUPPORTED,
                         "element " + std::to_string(e) + " is not affine (parallelogram/parallelepiped)");
             }
-        double J[3][3] = {{0}};  // J[r][c] = d x_r / d xi_c
-        for (int r = 0; r < dim; ++r)
-            for (int col = 0; col < dim; ++col) J[r][col] = E[col][r] / 2;
-        double det, inv[3][3];
-        if (dim == 2) {
-            det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-            inv[0][0] = J[1][1] / det;
-            inv[0][1] = -J[0][1] / det;
-            inv[1][0] = -J[1][0] / det;
-            inv[1][1] = J[0][0] / det;
-        } else {
-            det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                  J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-            for (int r = 0; r < 3; ++r)
-                for (int col = 0; col < 3; ++col) {
-                    const int r1 = (col + 1) % 3, r2 = (col + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
-                    inv[r][col] = (J[r1][c1] * J[r2][c2] - J[r1][c2] * J[r2][c1]) / det;
-                }
-        }
-        require(det != 0.0 && std::isfinite(det), HGDG_ERR_ARG, "degenerate element " + std::to_string(e));
-        c->tm.absdet[e] = std::fabs(det);
-        for (int r = 0; r < dim; ++r)
-            for (int col = 0; col < dim; ++col) c->tm.Jinv[(size_t(e) * dim + r) * dim + col] = inv[r][col];
+        affine_geometry(c, e, E);
     }
 
     // faces: match sorted node tuples; the lower element index is hpGEM's
@@ -653,18 +872,7 @@ void tensor_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     n_total += int64_t(open.size());
     c->n_faces = n_total;
     c->n_internal = n_internal;
-    c->n_fields = 1;
-    c->dof = ne * c->tm.nb;
-
-    c->tm.ops = tensor_ops(p);
-    std::vector<double> opsv;
-    opsv.insert(opsv.end(), c->tm.ops.Dr.begin(), c->tm.ops.Dr.end());
-    opsv.insert(opsv.end(), c->tm.ops.Mi.begin(), c->tm.ops.Mi.end());
-    opsv.insert(opsv.end(), c->tm.ops.M1.begin(), c->tm.ops.M1.end());
-    c->tm.d_ops.upload(opsv, c->stream);
-    c->tm.d_nbr.upload(c->tm.nbr, c->stream);
-    c->tm.d_absdet.upload(c->tm.absdet, c->stream);
-    HGDG_HIP(hipStreamSynchronize(c->stream));
+    finish_tensor_upload(c);
 }
 
 void tensor_operator_set(hgdg_ctx* c, const hgdg_op_desc* op) {
@@ -810,70 +1018,6 @@ void tensor_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, do
     *max_abs = m;
 }
 
-// L2 projection of f(x) = sum_m amp_m sin(2 pi k_m . x) onto the DG space
-// (HpgemAPISimplified::setInitialSolution = integrateInitialSolution +
-// solveMassMatrixEquations, HpgemAPISimplified_Impl.h:227-291,751-758), with
-// the element rule of order 2p+1 (Element.cpp:89-98): on an affine tensor
-// element the coefficients are (Mi B^T W)^{(x) dim} applied to f at the
-// Gauss points, B(q, a) = psi_a(x_q). Result in device layout.
-void tensor_project_modes(hgdg_ctx* c, int field, int n_modes, const double* amp, const double* kv,
-                          std::vector<double>& out) {
-    const int n1 = c->tm.n1, dim = c->dim, nb = c->tm.nb;
-    std::vector<double> xq, wq;
-    gauss_legendre(n1, xq, wq);
-    // P(a, q) = sum_b Mi(a, b) psi_b(x_q) w_q
-    std::vector<double> P(size_t(n1) * n1, 0.0);
-    for (int a = 0; a < n1; ++a)
-        for (int q = 0; q < n1; ++q) {
-            double s = 0;
-            for (int b = 0; b < n1; ++b) s += c->tm.ops.Mi[a * n1 + b] * psi(b, xq[q]);
-            P[a * n1 + q] = s * wq[q];
-        }
-    const int nq = nb;  // n1^dim points
-    const int nf = c->n_fields;
-    if (out.size() != size_t(c->n_dof())) out.assign(size_t(c->n_dof()), 0.0);
-    std::vector<double> F(nq), T1(nq), T2(nq);
-    for (int64_t e = 0; e < c->n_elem; ++e) {
-        const double* X0 = c->tm.X0.data() + size_t(e) * dim;
-        const double* E = c->tm.E.data() + size_t(e) * dim * dim;
-        for (int q = 0; q < nq; ++q) {
-            int r = q;
-            double x[3] = {0, 0, 0};
-            for (int d = 0; d < dim; ++d) x[d] = X0[d];
-            for (int d = 0; d < dim; ++d) {
-                const double t = (xq[r % n1] + 1) / 2;
-                r /= n1;
-                for (int k = 0; k < dim; ++k) x[k] += t * E[d * dim + k];
-            }
-            double f = 0;
-            for (int m = 0; m < n_modes; ++m) {
-                double arg = 0;
-                for (int d = 0; d < dim; ++d) arg += kv[m * 3 + d] * x[d];
-                f += amp[m] * std::sin(2 * M_PI * arg);
-            }
-            F[q] = f;
-        }
-        // apply P along each direction (sum factorisation)
-        std::vector<double>* src = &F;
-        std::vector<double>* dst = &T1;
-        int stride = 1;
-        for (int d = 0; d < dim; ++d) {
-            for (int i = 0; i < nq; ++i) {
-                const int id = (i / stride) % n1;
-                const int base = i - id * stride;
-                double s = 0;
-                for (int q = 0; q < n1; ++q) s += P[id * n1 + q] * (*src)[base + q * stride];
-                (*dst)[i] = s;
-            }
-            src = dst;
-            dst = (dst == &T1) ? &T2 : &T1;
-            stride *= n1;
-        }
-        for (int f = 0; f < nf; ++f)
-            if (field < 0 || field == f) std::copy(src->begin(), src->end(), out.begin() + (size_t(e) * nf + f) * nb);
-    }
-}
-
 void tensor_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev) {
     const int nb = c->tm.nb;
     const int64_t blocks = c->n_elem * c->n_fields;  // field-major blocks of nb per element
@@ -887,6 +1031,121 @@ void tensor_from_device(hgdg_ctx* c, const std::vector<double>& dev, double* hos
     const int64_t blocks = c->n_elem * c->n_fields;
     for (int64_t b = 0; b < blocks; ++b)
         for (int i = 0; i < nb; ++i) host[size_t(b) * nb + i] = dev[size_t(b) * nb + c->tm.perm[i]];
+}
+
+// ---------------------------------------------------------------- batched launches
+// Problems of one family (kind, dim, order) advance together: their per-problem
+// parameter blocks go into `params` (device, written by the caller beforehand
+// through `host_staging`), and one launch covers all their elements.
+size_t tensor_batch_param_bytes(const hgdg_ctx* c) {
+    return c->op.kind == HGDG_OP_ACOUSTIC_CENTRAL ? sizeof(AcousticParams) : sizeof(RhsParams);
+}
+
+int64_t tensor_batch_blocks(const hgdg_ctx* c) {
+    int64_t blocks = 0;
+    dispatch_tensor(c->dim, c->tm.n1, [&](auto D, auto N) {
+        using S = Pencil<decltype(D)::value, decltype(N)::value>;
+        blocks = (c->n_elem + S::EPB - 1) / S::EPB;
+    });
+    return blocks;
+}
+
+void tensor_batch_param(hgdg_ctx* c, const StageIO& io, void* out) {
+    if (c->op.kind == HGDG_OP_ACOUSTIC_CENTRAL) {
+        AcousticParams a{};
+        a.n_elem = c->n_elem;
+        a.io = io;
+        a.jinv = c->tm.d_geo.ptr;
+        a.cinv = c->tm.d_geo.ptr + size_t(c->dim) * c->dim * c->n_elem;
+        a.nbr = c->tm.d_nbr.ptr;
+        a.Dr = Dr_ptr(c);
+        a.Mi = Mi_ptr(c);
+        std::memcpy(out, &a, sizeof(a));
+    } else {
+        RhsParams r{};
+        r.n_elem = c->n_elem;
+        r.io = io;
+        r.ahat = c->tm.d_geo.ptr;
+        r.nbr = c->tm.d_nbr.ptr;
+        r.mode = c->tm.d_mode.ptr;
+        r.Dr = Dr_ptr(c);
+        r.Mi = Mi_ptr(c);
+        std::memcpy(out, &r, sizeof(r));
+    }
+}
+
+void tensor_batch_launch(hipStream_t stream, int kind, int dim, int n1, bool mixed, const void* params,
+                         const int64_t* off, int n, int64_t blocks) {
+    if (n == 0 || blocks == 0) return;
+    dispatch_tensor(dim, n1, [&](auto D, auto N) {
+        constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
+        const dim3 grid{unsigned(blocks), 1, 1}, block{unsigned(kBlock), 1, 1};
+        if (kind == HGDG_OP_ACOUSTIC_CENTRAL) {
+            const auto* ps = static_cast<const AcousticParams*>(params);
+            if (mixed) hipLaunchKernelGGL((tensor_acoustic_rhs_batched<DIMc, N1c, float>), grid, block, 0, stream, ps, off, n);
+            else hipLaunchKernelGGL((tensor_acoustic_rhs_batched<DIMc, N1c, double>), grid, block, 0, stream, ps, off, n);
+        } else {
+            const auto* ps = static_cast<const RhsParams*>(params);
+            if (mixed) hipLaunchKernelGGL((tensor_advection_rhs_batched<DIMc, N1c, float>), grid, block, 0, stream, ps, off, n);
+            else hipLaunchKernelGGL((tensor_advection_rhs_batched<DIMc, N1c, double>), grid, block, 0, stream, ps, off, n);
+        }
+    });
+    HGDG_HIP(hipGetLastError());
+}
+
+void tensor_project_device(hgdg_ctx* c, Vec& v, int field, int n_modes, const double* amp, const double* k) {
+    const int n1 = c->tm.n1, dim = c->dim;
+    std::vector<double> xq, wq;
+    gauss_legendre(n1, xq, wq);
+    std::vector<double> P(size_t(n1) * n1, 0.0);
+    for (int a = 0; a < n1; ++a)
+        for (int q = 0; q < n1; ++q) {
+            double s = 0;
+            for (int b = 0; b < n1; ++b) s += c->tm.ops.Mi[a * n1 + b] * psi(b, xq[q]);
+            P[a * n1 + q] = s * wq[q];
+        }
+    std::vector<double> tab(P);
+    tab.insert(tab.end(), xq.begin(), xq.end());
+    for (int m = 0; m < n_modes; ++m) {
+        tab.push_back(amp[m]);
+        for (int d = 0; d < 3; ++d) tab.push_back(k[3 * m + d]);
+    }
+    DevBuf<double> d_tab;
+    d_tab.upload(tab, c->stream);
+    DevBuf<double> d_x0, d_e;
+    ProjParams pp{};
+    pp.n_elem = c->n_elem;
+    pp.nf = c->n_fields;
+    pp.field = field;
+    pp.out = v.ptr();
+    pp.out_f32 = v.is_f32;
+    pp.n_modes = n_modes;
+    pp.P = d_tab.ptr;
+    pp.xq = d_tab.ptr + size_t(n1) * n1;
+    pp.modes = d_tab.ptr + size_t(n1) * n1 + n1;
+    pp.structured = c->tm.structured;
+    for (int d = 0; d < 3; ++d) pp.cells[d] = c->tm.cells[d];
+    if (!c->tm.structured) {
+        const int64_t ne = c->n_elem;
+        std::vector<double> x0(size_t(dim) * ne), ed(size_t(dim) * dim * ne);
+        for (int64_t e = 0; e < ne; ++e)
+            for (int d = 0; d < dim; ++d) {
+                x0[size_t(d) * ne + e] = c->tm.X0[size_t(e) * dim + d];
+                for (int r = 0; r < dim; ++r) ed[(size_t(d) * dim + r) * ne + e] = c->tm.E[(size_t(e) * dim + d) * dim + r];
+            }
+        d_x0.upload(x0, c->stream);
+        d_e.upload(ed, c->stream);
+        pp.X0 = d_x0.ptr;
+        pp.E = d_e.ptr;
+    }
+    dispatch_tensor(dim, n1, [&](auto D, auto N) {
+        constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
+        using S = Pencil<DIMc, N1c>;
+        const int64_t grid = (c->n_elem + S::EPB - 1) / S::EPB;
+        hipLaunchKernelGGL((tensor_project<DIMc, N1c>), dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, pp);
+    });
+    HGDG_HIP(hipGetLastError());
+    HGDG_HIP(hipStreamSynchronize(c->stream));  // before the temporary tables go
 }
 
 }  // namespace hgdg

@@ -168,6 +168,52 @@ int hgdg_sync(hgdg_ctx* ctx);
 /* The context's hipStream_t, for callers timing with HIP events. */
 int hgdg_get_stream(hgdg_ctx* ctx, void** stream);
 
+/* ---- serving: many independent solves with continuous batching ----------
+ * A solve: periodic unit square/cube with cells[d] cells per direction, one
+ * operator and order, sine-mode initial data (acoustic: into the pressure-like
+ * field v), n_steps classic RK4 steps of size dt. Solves are admitted in
+ * arrival order when their HBM footprint fits the budget and a slot is free;
+ * resident solves advance together, one batched launch per operator family
+ * and stage. No reference counterpart: hpGEM runs one problem per process
+ * (HpgemAPISimplified::solve, kernel/Base/HpgemAPISimplified_Impl.h:936-1074). */
+typedef struct hgdg_solve_desc {
+    int32_t dim;            /* 2 or 3                                          */
+    int32_t kind;           /* HGDG_OP_*                                       */
+    int32_t order;          /* 1..5                                            */
+    int32_t cells[3];
+    double velocity[3];     /* advection                                       */
+    double wave_speed;      /* acoustic: c (c_inv = 1/c^2 in every element)    */
+    int32_t n_modes;
+    const double* amp;      /* [n_modes]                                       */
+    const double* k;        /* [n_modes][3]                                    */
+    double dt;
+    int32_t n_steps;
+    int32_t keep_state;     /* 1: keep the final state for hgdg_server_result */
+} hgdg_solve_desc;
+
+typedef struct hgdg_server_stats {
+    int64_t submitted, completed, failed, active;
+    double latency_p50_s, latency_p99_s, latency_mean_s; /* submit -> result ready */
+    double dof_rhs_per_s;   /* sum of DOF x RHS evaluations / wall time        */
+    double wall_s;          /* first submit -> last completion                 */
+    int64_t peak_bytes;     /* peak admitted footprint                         */
+} hgdg_server_stats;
+
+typedef struct hgdg_server hgdg_server;
+
+int hgdg_server_create(hgdg_server** out, int device, int precision, int64_t hbm_budget_bytes, int max_active);
+/* Non-blocking; the ticket identifies the solve. */
+int hgdg_server_submit(hgdg_server* s, const hgdg_solve_desc* d, int64_t* ticket);
+/* Wait up to timeout_s for one ticket (or every submitted one with ticket = -1). */
+int hgdg_server_wait(hgdg_server* s, int64_t ticket, double timeout_s, int* done);
+/* Result of a finished solve: DOF count, final state in reference layout
+ * (if kept; state may be NULL), Euclidean norm of the final coefficients and
+ * the submit-to-done latency. */
+int hgdg_server_result(hgdg_server* s, int64_t ticket, int64_t* n_dof, double* state, double* norm,
+                       double* latency_s);
+int hgdg_server_get_stats(hgdg_server* s, hgdg_server_stats* out);
+int hgdg_server_destroy(hgdg_server* s);
+
 /* ---- host-only utilities (no device needed) ------------------------------ */
 
 /* Number of DG basis functions of hpGEM's H1 set for shape and order. */
