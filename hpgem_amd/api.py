@@ -86,17 +86,22 @@ class HpgemAPIDevice:
                              mesh.node_ids.ctypes.data_as(_lib._I64), self._order.ctypes.data_as(_lib._I32),
                              int(self._order[0]))
         check(self.lib.hgdg_mesh_upload(self.h, ctypes.byref(desc)))
-        ne, nf, ni, nd = (ctypes.c_int64() for _ in range(4))
-        check(self.lib.hgdg_mesh_info(self.h, ctypes.byref(ne), ctypes.byref(nf), ctypes.byref(ni), ctypes.byref(nd)))
-        self.n_elements, self.n_faces, self.n_internal_faces, self.n_dof = ne.value, nf.value, ni.value, nd.value
         vel = (ctypes.c_double * 3)(*(list(velocity or []) + [0.0, 0.0, 0.0])[:3])
         self._cinv = None if c_inv is None else np.ascontiguousarray(c_inv, dtype=np.float64)
         op = _lib.OpDesc(kind, self.precision, vel, dptr(self._cinv) if self._cinv is not None else None)
         check(self.lib.hgdg_operator_set(self.h, ctypes.byref(op)))
+        self._query_info()
         self.solutionVectorId_ = 0
         self.auxiliaryVectorIds_ = list(range(1, tableau.stages + 1))
         self.n_vectors = 1 + tableau.stages + n_extra_vectors
         check(self.lib.hgdg_state_alloc(self.h, self.n_vectors))
+
+    def _query_info(self):
+        # The operator fixes the number of fields (acoustics: dim + 1), so the
+        # DOF count is read after hgdg_operator_set.
+        ne, nf, ni, nd = (ctypes.c_int64() for _ in range(4))
+        check(self.lib.hgdg_mesh_info(self.h, ctypes.byref(ne), ctypes.byref(nf), ctypes.byref(ni), ctypes.byref(nd)))
+        self.n_elements, self.n_faces, self.n_internal_faces, self.n_dof = ne.value, nf.value, ni.value, nd.value
 
     def close(self):
         if getattr(self, "h", None):
