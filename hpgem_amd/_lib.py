@@ -9,6 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhgdg.so")
 
 SHAPE_SQUARE, SHAPE_CUBE, SHAPE_TETRAHEDRON = 2, 4, 5
+SHAPE_LINE, SHAPE_TRIANGLE, SHAPE_PRISM, SHAPE_PYRAMID = 1, 3, 6, 7
 OP_ADVECTION_UPWIND, OP_ACOUSTIC_CENTRAL = 1, 2
 PREC_FP64, PREC_MIXED_FP32 = 0, 1
 
@@ -116,4 +117,17 @@ SIGNATURES += [
     ("hgdg_server_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), _D, _D, _D]),
     ("hgdg_server_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
     ("hgdg_server_destroy", ctypes.c_int, [ctypes.c_void_p]),
+]
+
+
+class MeshFileInfo(ctypes.Structure):
+    _fields_ = [("version", ctypes.c_int32), ("dim", ctypes.c_int32), ("shape", ctypes.c_int32),
+                ("n_vertices", ctypes.c_int32), ("n_nodes", ctypes.c_int64), ("n_elements", ctypes.c_int64),
+                ("n_faces", ctypes.c_int64), ("n_boundary_faces", ctypes.c_int64), ("n_edges", ctypes.c_int64),
+                ("n_zones", ctypes.c_int32), ("zone_name_bytes", ctypes.c_int32)]
+
+
+SIGNATURES += [
+    ("hgdg_mesh_read", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(MeshFileInfo), _D, _I64, _I32, _I64, _I32,
+                                      ctypes.c_char_p]),
 ]

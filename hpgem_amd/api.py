@@ -23,11 +23,36 @@ PRECISIONS = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED_FP32, "mixed_fp32
 class Mesh:
     """Element vertex coordinates + topological node ids (a read .hpgem mesh)."""
 
-    def __init__(self, shape, coords, node_ids):
+    def __init__(self, shape, coords, node_ids, zones=None, zone_names=("Main",), faces=None):
         self.shape = int(shape)
         self.coords = np.ascontiguousarray(coords, dtype=np.float64)
         self.node_ids = np.ascontiguousarray(node_ids, dtype=np.int64)
         self.n_elements, self.n_vertices, self.dim = self.coords.shape
+        self.zones = np.zeros(self.n_elements, dtype=np.int32) if zones is None else np.asarray(zones, np.int32)
+        self.zone_names = list(zone_names)
+        # faces listed by a mesh file: (element, local face) per side, -1 on the boundary side
+        self.face_elements, self.face_local = (None, None) if faces is None else faces
+
+    @classmethod
+    def read(cls, path):
+        """Read a .hpgem file (MeshManipulator::readMesh, kernel/Base/MeshManipulator_Impl.h:1505-1800)."""
+        lib = _lib.load()
+        info = _lib.MeshFileInfo()
+        name = str(path).encode()
+        check(lib.hgdg_mesh_read(name, ctypes.byref(info), None, None, None, None, None, None))
+        ne, nv, dim, nf = info.n_elements, info.n_vertices, info.dim, info.n_faces
+        coords = np.zeros((ne, nv, dim))
+        nodes = np.zeros((ne, nv), dtype=np.int64)
+        zones = np.zeros(ne, dtype=np.int32)
+        fel = np.zeros((nf, 2), dtype=np.int64)
+        flo = np.zeros((nf, 2), dtype=np.int32)
+        names = ctypes.create_string_buffer(info.zone_name_bytes)
+        check(lib.hgdg_mesh_read(name, ctypes.byref(info), dptr(coords), nodes.ctypes.data_as(_lib._I64),
+                                 zones.ctypes.data_as(_lib._I32), fel.ctypes.data_as(_lib._I64),
+                                 flo.ctypes.data_as(_lib._I32), names))
+        mesh = cls(info.shape, coords, nodes, zones, names.value.decode().split("\n"), (fel, flo))
+        mesh.version, mesh.n_nodes, mesh.n_edges = info.version, info.n_nodes, info.n_edges
+        return mesh
 
     @classmethod
     def structured(cls, n, lower=None, upper=None, periodic=True, simplices=False):

@@ -31,6 +31,16 @@ struct Failure : std::runtime_error {
 constexpr int kMaxIn = 8;
 constexpr int kBlock = 256;
 
+// MI355X hands consecutive workgroups of a launch to its 8 XCDs in turn, and
+// each XCD has its own L2. The element kernels read the traces of
+// neighbouring elements, which sit close by in element order; renumbering the
+// blocks so that XCD x runs one contiguous range of logical blocks keeps
+// those reads inside one L2. A bijection of [0, n).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t n) {
+    const int64_t q = n / kXcds, r = n % kXcds, x = b % kXcds;
+    return x * q + (x < r ? x : r) + b / kXcds;
+}
 
 
 // Inputs and output of one stage evaluation: out = f(sum_i coef_i in_i).

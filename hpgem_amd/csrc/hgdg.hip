@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "context.hpp"
+#include "mesh_io.hpp"
 
 namespace hgdg {
 
@@ -135,8 +136,9 @@ void download_vector(hgdg_ctx* c, Vec& v, std::vector<double>& dev) {
 
 void mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     require(m != nullptr, HGDG_ERR_ARG, "mesh descriptor is NULL");
-    require(m->shape == SHAPE_SQUARE || m->shape == SHAPE_CUBE || m->shape == SHAPE_TETRAHEDRON, HGDG_ERR_ARG,
-            "unknown element shape");
+    require(m->shape >= HGDG_SHAPE_LINE && m->shape <= HGDG_SHAPE_PYRAMID, HGDG_ERR_ARG, "unknown element shape");
+    require(m->shape == SHAPE_SQUARE || m->shape == SHAPE_CUBE || m->shape == SHAPE_TETRAHEDRON,
+            HGDG_ERR_UNSUPPORTED, "the device operators take squares, cubes and tetrahedra only");
     const int dim = shape_dim(m->shape);
     require(m->dim == dim, HGDG_ERR_ARG, "dim does not match the element shape");
     require(m->n_elements > 0 && m->n_elements < (int64_t(1) << 31), HGDG_ERR_ARG, "element count out of range");
@@ -553,6 +555,34 @@ int hgdg_structured_mesh(int dim, const int32_t* n, const double* lo, const doub
             std::copy(m.coords.begin(), m.coords.end(), coords);
             std::copy(m.nodes.begin(), m.nodes.end(), node_ids);
         }
+    });
+}
+
+int hgdg_mesh_read(const char* path, hgdg_mesh_file_info* info, double* coords, int64_t* node_ids, int32_t* zones,
+                   int64_t* face_elements, int32_t* face_local, char* zone_names) {
+    return run([&] {
+        require(path && info, HGDG_ERR_ARG, "NULL argument");
+        const hgdg::MeshFile m = hgdg::read_mesh_file(path);
+        std::string names;
+        for (size_t z = 0; z < m.zone_names.size(); ++z) names += (z ? "\n" : "") + m.zone_names[z];
+        *info = hgdg_mesh_file_info{};
+        info->version = m.version;
+        info->dim = m.dim;
+        info->shape = m.shape;
+        info->n_vertices = m.n_vertices;
+        info->n_nodes = m.n_nodes;
+        info->n_elements = m.n_elements;
+        info->n_faces = m.n_faces;
+        info->n_boundary_faces = m.n_boundary_faces;
+        info->n_edges = m.n_edges;
+        info->n_zones = int32_t(m.zone_names.size());
+        info->zone_name_bytes = int32_t(names.size() + 1);
+        if (coords) std::copy(m.coords.begin(), m.coords.end(), coords);
+        if (node_ids) std::copy(m.node_ids.begin(), m.node_ids.end(), node_ids);
+        if (zones) std::copy(m.zones.begin(), m.zones.end(), zones);
+        if (face_elements) std::copy(m.face_elements.begin(), m.face_elements.end(), face_elements);
+        if (face_local) std::copy(m.face_local.begin(), m.face_local.end(), face_local);
+        if (zone_names) std::copy(names.c_str(), names.c_str() + names.size() + 1, zone_names);
     });
 }
 

@@ -233,7 +233,7 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int blk, double* 
 
 __global__ __launch_bounds__(kBlock) void simplex_acoustic_rhs(const SimplexParams p) {
     __shared__ double lds[lds_doubles()];
-    const int b = blockIdx.x;
+    const int b = int(xcd_block(blockIdx.x, gridDim.x));
     if (b < p.group_block[1]) simplex_acoustic_block<1>(p, b - p.group_block[0], lds);
     else if (b < p.group_block[2]) simplex_acoustic_block<2>(p, b - p.group_block[1], lds);
     else if (b < p.group_block[3]) simplex_acoustic_block<3>(p, b - p.group_block[2], lds);

@@ -190,14 +190,17 @@ __device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk)
         double other = 0.0;
         if (m == MODE_NBR || m == MODE_AVG) {
             const int64_t nb = p.nbr[s * p.n_elem + e];
-            double v[1];
-            const int64_t idx = nb * S::NB + pc * N1 + (1 - s);
-            v[0] = p.io.coef[0] * load_in(p.io, 0, idx);
-            for (int i = 1; i < kMaxIn; ++i) {
-                if (i >= p.io.n_in) break;
-                v[0] += p.io.coef[i] * load_in(p.io, i, idx);
+            const int64_t lnb = nb - blk * S::EPB;
+            if (lnb >= 0 && lnb < S::EPB) {
+                other = s_w[lnb * S::NB + pc * N1 + (1 - s)];  // neighbour in this block
+            } else {
+                const int64_t idx = nb * S::NB + pc * N1 + (1 - s);
+                other = p.io.coef[0] * load_in(p.io, 0, idx);
+                for (int i = 1; i < kMaxIn; ++i) {
+                    if (i >= p.io.n_in) break;
+                    other += p.io.coef[i] * load_in(p.io, i, idx);
+                }
             }
-            other = v[0];
         }
         const T jump = T(face_jump(m, w[s], other));
         const T sc = (s ? T(-1) : T(1)) * ah[0] * jump;
@@ -216,7 +219,13 @@ __device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk)
             double other[N1];
             if (m == MODE_NBR || m == MODE_AVG) {
                 const int64_t nb = p.nbr[f * p.n_elem + e];
-                combine_pencil<N1>(p.io, nb * S::NB + nbr_pc * N1, other);
+                const int64_t lnb = nb - blk * S::EPB;
+                if (lnb >= 0 && lnb < S::EPB) {
+#pragma unroll
+                    for (int a = 0; a < N1; ++a) other[a] = s_w[lnb * S::NB + nbr_pc * N1 + a];
+                } else {
+                    combine_pencil<N1>(p.io, nb * S::NB + nbr_pc * N1, other);
+                }
             } else {
 #pragma unroll
                 for (int a = 0; a < N1; ++a) other[a] = 0.0;
@@ -241,7 +250,7 @@ __device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk)
 
 template <int DIM, int N1, typename T>
 __global__ __launch_bounds__(kBlock) void tensor_advection_rhs(const RhsParams p) {
-    advection_block<DIM, N1, T>(p, blockIdx.x);
+    advection_block<DIM, N1, T>(p, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // Block b of a batched launch belongs to problem s with off[s] <= b < off[s+1].
@@ -260,8 +269,9 @@ __device__ __forceinline__ int find_problem(const int64_t* off, int n, int64_t b
 template <int DIM, int N1, typename T>
 __global__ __launch_bounds__(kBlock) void tensor_advection_rhs_batched(const RhsParams* ps, const int64_t* off,
                                                                        int n) {
-    const int s = find_problem(off, n, blockIdx.x);
-    advection_block<DIM, N1, T>(ps[s], int64_t(blockIdx.x) - off[s]);
+    const int64_t b = xcd_block(blockIdx.x, gridDim.x);
+    const int s = find_problem(off, n, b);
+    advection_block<DIM, N1, T>(ps[s], b - off[s]);
 }
 
 // Acoustic wave system on tensor elements (AcousticWaveLinear_Impl.h:102-283):
@@ -395,8 +405,14 @@ __device__ __forceinline__ void acoustic_block(const AcousticParams& p, int64_t 
 #pragma unroll
                 for (int q = 0; q < NF; ++q) own[q] = w[q][s];
                 if (nb >= 0) {
+                    const int64_t lnb = nb - blk * S::EPB;
+                    if (lnb >= 0 && lnb < S::EPB) {  // neighbour in this block
 #pragma unroll
-                    for (int q = 0; q < NF; ++q) oth[q] = combine(p.io, nb * NF * NB + q * NB + pc * N1 + (1 - s));
+                        for (int q = 0; q < NF; ++q) oth[q] = s_w[(lnb * NF + q) * NB + pc * N1 + (1 - s)];
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < NF; ++q) oth[q] = combine(p.io, nb * NF * NB + q * NB + pc * N1 + (1 - s));
+                    }
                 }
                 double cv = 0, cd = 0;
                 if (nb >= 0) {
@@ -421,8 +437,17 @@ __device__ __forceinline__ void acoustic_block(const AcousticParams& p, int64_t 
                 const int my = df == 1 ? ib : ic;
                 double oth[NF][N1];
                 if (nb >= 0) {
+                    const int64_t lnb = nb - blk * S::EPB;
+                    if (lnb >= 0 && lnb < S::EPB) {
 #pragma unroll
-                    for (int q = 0; q < NF; ++q) combine_pencil<N1>(p.io, nb * NF * NB + q * NB + nbr_pc * N1, oth[q]);
+                        for (int q = 0; q < NF; ++q)
+#pragma unroll
+                            for (int a = 0; a < N1; ++a) oth[q][a] = s_w[(lnb * NF + q) * NB + nbr_pc * N1 + a];
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < NF; ++q)
+                            combine_pencil<N1>(p.io, nb * NF * NB + q * NB + nbr_pc * N1, oth[q]);
+                    }
                 }
                 const T m = s_Mi[s * N1 + my];
 #pragma unroll
@@ -468,14 +493,15 @@ __device__ __forceinline__ void acoustic_block(const AcousticParams& p, int64_t 
 
 template <int DIM, int N1, typename T>
 __global__ __launch_bounds__(kBlock) void tensor_acoustic_rhs(const AcousticParams p) {
-    acoustic_block<DIM, N1, T>(p, blockIdx.x);
+    acoustic_block<DIM, N1, T>(p, xcd_block(blockIdx.x, gridDim.x));
 }
 
 template <int DIM, int N1, typename T>
 __global__ __launch_bounds__(kBlock) void tensor_acoustic_rhs_batched(const AcousticParams* ps, const int64_t* off,
                                                                       int n) {
-    const int s = find_problem(off, n, blockIdx.x);
-    acoustic_block<DIM, N1, T>(ps[s], int64_t(blockIdx.x) - off[s]);
+    const int64_t b = xcd_block(blockIdx.x, gridDim.x);
+    const int s = find_problem(off, n, b);
+    acoustic_block<DIM, N1, T>(ps[s], b - off[s]);
 }
 
 // L2 projection of f(x) = sum_m amp_m sin(2 pi k_m . x) on affine tensor

@@ -41,7 +41,12 @@ enum {
 enum {
     HGDG_SHAPE_SQUARE = 2,      /* kernel/Geometry/ReferenceSquare.cpp      */
     HGDG_SHAPE_CUBE = 4,        /* kernel/Geometry/ReferenceCube.cpp        */
-    HGDG_SHAPE_TETRAHEDRON = 5  /* kernel/Geometry/ReferenceTetrahedron.cpp */
+    HGDG_SHAPE_TETRAHEDRON = 5, /* kernel/Geometry/ReferenceTetrahedron.cpp */
+    /* shapes a mesh file may hold but the device operators do not take */
+    HGDG_SHAPE_LINE = 1,
+    HGDG_SHAPE_TRIANGLE = 3,
+    HGDG_SHAPE_PRISM = 6,
+    HGDG_SHAPE_PYRAMID = 7
 };
 
 /* ---- operators ----------------------------------------------------------- */
@@ -229,6 +234,30 @@ int hgdg_basis_eval(int shape, int p, const double* point, double* values);
 int hgdg_structured_mesh(int dim, const int32_t* n, const double* lo, const double* hi,
                          const int32_t* periodic, int simplices, int64_t* n_elements,
                          int32_t* n_vertices, int32_t* shape, double* coords, int64_t* node_ids);
+
+/* Sizes of a .hpgem mesh file (format version 1 or 2, one partition). */
+typedef struct hgdg_mesh_file_info {
+    int32_t version, dim, shape, n_vertices; /* shape: HGDG_SHAPE_*, one type per file */
+    int64_t n_nodes, n_elements;
+    int64_t n_faces, n_boundary_faces;       /* codim-1 entities listed in the file     */
+    int64_t n_edges;                         /* codim-2 entities (3D)                   */
+    int32_t n_zones;
+    int32_t zone_name_bytes;                 /* '\n'-separated names plus the final NUL */
+} hgdg_mesh_file_info;
+
+/* Read a .hpgem file. Replaces MeshManipulator::readMesh and
+ * MeshFileInformation::readInformation (kernel/Base/MeshManipulator_Impl.h:1505-1800,
+ * kernel/Base/MeshFileInformation.cpp:48-103). Fills info; every array that is
+ * not NULL receives, in file order:
+ *   coords        [n_elements][n_vertices][dim]  coordinate copy each element uses
+ *   node_ids      [n_elements][n_vertices]       node (periodic copies share one)
+ *   zones         [n_elements]                   zone index
+ *   face_elements [n_faces][2]                   element on each side, -1 for none
+ *   face_local    [n_faces][2]                   local face index, -1 for none
+ *   zone_names    zone_name_bytes chars
+ * coords and node_ids are what hgdg_mesh_desc takes. */
+int hgdg_mesh_read(const char* path, hgdg_mesh_file_info* info, double* coords, int64_t* node_ids,
+                   int32_t* zones, int64_t* face_elements, int32_t* face_local, char* zone_names);
 
 #ifdef __cplusplus
 }

@@ -154,3 +154,27 @@ def test_tensor_acoustic_mixed_within_1e6():
     assert l2d / l2 < 1e-6
     for d, _ in runs.values():
         d.close()
+
+
+@pytest.mark.parametrize("name", ["3Dtriangular2mesh.hpgem", "unitCubeN4Tet.hpgem"])
+def test_tet_acoustic_on_reference_mesh_file(name):
+    """Tetrahedral meshes read from the reference's .hpgem files, random
+    per-element orders 1..4, walls on every boundary face."""
+    import os
+    print(f"[case] tet acoustic mesh file {name}", flush=True)
+    mesh = Mesh.read(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "meshes", name))
+    rng = np.random.default_rng(11)
+    p = rng.integers(1, 5, size=mesh.n_elements).astype(np.int32)
+    cinv = 1.0 / rng.uniform(0.5, 2.0, size=mesh.n_elements) ** 2
+    dev = HpgemAPIDevice(mesh, p, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=RK4)
+    ref = orc.Problem(mesh.shape, mesh.coords, mesh.node_ids, p, n_fields=4)
+    ref.set_acoustic(cinv)
+    ref_ld = orc.Problem(mesh.shape, mesh.coords, mesh.node_ids, p, n_fields=4, extended=True)
+    ref_ld.set_acoustic(cinv)
+    assert dev.n_faces == len(mesh.face_elements)
+    assert dev.n_faces - dev.n_internal_faces == int(np.sum(mesh.face_elements[:, 1] < 0))
+    u = rng.standard_normal(dev.n_dof)
+    dev.setTimeIntegrationVector(0, u)
+    dev.computeTimeDerivative([0], [1.0], 1)
+    assert_parity(dev.getTimeIntegrationVector(1), ref.time_derivative([u], [1.0]), ref_ld.time_derivative([u], [1.0]))
+    dev.close()
