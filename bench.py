@@ -198,7 +198,101 @@ def run_profile_workload(steps):
                 api.copyVector(5, 0)
         api.synchronize()
         api.close()
+    # the hp-tet acoustic workload: `steps` RHS evaluations per arithmetic path
+    from hpgem_amd import OP_ACOUSTIC_CENTRAL
+    rng = np.random.default_rng(SEED)
+    tets = Mesh.structured([24, 24, 24], periodic=[0, 0, 0], simplices=True)
+    orders = rng.integers(1, 5, size=tets.n_elements).astype(np.int32)
+    cinv = 1.0 / rng.uniform(0.5, 2.0, size=tets.n_elements) ** 2
+    for prec in ("fp64", "mixed"):
+        api = HpgemAPIDevice(tets, orders, precision=prec, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=RK4,
+                             n_extra_vectors=2)
+        api.setInitialSolutionGaussian([0.5, 0.5, 0.5], 0.05, 1.0, field=0)
+        if prec == "mixed":
+            for v in (1, 2):
+                api.setStorage(v, "mixed")
+        api.computeTimeDerivative([0], [1.0], 1)
+        for _ in range(steps):
+            api.computeTimeDerivative([0, 1], [1.0, 1e-4], 2)
+        api.synchronize()
+        api.close()
     print(json.dumps({"profile_workload": "done", "steps": steps}))
+
+
+def run_other_config(name, steps, warmup):
+    """The other RHS workloads, both arithmetic paths, timed like the headline
+    one (one RHS evaluation as RK stage 2 performs it), plus the mixed path's
+    L2 distance to FP64 after the configuration's RK4 run:
+
+    quad16_p2_advection: 2D advection a = (1, 0.5), periodic 16^2 quads, p=2,
+      4 seeded sine modes, dt = 0.1 h/|a|, 100 steps.
+    tet24_hp_acoustic: acoustic system on 24^3 cubes x 5 tets (69120, walls),
+      per-element p in {1..4} and c in U(0.5, 2) from the seed, Gaussian
+      pressure pulse (width 0.05) at a seeded centre, dt = 0.04 h / c_max,
+      100 steps."""
+    from hpgem_amd import OP_ACOUSTIC_CENTRAL, HpgemAPIDevice, Mesh, RK4
+
+    events = HipEvents()
+    rng = np.random.default_rng(SEED)
+    if name == "quad16_p2_advection":
+        mesh = Mesh.structured([16, 16])
+        vel = [1.0, 0.5]
+        amp = rng.uniform(-1.0, 1.0, 4)
+        kk = rng.integers(1, 5, size=(4, 2)).astype(np.float64)
+        dt = 0.1 * (1 / 16) / float(np.linalg.norm(vel))
+        make = lambda prec: HpgemAPIDevice(mesh, 2, velocity=vel, precision=prec, tableau=RK4, n_extra_vectors=2)
+        init = lambda api: api.setInitialSolutionModes(amp, kk)
+        kernel = "tensor_advection_rhs<2,3,{}>"
+    else:
+        mesh = Mesh.structured([24, 24, 24], periodic=[0, 0, 0], simplices=True)
+        orders = rng.integers(1, 5, size=mesh.n_elements).astype(np.int32)
+        c = rng.uniform(0.5, 2.0, size=mesh.n_elements)
+        cinv = 1.0 / c ** 2
+        centre = rng.uniform(0.3, 0.7, size=3)
+        dt = 0.04 * (1 / 24) / float(c.max())
+        make = lambda prec: HpgemAPIDevice(mesh, orders, precision=prec, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv,
+                                           tableau=RK4, n_extra_vectors=2)
+        init = lambda api: api.setInitialSolutionGaussian(centre, 0.05, 1.0, field=0)
+        kernel = "simplex_acoustic_rhs<{}>"
+    out, finals = {}, {}
+    ref = None
+    for prec in ("fp64", "mixed"):
+        api = make(prec)
+        init(api)
+        if prec == "mixed":
+            for v in (1, 2):
+                api.setStorage(v, "mixed")
+        api.computeTimeDerivative([0], [1.0], 1)
+        ms, _ = time_rhs(api, events, steps, warmup, [0, 1], [1.0, 0.5 * dt], 2)
+        b = 4 if prec == "mixed" else 8
+        bytes_alg = algorithmic_bytes(api.n_dof, api.n_elements, api.n_faces, [8, b], b)
+        init(api)
+        stream = api.stream()
+        e0, e1 = events.event(), events.event()
+        api.synchronize()
+        events.record(e0, stream)
+        api.computeOneTimeStep(0.0, dt, 100)
+        events.record(e1, stream)
+        run_ms = events.elapsed_ms(e0, e1)
+        finals[prec] = api.getTimeIntegrationVector(0)
+        out[prec] = {"gdofs": api.n_dof / (ms * 1e-3) / 1e9, "ms_per_rhs": ms, "bytes_alg_per_rhs": bytes_alg,
+                     "achieved_gbs": bytes_alg / (ms * 1e-3) / 1e9,
+                     "roofline_frac": bytes_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "kernel": kernel.format("double" if prec == "fp64" else "float"),
+                     "rk4_100_steps_ms": run_ms}
+        out["dof"], out["elements"], out["faces"], out["dt"] = api.n_dof, api.n_elements, api.n_faces, dt
+        if prec == "fp64":
+            ref = api
+        else:
+            api.close()
+    ref.setTimeIntegrationVector(5, finals["mixed"])
+    l2d, mx = ref.difference(0, 5)
+    ref.setTimeIntegrationVector(6, np.zeros(ref.n_dof))
+    l2, _ = ref.difference(0, 6)
+    ref.close()
+    out["mixed"]["rel_l2_err_vs_fp64"] = l2d / l2
+    out["mixed"]["max_coef_err_vs_fp64"] = mx
+    return out
 
 
 def run_serving(precision, n_requests, seed=SEED):
@@ -261,6 +355,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--serving-requests", type=int, default=512,
                     help="requests in the serving stream (0 skips it)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the 2D quad and hp-tet workloads")
     ap.add_argument("--profile-workload", action="store_true",
                     help="launch only the RHS and copy kernels (for rocprofv3 counter runs)")
     args = ap.parse_args()
@@ -274,6 +370,10 @@ def main():
         run_profile_workload(args.steps)
         return
     gpu, info = run_gpu(args.steps, args.warmup)
+    others = None
+    if not args.headline_only:
+        others = {name: run_other_config(name, args.steps, args.warmup)
+                  for name in ("quad16_p2_advection", "tet24_hp_acoustic")}
     serving = None
     if args.serving_requests > 0:
         serving = {prec: run_serving(prec, args.serving_requests) for prec in ("fp64", "mixed")}
@@ -308,6 +408,7 @@ def main():
         "fp64": gpu["fp64"],
         "mixed": gpu["mixed"],
         "serving": serving,
+        "other_workloads": others,
     }
     if cpu:
         out["speedup_vs_cpu"] = {"fp64": gpu["fp64"]["gdofs"] / cpu["value"],

@@ -163,6 +163,13 @@ class HpgemAPIDevice:
         check(self.lib.hgdg_project_field_modes(self.h, self.solutionVectorId_ if vec is None else vec, field,
                                                 len(amp), dptr(amp), dptr(kk)))
 
+    def setInitialSolutionGaussian(self, center, width, amplitude=1.0, vec=None, field=-1):
+        """L2 projection of amplitude * exp(-|x - center|^2 / width^2)."""
+        c = np.zeros(3)
+        c[: len(center)] = center
+        check(self.lib.hgdg_project_gaussian(self.h, self.solutionVectorId_ if vec is None else vec, field, dptr(c),
+                                             float(width), float(amplitude)))
+
     # ---- the hot path ------------------------------------------------------
     def computeTimeDerivative(self, inputVectorIds, coefficientsInputVectors, resultVectorId, time=0.0):
         ids = (ctypes.c_int * len(inputVectorIds))(*inputVectorIds)

@@ -39,6 +39,7 @@ struct SimplexMesh {
     std::vector<int64_t> dof_off;                // per device element: first DOF
     std::vector<SimplexOps> ops;                 // per order (index p-1), built on demand
     DevBuf<double> d_tables;                     // Dr, proj, lift of every order
+    DevBuf<float> d_tables32;                    // the same in FP32 (mixed path)
     DevBuf<int32_t> d_tab_index;                 // offsets and column lists into d_tables
     DevBuf<double> d_geo;                        // J^{-1} [9][E], then c_inv [E], sigma [4][E], normal [3][4][E]
     DevBuf<int32_t> d_face;                      // [4][E]: neighbour device element or -1, then packed info [4][E]
@@ -49,6 +50,7 @@ struct SimplexMesh {
     std::vector<int32_t> nbr_dev;                // [4][E]
     std::vector<int32_t> info;                   // [4][E]
     std::vector<double> sigma, normal;           // [4][E], [3][4][E]
+    std::vector<double> x0, jac;                 // device order: vertex 0 [E][3], J [E][3][3] (row-major)
 };
 
 }  // namespace hgdg
@@ -89,7 +91,11 @@ Vec& vec_at(hgdg_ctx* c, int id);
 // tensor.hip
 void tensor_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m);
 void tensor_structured_upload(hgdg_ctx* c, int dim, const int32_t* n, int p);
-void tensor_project_device(hgdg_ctx* c, Vec& v, int field, int n_modes, const double* amp, const double* k);
+// L2 projection of f into one field (field = -1: every field) of vector v:
+// f = sum_m amp_m sin(2 pi k_m . x), or, with gauss = {cx, cy, cz, width,
+// amplitude}, f = amplitude exp(-|x - c|^2 / width^2).
+void tensor_project_device(hgdg_ctx* c, Vec& v, int field, int n_modes, const double* amp, const double* k,
+                           const double* gauss = nullptr);
 void tensor_operator_set(hgdg_ctx* c, const hgdg_op_desc* op);
 void tensor_rhs(hgdg_ctx* c, const StageIO& io);
 void tensor_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* l2, double* max_abs);
@@ -125,6 +131,8 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m);
 void simplex_operator_set(hgdg_ctx* c, const hgdg_op_desc* op);
 void simplex_rhs(hgdg_ctx* c, const StageIO& io);
 void simplex_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* l2, double* max_abs);
+void simplex_project_host(hgdg_ctx* c, Vec& v, int field, int n_modes, const double* amp, const double* k,
+                         const double* gauss = nullptr);
 void simplex_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev);
 void simplex_from_device(hgdg_ctx* c, const std::vector<double>& dev, double* host);
 

@@ -458,10 +458,25 @@ int hgdg_project_field_modes(hgdg_ctx* ctx, int vec, int field, int n_modes, con
         require(n_modes >= 0, HGDG_ERR_ARG, "negative mode count");
         HGDG_HIP(hipSetDevice(ctx->device));
         require(ctx->has_mesh, HGDG_ERR_STATE, "no mesh");
-        require(!ctx->simplex(), HGDG_ERR_UNSUPPORTED, "sine-mode projection is available on tensor elements");
         require(field >= -1 && field < ctx->n_fields, HGDG_ERR_ARG, "field out of range");
         hgdg::Vec& v = hgdg::vec_at(ctx, vec);
-        hgdg::tensor_project_device(ctx, v, field, n_modes, amp, k);
+        if (ctx->simplex()) hgdg::simplex_project_host(ctx, v, field, n_modes, amp, k);
+        else hgdg::tensor_project_device(ctx, v, field, n_modes, amp, k);
+    });
+}
+
+int hgdg_project_gaussian(hgdg_ctx* ctx, int vec, int field, const double* center, double width, double amplitude) {
+    return run([&] {
+        require(ctx != nullptr && center != nullptr, HGDG_ERR_ARG, "NULL argument");
+        require(width > 0 && std::isfinite(width) && std::isfinite(amplitude), HGDG_ERR_ARG,
+                "the pulse needs a positive width");
+        HGDG_HIP(hipSetDevice(ctx->device));
+        require(ctx->has_mesh, HGDG_ERR_STATE, "no mesh");
+        require(field >= -1 && field < ctx->n_fields, HGDG_ERR_ARG, "field out of range");
+        hgdg::Vec& v = hgdg::vec_at(ctx, vec);
+        const double g[5] = {center[0], center[1], ctx->dim == 3 ? center[2] : 0.0, width, amplitude};
+        if (ctx->simplex()) hgdg::simplex_project_host(ctx, v, field, 0, nullptr, nullptr, g);
+        else hgdg::tensor_project_device(ctx, v, field, 0, nullptr, nullptr, g);
     });
 }
 

@@ -21,6 +21,7 @@
 #include <array>
 #include <cmath>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 
 #include "context.hpp"
@@ -61,6 +62,7 @@ struct SimplexParams {
     const int32_t* nbr;    // [4][E]: neighbour device element, -1 on walls
     const int32_t* info;   // [4][E]
     const double* tab;
+    const float* tab32;    // the same tables in FP32 (mixed path), same offsets
     const int32_t* tix;
 };
 
@@ -90,7 +92,11 @@ constexpr int lds_doubles() {
     return m;
 }
 
-template <int P>
+// T = double: the FP64 path. T = float: the mixed path; the volume
+// contractions and the face lifts run in FP32 on FP32 tables, the stage
+// input, the face traces and the flux jumps stay FP64, and the per-row sums
+// over faces are compensated.
+template <int P, typename T>
 __device__ void simplex_acoustic_block(const SimplexParams& p, int blk, double* lds) {
     using S = SimplexShape<P>;
     constexpr int NB = S::NB;
@@ -126,21 +132,25 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int blk, double* 
     __syncthreads();
 
     // volume: k_v = sum_e Dr_e g_e ; k_s_d = (1/c_inv) sum_e (J^{-1})_{e,d} Dr_e v
-    double kv = 0, ks[3] = {0, 0, 0};
+    Acc<T> kv, ks[3];
     if (row_ok) {
-        const double* Dr = p.tab + tx[kTixDr];
-        double t[3];
+        const T* Dr = (std::is_same<T, float>::value ? reinterpret_cast<const T*>(p.tab32)
+                                                     : reinterpret_cast<const T*>(p.tab)) + tx[kTixDr];
+        T t[3];
+        T vsum = T(0);
         for (int r = 0; r < 3; ++r) {
-            const double* row = Dr + (r * NB + i) * NB;
-            double a = 0, b = 0;
+            const T* row = Dr + (r * NB + i) * NB;
+            T a = T(0), b = T(0);
             for (int j = 0; j < NB; ++j) {
-                a += row[j] * sg[r * NB + j];
-                b += row[j] * su[j];
+                a += row[j] * T(sg[r * NB + j]);
+                b += row[j] * T(su[j]);
             }
-            kv += a;
+            vsum += a;
             t[r] = b;
         }
-        for (int d = 0; d < 3; ++d) ks[d] = (Ji[0 * 3 + d] * t[0] + Ji[1 * 3 + d] * t[1] + Ji[2 * 3 + d] * t[2]) / cinv;
+        kv = Acc<T>(vsum);
+        for (int d = 0; d < 3; ++d)
+            ks[d] = Acc<T>((T(Ji[0 * 3 + d]) * t[0] + T(Ji[1 * 3 + d]) * t[1] + T(Ji[2 * 3 + d]) * t[2]) / T(cinv));
     }
 
     // faces
@@ -208,21 +218,24 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int blk, double* 
         __syncthreads();
         // (c) lift back: k += sigma M_ref^{-1} P^T J
         if (row_ok) {
-            const double* L = p.tab + tx[kTixLift + f * 6 + info_oown(inf)] + i * kFaceBasis;
-            double lv = 0, ls = 0;
+            const T* L = (std::is_same<T, float>::value ? reinterpret_cast<const T*>(p.tab32)
+                                                        : reinterpret_cast<const T*>(p.tab)) +
+                         tx[kTixLift + f * 6 + info_oown(inf)] + i * kFaceBasis;
+            T lv = T(0), ls = T(0);
             for (int m = 0; m < nF; ++m) {
-                lv += L[m] * sJ[m];
-                ls += L[m] * sJ[kFaceBasis + m];
+                lv += L[m] * T(sJ[m]);
+                ls += L[m] * T(sJ[kFaceBasis + m]);
             }
-            kv += sig * lv;
-            const double c = sig * ls / cinv;
-            for (int d = 0; d < 3; ++d) ks[d] += c * n[d];
+            kv.add(T(sig) * lv);
+            const T c = T(sig) * ls / T(cinv);
+            for (int d = 0; d < 3; ++d) ks[d].add(c * T(n[d]));
         }
         __syncthreads();
     }
 
     if (row_ok) {
-        const double vals[4] = {kv, ks[0], ks[1], ks[2]};
+        const double vals[4] = {double(kv.value()), double(ks[0].value()), double(ks[1].value()),
+                                double(ks[2].value())};
         for (int f = 0; f < 4; ++f) {
             const int64_t o = base + f * NB + i;
             if (p.io.out_f32) static_cast<float*>(p.io.out)[o] = float(vals[f]);
@@ -231,13 +244,14 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int blk, double* 
     }
 }
 
+template <typename T>
 __global__ __launch_bounds__(kBlock) void simplex_acoustic_rhs(const SimplexParams p) {
     __shared__ double lds[lds_doubles()];
     const int b = int(xcd_block(blockIdx.x, gridDim.x));
-    if (b < p.group_block[1]) simplex_acoustic_block<1>(p, b - p.group_block[0], lds);
-    else if (b < p.group_block[2]) simplex_acoustic_block<2>(p, b - p.group_block[1], lds);
-    else if (b < p.group_block[3]) simplex_acoustic_block<3>(p, b - p.group_block[2], lds);
-    else simplex_acoustic_block<4>(p, b - p.group_block[3], lds);
+    if (b < p.group_block[1]) simplex_acoustic_block<1, T>(p, b - p.group_block[0], lds);
+    else if (b < p.group_block[2]) simplex_acoustic_block<2, T>(p, b - p.group_block[1], lds);
+    else if (b < p.group_block[3]) simplex_acoustic_block<3, T>(p, b - p.group_block[2], lds);
+    else simplex_acoustic_block<4, T>(p, b - p.group_block[3], lds);
 }
 
 // Per-element |det J| sum_f dx_f^T M_ref dx_f (plain L2 of all fields) and
@@ -369,8 +383,11 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     sm.absdet.assign(ne, 0.0);
     std::vector<double> jinv_soa(9 * size_t(ne)), detJ(ne);
     std::vector<std::array<double, 9>> Jm(ne);
+    sm.x0.assign(size_t(ne) * 3, 0.0);
+    sm.jac.assign(size_t(ne) * 9, 0.0);
     for (int64_t d = 0; d < ne; ++d) {
         const double* X = m->coords + size_t(sm.mesh_of[d]) * 12;
+        for (int r = 0; r < 3; ++r) sm.x0[size_t(d) * 3 + r] = X[r];
         double J[3][3];
         for (int r = 0; r < 3; ++r)
             for (int k = 0; k < 3; ++k) J[r][k] = X[(k + 1) * 3 + r] - X[r];
@@ -388,6 +405,7 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
             for (int k = 0; k < 3; ++k) {
                 jinv_soa[(r * 3 + k) * size_t(ne) + d] = inv[r][k];
                 Jm[d][r * 3 + k] = J[r][k];
+                sm.jac[size_t(d) * 9 + r * 3 + k] = J[r][k];
             }
         detJ[d] = det;
         sm.absdet[d] = std::fabs(det);
@@ -491,6 +509,8 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     tix.insert(tix.end(), cols.begin(), cols.end());
 
     sm.d_tables.upload(tab, c->stream);
+    const std::vector<float> tab32(tab.begin(), tab.end());  // lives until the synchronize below
+    sm.d_tables32.upload(tab32, c->stream);
     sm.d_tab_index.upload(tix, c->stream);
     sm.d_absdet.upload(sm.absdet, c->stream);
     std::vector<double> geo(jinv_soa);
@@ -507,8 +527,6 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
 void simplex_operator_set(hgdg_ctx* c, const hgdg_op_desc* op) {
     require(op->kind == HGDG_OP_ACOUSTIC_CENTRAL, HGDG_ERR_UNSUPPORTED,
             "tetrahedral meshes run the acoustic operator");
-    require(op->precision == HGDG_PREC_FP64, HGDG_ERR_UNSUPPORTED,
-            "the tetrahedral kernels run in FP64 in this build");
     SimplexMesh& sm = c->sm;
     const int64_t ne = c->n_elem;
     std::vector<double> cinv(ne);
@@ -536,9 +554,15 @@ void simplex_rhs(hgdg_ctx* c, const StageIO& io) {
     p.nbr = sm.d_face.ptr;
     p.info = sm.d_face.ptr + 4 * ne;
     p.tab = sm.d_tables.ptr;
+    p.tab32 = sm.d_tables32.ptr;
     p.tix = sm.d_tab_index.ptr;
     const int grid = p.group_block[4];
-    if (grid > 0) hipLaunchKernelGGL(simplex_acoustic_rhs, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
+    if (grid > 0) {
+        if (c->op.precision == HGDG_PREC_MIXED_FP32)
+            hipLaunchKernelGGL(simplex_acoustic_rhs<float>, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
+        else
+            hipLaunchKernelGGL(simplex_acoustic_rhs<double>, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
+    }
     HGDG_HIP(hipGetLastError());
 }
 
@@ -573,6 +597,112 @@ void simplex_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, d
     }
     *l2 = std::sqrt(std::max(s, 0.0));
     *max_abs = mx;
+}
+
+namespace {
+
+// L2 projector of order p on the reference tetrahedron: points of a
+// collapsed Gauss rule exact to degree 2p+3 and R[i][q] = sum_j
+// (M_ref^{-1})_{ij} w_q phi_j(xi_q), so that the coefficients of the
+// projection of f on an affine element are sum_q R[i][q] f(x(xi_q)).
+struct TetProjector {
+    int nb = 0, nq = 0;
+    std::vector<double> pts;  // [nq][3]
+    std::vector<double> R;    // [nb][nq]
+};
+
+TetProjector tet_projector(int p) {
+    TetProjector t;
+    t.nb = tet_nb(p);
+    std::vector<long double> pts, w;
+    tet_rule(2 * p + 3, pts, w);
+    t.nq = int(w.size());
+    const int nb = t.nb, nq = t.nq;
+    std::vector<long double> phi(size_t(nq) * nb), grad(size_t(nb) * 3), M(size_t(nb) * nb, 0.0L);
+    for (int q = 0; q < nq; ++q) {
+        tet_basis(p, &pts[size_t(q) * 3], &phi[size_t(q) * nb], grad.data());
+        for (int i = 0; i < nb; ++i)
+            for (int j = 0; j < nb; ++j) M[size_t(i) * nb + j] += w[q] * phi[size_t(q) * nb + i] * phi[size_t(q) * nb + j];
+    }
+    // Gauss-Jordan inverse of the SPD reference mass matrix (long double)
+    std::vector<long double> inv(size_t(nb) * nb, 0.0L);
+    for (int i = 0; i < nb; ++i) inv[size_t(i) * nb + i] = 1.0L;
+    for (int c = 0; c < nb; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < nb; ++r)
+            if (fabsl(M[size_t(r) * nb + c]) > fabsl(M[size_t(piv) * nb + c])) piv = r;
+        for (int k = 0; k < nb; ++k) {
+            std::swap(M[size_t(c) * nb + k], M[size_t(piv) * nb + k]);
+            std::swap(inv[size_t(c) * nb + k], inv[size_t(piv) * nb + k]);
+        }
+        const long double d = M[size_t(c) * nb + c];
+        for (int k = 0; k < nb; ++k) {
+            M[size_t(c) * nb + k] /= d;
+            inv[size_t(c) * nb + k] /= d;
+        }
+        for (int r = 0; r < nb; ++r) {
+            if (r == c) continue;
+            const long double f = M[size_t(r) * nb + c];
+            if (f == 0.0L) continue;
+            for (int k = 0; k < nb; ++k) {
+                M[size_t(r) * nb + k] -= f * M[size_t(c) * nb + k];
+                inv[size_t(r) * nb + k] -= f * inv[size_t(c) * nb + k];
+            }
+        }
+    }
+    t.pts.assign(pts.begin(), pts.end());
+    t.R.assign(size_t(nb) * nq, 0.0);
+    for (int i = 0; i < nb; ++i)
+        for (int q = 0; q < nq; ++q) {
+            long double s = 0;
+            for (int j = 0; j < nb; ++j) s += inv[size_t(i) * nb + j] * w[q] * phi[size_t(q) * nb + j];
+            t.R[size_t(i) * nq + q] = double(s);
+        }
+    return t;
+}
+
+}  // namespace
+
+// L2 projection of sine modes or a Gaussian pulse (see tensor_project_device)
+// into one field or all fields of a tetrahedral vector, computed on the host
+// element by element and uploaded; the other fields keep their values.
+void simplex_project_host(hgdg_ctx* c, Vec& v, int field, int n_modes, const double* amp, const double* k,
+                          const double* gauss) {
+    SimplexMesh& sm = c->sm;
+    std::vector<double> dev;
+    download_vector(c, v, dev);
+    TetProjector pr[kMaxSimplexOrder];
+    std::vector<double> fq;
+    for (int64_t d = 0; d < c->n_elem; ++d) {
+        const int p = sm.order[d];
+        TetProjector& P = pr[p - 1];
+        if (P.nb == 0) P = tet_projector(p);
+        fq.assign(size_t(P.nq), 0.0);
+        const double* x0 = &sm.x0[size_t(d) * 3];
+        const double* J = &sm.jac[size_t(d) * 9];
+        for (int q = 0; q < P.nq; ++q) {
+            const double* xi = &P.pts[size_t(q) * 3];
+            double x[3];
+            for (int r = 0; r < 3; ++r) x[r] = x0[r] + J[r * 3 + 0] * xi[0] + J[r * 3 + 1] * xi[1] + J[r * 3 + 2] * xi[2];
+            double f = 0;
+            if (gauss) {
+                const double r2 = (x[0] - gauss[0]) * (x[0] - gauss[0]) + (x[1] - gauss[1]) * (x[1] - gauss[1]) +
+                                  (x[2] - gauss[2]) * (x[2] - gauss[2]);
+                f = gauss[4] * std::exp(-r2 / (gauss[3] * gauss[3]));
+            } else {
+                for (int m = 0; m < n_modes; ++m)
+                    f += amp[m] * std::sin(2 * M_PI * (k[3 * m] * x[0] + k[3 * m + 1] * x[1] + k[3 * m + 2] * x[2]));
+            }
+            fq[q] = f;
+        }
+        for (int i = 0; i < P.nb; ++i) {
+            double s = 0;
+            for (int q = 0; q < P.nq; ++q) s += P.R[size_t(i) * P.nq + q] * fq[q];
+            for (int fld = 0; fld < c->n_fields; ++fld)
+                if (field < 0 || field == fld) dev[size_t(sm.dof_off[d] + int64_t(fld) * P.nb + i)] = s;
+        }
+    }
+    upload_vector(c, v, dev);
 }
 
 void simplex_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev) {

@@ -149,7 +149,8 @@ int hgdg_rk_steps(hgdg_ctx* ctx, int solution, const int* aux_ids, int stages, c
 
 /* vec = L2 projection of f(x) = sum_m amp[m] sin(2 pi k[m] . x), k given as
  * [n_modes][3] (unused components ignored), with hpGEM's element rule of
- * order 2p+1. Replaces HpgemAPISimplified::setInitialSolution
+ * order 2p+1 on squares and cubes (a collapsed Gauss rule exact to degree
+ * 2p+3 on tetrahedra, where the projection is formed on the host). Replaces HpgemAPISimplified::setInitialSolution
  * (kernel/Base/HpgemAPISimplified_Impl.h:227-291,751-758) for this family of
  * initial conditions. */
 int hgdg_project_modes(hgdg_ctx* ctx, int vec, int n_modes, const double* amp, const double* k);
@@ -157,6 +158,13 @@ int hgdg_project_modes(hgdg_ctx* ctx, int vec, int n_modes, const double* amp, c
 /* The same projection into one field of a multi-field operator (field = -1:
  * every field); the other fields of the vector are left as they are. */
 int hgdg_project_field_modes(hgdg_ctx* ctx, int vec, int field, int n_modes, const double* amp, const double* k);
+
+/* vec (one field, or every field with field = -1) = L2 projection of the
+ * Gaussian pulse f(x) = amplitude exp(-|x - center|^2 / width^2), center
+ * given as 3 numbers (the third ignored in 2D). Same role as
+ * hgdg_project_field_modes (HpgemAPISimplified::setInitialSolution with a
+ * pulse as the initial function). */
+int hgdg_project_gaussian(hgdg_ctx* ctx, int vec, int field, const double* center, double width, double amplitude);
 
 /* L2 norm of (vec[x] - vec[y]) under the element mass matrices, and the max
  * abs coefficient difference (FP64, on the device). */

@@ -178,3 +178,108 @@ def test_tet_acoustic_on_reference_mesh_file(name):
     dev.computeTimeDerivative([0], [1.0], 1)
     assert_parity(dev.getTimeIntegrationVector(1), ref.time_derivative([u], [1.0]), ref_ld.time_derivative([u], [1.0]))
     dev.close()
+
+
+def test_tet_mixed_rhs_single_eval():
+    """One mixed-precision tet RHS (FP32 volume/lift arithmetic and FP32
+    stage derivative, FP64 stage input and jumps) against the FP64 path."""
+    mesh = Mesh.structured([3, 3, 3], periodic=[0, 0, 0], simplices=True)
+    rng = np.random.default_rng(21)
+    p = rng.integers(1, 5, size=mesh.n_elements).astype(np.int32)
+    cinv = 1.0 / rng.uniform(0.5, 2.0, size=mesh.n_elements) ** 2
+    devs = {prec: HpgemAPIDevice(mesh, p, precision=prec, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=RK4,
+                                 n_extra_vectors=2) for prec in ("fp64", "mixed")}
+    u = rng.standard_normal(devs["fp64"].n_dof)
+    k = rng.standard_normal(devs["fp64"].n_dof)
+    out = {}
+    for prec, d in devs.items():
+        if prec == "mixed":
+            d.setStorage(1, "mixed")
+            d.setStorage(2, "mixed")
+        d.setTimeIntegrationVector(0, u)
+        d.setTimeIntegrationVector(1, k)
+        d.computeTimeDerivative([0, 1], [1.0, 0.01], 2)
+        out[prec] = d.getTimeIntegrationVector(2)
+    k32 = np.float32(k).astype(np.float64)  # the mixed path reads k in FP32
+    devs["fp64"].setTimeIntegrationVector(1, k32)
+    devs["fp64"].computeTimeDerivative([0, 1], [1.0, 0.01], 3)
+    ref = devs["fp64"].getTimeIntegrationVector(3)
+    err = rel(out["mixed"], ref)
+    print(f"tet mixed single RHS rel err {err:.3e}")
+    assert err < 2e-6
+    for d in devs.values():
+        d.close()
+
+
+def test_tet_mixed_within_1e6_after_rk4():
+    """hp tets (p in 1..4), RK4 with FP32 stage derivatives: relative L2
+    distance to the FP64 run stays below 1e-6."""
+    mesh = Mesh.structured([4, 4, 4], periodic=[0, 0, 0], simplices=True)
+    rng = np.random.default_rng(5)
+    p = rng.integers(1, 5, size=mesh.n_elements).astype(np.int32)
+    c = rng.uniform(0.5, 2.0, size=mesh.n_elements)
+    cinv = 1.0 / c ** 2
+    h = 0.25
+    dt = 0.05 * h / (c.max() * 9)
+    finals = {}
+    devs = {}
+    for prec in ("fp64", "mixed"):
+        d = HpgemAPIDevice(mesh, p, precision=prec, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=RK4,
+                           n_extra_vectors=2)
+        d.setInitialSolutionModes([0.8, -0.5], [[1, 1, 2], [2, 1, 1]], field=0)
+        d.computeOneTimeStep(0.0, dt, 40)
+        finals[prec] = d.getTimeIntegrationVector(0)
+        devs[prec] = d
+    d64 = devs["fp64"]
+    d64.setTimeIntegrationVector(5, finals["mixed"])
+    l2d, mx = d64.difference(0, 5)
+    d64.setTimeIntegrationVector(6, np.zeros(d64.n_dof))
+    l2, _ = d64.difference(0, 6)
+    print(f"tet mixed vs fp64 after 40 RK4 steps: rel L2 {l2d / l2:.3e}, max {mx:.3e}")
+    assert l2 > 0 and l2d / l2 < 1e-6
+    for d in devs.values():
+        d.close()
+
+
+def test_tet_projection_is_the_l2_projection():
+    """Sine modes projected into the pressure-like field of an hp tet mesh
+    (host-side projection, rule exact to degree 2p+3) have the same L2 error
+    as the oracle's projection with the reference's rules (degree 2p+1), to
+    quadrature accuracy, and leave the other fields at zero."""
+    mesh = Mesh.structured([3, 3, 3], periodic=[0, 0, 0], simplices=True)
+    rng = np.random.default_rng(8)
+    p = rng.integers(1, 5, size=mesh.n_elements).astype(np.int32)
+    cinv = np.ones(mesh.n_elements)
+    dev = HpgemAPIDevice(mesh, p, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=RK4)
+    amp, k = [0.7, -0.4], [[1, 0, 1], [0, 1, 1]]
+    dev.setInitialSolutionModes(amp, k, field=0)
+    u = dev.getTimeIntegrationVector(0)
+    ref = orc.Problem(mesh.shape, mesh.coords, mesh.node_ids, p, n_fields=4)
+    ref.set_acoustic(cinv)
+    u_ref = ref.project_modes(amp, k, field_scale=[1.0, 0.0, 0.0, 0.0])
+    e_dev = ref.l2_error_modes(u, amp, k, 0.0)
+    e_ref = ref.l2_error_modes(u_ref, amp, k, 0.0)
+    print(f"projection L2 error: library {e_dev:.6e}, oracle {e_ref:.6e}")
+    assert abs(e_dev - e_ref) < 0.02 * e_ref
+    nb = (p + 1) * (p + 2) * (p + 3) // 6
+    off = np.concatenate([[0], np.cumsum(4 * nb)])
+    for e in range(mesh.n_elements):
+        assert np.all(u[off[e] + nb[e]: off[e + 1]] == 0.0)
+    dev.close()
+
+
+@pytest.mark.parametrize("simplices", [False, True])
+def test_gaussian_pulse_projection_keeps_its_norm(simplices):
+    """||Pi f||^2 of a resolved pulse f = A exp(-|x-c|^2/w^2) inside the box
+    equals int f^2 = A^2 (pi w^2 / 2)^(3/2) to the projection error."""
+    n = [8, 8, 8] if simplices else [10, 10, 10]
+    mesh = Mesh.structured(n, periodic=[0, 0, 0], simplices=simplices)
+    cinv = np.ones(mesh.n_elements)
+    dev = HpgemAPIDevice(mesh, 3, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=RK4, n_extra_vectors=1)
+    A, w, c = 1.7, 0.15, [0.45, 0.52, 0.5]
+    dev.setInitialSolutionGaussian(c, w, A, field=0)
+    l2, _ = dev.difference(0, dev.n_vectors - 1)  # the extra vector is zero
+    exact = A ** 2 * (np.pi * w * w / 2) ** 1.5
+    print(f"||Pi f||^2 = {l2 ** 2:.6e}, int f^2 = {exact:.6e}")
+    assert abs(l2 ** 2 - exact) < 2e-3 * exact
+    dev.close()
