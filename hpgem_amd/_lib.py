@@ -50,6 +50,8 @@ SIGNATURES = [
                                      ctypes.c_double, ctypes.c_int]),
     ("hgdg_project_modes", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, _D]),
     ("hgdg_project_field_modes", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D]),
+    ("hgdg_error_modes", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D,
+                                        _D]),
     ("hgdg_project_gaussian", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, ctypes.c_double,
                                              ctypes.c_double]),
     ("hgdg_state_diff", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, _D]),

@@ -163,6 +163,20 @@ class HpgemAPIDevice:
         check(self.lib.hgdg_project_field_modes(self.h, self.solutionVectorId_ if vec is None else vec, field,
                                                 len(amp), dptr(amp), dptr(kk)))
 
+    def computeErrorModes(self, amp, k, shift=None, vec=None, field=-1):
+        """(L2 error, max error) of a vector against sum_m amp_m sin(2 pi k_m . (x - shift))
+        (computeTotalError / computeMaxError; for advection shift = a t)."""
+        amp = np.ascontiguousarray(amp, dtype=np.float64)
+        kk = np.zeros((len(amp), 3))
+        kk[:, : np.asarray(k).shape[1]] = k
+        sh = np.zeros(3)
+        if shift is not None:
+            sh[: len(shift)] = shift
+        l2, mx = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.hgdg_error_modes(self.h, self.solutionVectorId_ if vec is None else vec, field, len(amp),
+                                        dptr(amp), dptr(kk), dptr(sh), ctypes.byref(l2), ctypes.byref(mx)))
+        return l2.value, mx.value
+
     def setInitialSolutionGaussian(self, center, width, amplitude=1.0, vec=None, field=-1):
         """L2 projection of amplitude * exp(-|x - center|^2 / width^2)."""
         c = np.zeros(3)

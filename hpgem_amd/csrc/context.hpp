@@ -99,6 +99,8 @@ void tensor_project_device(hgdg_ctx* c, Vec& v, int field, int n_modes, const do
 void tensor_operator_set(hgdg_ctx* c, const hgdg_op_desc* op);
 void tensor_rhs(hgdg_ctx* c, const StageIO& io);
 void tensor_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* l2, double* max_abs);
+void tensor_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, const double* amp, const double* k,
+                        const double* shift, double* l2, double* max_abs);
 void tensor_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev);
 void tensor_from_device(hgdg_ctx* c, const std::vector<double>& dev, double* host);
 
@@ -131,6 +133,8 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m);
 void simplex_operator_set(hgdg_ctx* c, const hgdg_op_desc* op);
 void simplex_rhs(hgdg_ctx* c, const StageIO& io);
 void simplex_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* l2, double* max_abs);
+void simplex_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, const double* amp, const double* k,
+                         const double* shift, double* l2, double* max_abs);
 void simplex_project_host(hgdg_ctx* c, Vec& v, int field, int n_modes, const double* amp, const double* k,
                          const double* gauss = nullptr);
 void simplex_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev);

@@ -465,6 +465,21 @@ int hgdg_project_field_modes(hgdg_ctx* ctx, int vec, int field, int n_modes, con
     });
 }
 
+int hgdg_error_modes(hgdg_ctx* ctx, int vec, int field, int n_modes, const double* amp, const double* k,
+                     const double* shift, double* l2, double* max_abs) {
+    return run([&] {
+        require(ctx != nullptr && l2 && max_abs && (n_modes == 0 || (amp && k)), HGDG_ERR_ARG, "NULL argument");
+        require(n_modes >= 0, HGDG_ERR_ARG, "negative mode count");
+        HGDG_HIP(hipSetDevice(ctx->device));
+        require(ctx->has_mesh, HGDG_ERR_STATE, "no mesh");
+        require(field >= -1 && field < ctx->n_fields, HGDG_ERR_ARG, "field out of range");
+        const double zero[3] = {0, 0, 0};
+        const hgdg::Vec& v = hgdg::vec_at(ctx, vec);
+        if (ctx->simplex()) hgdg::simplex_error_modes(ctx, v, field, n_modes, amp, k, shift ? shift : zero, l2, max_abs);
+        else hgdg::tensor_error_modes(ctx, v, field, n_modes, amp, k, shift ? shift : zero, l2, max_abs);
+    });
+}
+
 int hgdg_project_gaussian(hgdg_ctx* ctx, int vec, int field, const double* center, double width, double amplitude) {
     return run([&] {
         require(ctx != nullptr && center != nullptr, HGDG_ERR_ARG, "NULL argument");

@@ -705,6 +705,67 @@ void simplex_project_host(hgdg_ctx* c, Vec& v, int field, int n_modes, const dou
     upload_vector(c, v, dev);
 }
 
+// Error against f(x - shift) for sine modes f, as tensor_error_modes: L2 with
+// a rule exact to degree 2p+2, maximum over the points of a rule exact to
+// degree 2p+1 (collapsed Gauss rules, so the figures agree with hpGEM's
+// tetrahedral rules to quadrature accuracy). Host side, after a download.
+void simplex_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, const double* amp, const double* k,
+                         const double* shift, double* l2, double* max_abs) {
+    SimplexMesh& sm = c->sm;
+    std::vector<double> dev;
+    download_vector(c, const_cast<Vec&>(v), dev);
+    struct Pts {
+        int nq = 0, nb = 0;
+        std::vector<double> x, w, phi;  // [nq][3], [nq], [nq][nb]
+    };
+    auto make = [](int p, int degree) {
+        Pts t;
+        std::vector<long double> pts, w;
+        tet_rule(degree, pts, w);
+        t.nq = int(w.size());
+        t.nb = tet_nb(p);
+        t.x.assign(pts.begin(), pts.end());
+        t.w.assign(w.begin(), w.end());
+        std::vector<long double> phi(size_t(t.nb)), grad(size_t(t.nb) * 3);
+        for (int q = 0; q < t.nq; ++q) {
+            tet_basis(p, &pts[size_t(q) * 3], phi.data(), grad.data());
+            for (int i = 0; i < t.nb; ++i) t.phi.push_back(double(phi[i]));
+        }
+        return t;
+    };
+    Pts rules[2][kMaxSimplexOrder];
+    double sum = 0, mx = 0;
+    for (int64_t d = 0; d < c->n_elem; ++d) {
+        const int p = sm.order[d];
+        const double* x0 = &sm.x0[size_t(d) * 3];
+        const double* J = &sm.jac[size_t(d) * 9];
+        for (int pass = 0; pass < 2; ++pass) {
+            Pts& R = rules[pass][p - 1];
+            if (R.nq == 0) R = make(p, pass == 0 ? 2 * p + 2 : 2 * p + 1);
+            for (int q = 0; q < R.nq; ++q) {
+                const double* xi = &R.x[size_t(q) * 3];
+                double x[3];
+                for (int r = 0; r < 3; ++r) x[r] = x0[r] + J[r * 3] * xi[0] + J[r * 3 + 1] * xi[1] + J[r * 3 + 2] * xi[2];
+                double ex = 0;
+                for (int m = 0; m < n_modes; ++m)
+                    ex += amp[m] * std::sin(2 * M_PI * (k[3 * m] * (x[0] - shift[0]) + k[3 * m + 1] * (x[1] - shift[1]) +
+                                                        k[3 * m + 2] * (x[2] - shift[2])));
+                for (int fld = 0; fld < c->n_fields; ++fld) {
+                    if (field >= 0 && fld != field) continue;
+                    double val = 0;
+                    const double* cf = &dev[size_t(sm.dof_off[d] + int64_t(fld) * R.nb)];
+                    for (int i = 0; i < R.nb; ++i) val += cf[i] * R.phi[size_t(q) * R.nb + i];
+                    const double err = ex - val;
+                    if (pass == 0) sum += R.w[q] * sm.absdet[d] * err * err;
+                    else mx = std::max(mx, std::fabs(err));
+                }
+            }
+        }
+    }
+    *l2 = std::sqrt(sum);
+    *max_abs = mx;
+}
+
 void simplex_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev) {
     const SimplexMesh& sm = c->sm;
     dev.assign(size_t(c->n_dof()), 0.0);

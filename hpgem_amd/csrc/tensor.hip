@@ -728,6 +728,108 @@ void dispatch_tensor(int dim, int n1, F&& f) {
     throw Failure(HGDG_ERR_UNSUPPORTED, "no tensor kernel for this dimension and order");
 }
 
+// Error of one field (or every field) against f(x - shift), f = sum_m amp_m
+// sin(2 pi k_m . x) (HpgemAPISimplified::computeTotalError and computeMaxError,
+// kernel/Base/HpgemAPISimplified_Impl.h:293-530): the squared L2 error with
+// the (p+2)-point Gauss rule per direction (order 2p+2) and the largest
+// pointwise error at the element rule's (p+1)^dim points (order 2p+1). One
+// thread per element; per-block partial sums and maxima.
+struct ErrParams {
+    int64_t n_elem;
+    int32_t nf, field;
+    const void* u;
+    int32_t u_f32;
+    int32_t n_modes;
+    const double* modes;   // [n_modes][4]: amp, k
+    double shift[3];
+    const double* X0;      // [dim][n_elem]
+    const double* E;       // [dim*dim][n_elem]
+    const double* absdet;  // [n_elem]
+    const double* tab;     // psi at the L2 points [N1][N1+1], their weights and points [N1+1] each,
+                           // psi at the element rule points [N1][N1], those points [N1]
+    double* partial;       // [blocks][2]
+};
+
+template <int DIM, int N1>
+__global__ __launch_bounds__(kBlock) void tensor_error(const ErrParams p) {
+    constexpr int NB = N1 * N1 * (DIM == 3 ? N1 : 1);
+    __shared__ double s_red[2][kBlock];
+    const int64_t e = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    double sum = 0, mx = 0;
+    if (e < p.n_elem) {
+        double X0[3] = {0, 0, 0}, E[3][3] = {{0}};
+        for (int d = 0; d < DIM; ++d) {
+            X0[d] = p.X0[d * p.n_elem + e];
+            for (int k = 0; k < DIM; ++k) E[d][k] = p.E[(d * DIM + k) * p.n_elem + e];
+        }
+        const double det = p.absdet[e];
+        for (int fld = 0; fld < p.nf; ++fld) {
+            if (p.field >= 0 && fld != p.field) continue;
+            const int64_t base = (e * p.nf + fld) * NB;
+            for (int pass = 0; pass < 2; ++pass) {
+                const int NQ = pass == 0 ? N1 + 1 : N1;
+                const double* psi = pass == 0 ? p.tab : p.tab + N1 * (N1 + 1) + 2 * (N1 + 1);
+                const double* wq = p.tab + N1 * (N1 + 1);
+                const double* xq = pass == 0 ? wq + (N1 + 1) : psi + N1 * N1;
+                const int nz = DIM == 3 ? NQ : 1;
+                for (int qz = 0; qz < nz; ++qz)
+                    for (int qy = 0; qy < NQ; ++qy)
+                        for (int qx = 0; qx < NQ; ++qx) {
+                            double val = 0;
+                            for (int c = 0; c < (DIM == 3 ? N1 : 1); ++c) {
+                                double vy = 0;
+                                for (int b = 0; b < N1; ++b) {
+                                    double vx = 0;
+                                    for (int a = 0; a < N1; ++a) {
+                                        const int64_t i = base + a + N1 * (b + N1 * c);
+                                        const double cf = p.u_f32 ? double(static_cast<const float*>(p.u)[i])
+                                                                  : static_cast<const double*>(p.u)[i];
+                                        vx += cf * psi[a * NQ + qx];
+                                    }
+                                    vy += vx * psi[b * NQ + qy];
+                                }
+                                val += DIM == 3 ? vy * psi[c * NQ + qz] : vy;
+                            }
+                            const int q[3] = {qx, qy, qz};
+                            double x[3] = {X0[0], X0[1], X0[2]};
+                            for (int d = 0; d < DIM; ++d) {
+                                const double t = (xq[q[d]] + 1) / 2;
+                                for (int k = 0; k < DIM; ++k) x[k] += t * E[d][k];
+                            }
+                            double ex = 0;
+                            for (int m = 0; m < p.n_modes; ++m) {
+                                double arg = 0;
+                                for (int d = 0; d < DIM; ++d) arg += p.modes[4 * m + 1 + d] * (x[d] - p.shift[d]);
+                                ex += p.modes[4 * m] * sin(2 * M_PI * arg);
+                            }
+                            const double err = ex - val;
+                            if (pass == 0) {
+                                double w = det;
+                                for (int d = 0; d < DIM; ++d) w *= wq[q[d]];
+                                sum += w * err * err;
+                            } else {
+                                mx = fmax(mx, fabs(err));
+                            }
+                        }
+            }
+        }
+    }
+    s_red[0][threadIdx.x] = sum;
+    s_red[1][threadIdx.x] = mx;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h /= 2) {
+        if (int(threadIdx.x) < h) {
+            s_red[0][threadIdx.x] += s_red[0][threadIdx.x + h];
+            s_red[1][threadIdx.x] = fmax(s_red[1][threadIdx.x], s_red[1][threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        p.partial[2 * blockIdx.x] = s_red[0][0];
+        p.partial[2 * blockIdx.x + 1] = s_red[1][0];
+    }
+}
+
 // ---------------------------------------------------------------- host side
 namespace {
 const double* Dr_ptr(hgdg_ctx* c) { return c->tm.d_ops.ptr; }
@@ -1054,6 +1156,80 @@ void tensor_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, do
         if (c->d_partial.n < size_t(2 * grid)) c->d_partial.alloc(size_t(2 * grid));
         p.partial = c->d_partial.ptr;
         hipLaunchKernelGGL((tensor_diff_norm<DIMc, N1c>), dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
+    });
+    HGDG_HIP(hipGetLastError());
+    std::vector<double> h(size_t(2 * grid));
+    HGDG_HIP(hipMemcpyAsync(h.data(), c->d_partial.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HGDG_HIP(hipStreamSynchronize(c->stream));
+    double s = 0, m = 0;
+    for (int64_t i = 0; i < grid; ++i) {
+        s += h[2 * i];
+        m = std::max(m, h[2 * i + 1]);
+    }
+    *l2 = std::sqrt(std::max(s, 0.0));
+    *max_abs = m;
+}
+
+void tensor_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, const double* amp, const double* k,
+                        const double* shift, double* l2, double* max_abs) {
+    const int n1 = c->tm.n1, dim = c->dim;
+    const int64_t ne = c->n_elem;
+    std::vector<double> x2, w2, x1, w1;
+    gauss_legendre(n1 + 1, x2, w2);
+    gauss_legendre(n1, x1, w1);
+    std::vector<double> tab;
+    for (int a = 0; a < n1; ++a)
+        for (int q = 0; q <= n1; ++q) tab.push_back(psi(a, x2[q]));
+    tab.insert(tab.end(), w2.begin(), w2.end());
+    tab.insert(tab.end(), x2.begin(), x2.end());
+    for (int a = 0; a < n1; ++a)
+        for (int q = 0; q < n1; ++q) tab.push_back(psi(a, x1[q]));
+    tab.insert(tab.end(), x1.begin(), x1.end());
+    for (int m = 0; m < n_modes; ++m) {
+        tab.push_back(amp[m]);
+        for (int d = 0; d < 3; ++d) tab.push_back(k[3 * m + d]);
+    }
+    // element origins and edge vectors, SoA
+    std::vector<double> x0(size_t(dim) * ne), ed(size_t(dim) * dim * ne);
+    for (int64_t e = 0; e < ne; ++e) {
+        if (c->tm.structured) {
+            const int64_t idx[3] = {e % c->tm.cells[0], (e / c->tm.cells[0]) % c->tm.cells[1],
+                                    dim == 3 ? e / (int64_t(c->tm.cells[0]) * c->tm.cells[1]) : 0};
+            for (int d = 0; d < dim; ++d) {
+                x0[size_t(d) * ne + e] = double(idx[d]) / c->tm.cells[d];
+                for (int r = 0; r < dim; ++r)
+                    ed[(size_t(d) * dim + r) * ne + e] = r == d ? double(idx[d] + 1) / c->tm.cells[d] - double(idx[d]) / c->tm.cells[d] : 0.0;
+            }
+        } else {
+            for (int d = 0; d < dim; ++d) {
+                x0[size_t(d) * ne + e] = c->tm.X0[size_t(e) * dim + d];
+                for (int r = 0; r < dim; ++r) ed[(size_t(d) * dim + r) * ne + e] = c->tm.E[(size_t(e) * dim + d) * dim + r];
+            }
+        }
+    }
+    DevBuf<double> d_tab, d_x0, d_e;
+    d_tab.upload(tab, c->stream);
+    d_x0.upload(x0, c->stream);
+    d_e.upload(ed, c->stream);
+    ErrParams p{};
+    p.n_elem = ne;
+    p.nf = c->n_fields;
+    p.field = field;
+    p.u = v.ptr();
+    p.u_f32 = v.is_f32;
+    p.n_modes = n_modes;
+    p.modes = d_tab.ptr + (tab.size() - size_t(4) * n_modes);
+    for (int d = 0; d < 3; ++d) p.shift[d] = shift ? shift[d] : 0.0;
+    p.X0 = d_x0.ptr;
+    p.E = d_e.ptr;
+    p.absdet = c->tm.d_absdet.ptr;
+    p.tab = d_tab.ptr;
+    const int64_t grid = (ne + kBlock - 1) / kBlock;
+    if (c->d_partial.n < size_t(2 * grid)) c->d_partial.alloc(size_t(2 * grid));
+    p.partial = c->d_partial.ptr;
+    dispatch_tensor(dim, n1, [&](auto D, auto N) {
+        constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
+        hipLaunchKernelGGL((tensor_error<DIMc, N1c>), dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
     });
     HGDG_HIP(hipGetLastError());
     std::vector<double> h(size_t(2 * grid));

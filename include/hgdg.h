@@ -159,6 +159,16 @@ int hgdg_project_modes(hgdg_ctx* ctx, int vec, int n_modes, const double* amp, c
  * every field); the other fields of the vector are left as they are. */
 int hgdg_project_field_modes(hgdg_ctx* ctx, int vec, int field, int n_modes, const double* amp, const double* k);
 
+/* Error of vec against the exact solution f(x - shift), f(x) = sum_m amp[m]
+ * sin(2 pi k[m] . x) (for advection, shift = a t), in one field or summed over
+ * all fields (field = -1): l2 = sqrt(sum_e int_e |f - u_h|^2) with the rule of
+ * order 2p+2, max_abs = the largest |f - u_h| at the element rule's points.
+ * Replaces HpgemAPISimplified::computeTotalError / computeMaxError
+ * (kernel/Base/HpgemAPISimplified_Impl.h:293-530). On the device for squares
+ * and cubes; on the host, with collapsed Gauss rules, for tetrahedra. */
+int hgdg_error_modes(hgdg_ctx* ctx, int vec, int field, int n_modes, const double* amp, const double* k,
+                     const double* shift, double* l2, double* max_abs);
+
 /* vec (one field, or every field with field = -1) = L2 projection of the
  * Gaussian pulse f(x) = amplitude exp(-|x - center|^2 / width^2), center
  * given as 3 numbers (the third ignored in 2D). Same role as

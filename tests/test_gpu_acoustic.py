@@ -283,3 +283,23 @@ def test_gaussian_pulse_projection_keeps_its_norm(simplices):
     print(f"||Pi f||^2 = {l2 ** 2:.6e}, int f^2 = {exact:.6e}")
     assert abs(l2 ** 2 - exact) < 2e-3 * exact
     dev.close()
+
+
+def test_tet_error_norm_matches_oracle_to_quadrature_accuracy():
+    """computeTotalError on an hp tet mesh (host-side, collapsed Gauss rule of
+    degree 2p+2) against the oracle's reference rules of the same order."""
+    mesh = Mesh.structured([3, 3, 3], periodic=[0, 0, 0], simplices=True)
+    rng = np.random.default_rng(9)
+    p = rng.integers(1, 5, size=mesh.n_elements).astype(np.int32)
+    cinv = np.ones(mesh.n_elements)
+    dev = HpgemAPIDevice(mesh, p, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=RK4)
+    amp, k = [0.7, -0.4], [[1, 0, 1], [0, 1, 1]]
+    dev.setInitialSolutionModes(amp, k, field=0)
+    u = dev.getTimeIntegrationVector(0)
+    ref = orc.Problem(mesh.shape, mesh.coords, mesh.node_ids, p, n_fields=4)
+    l2, mx = dev.computeErrorModes(amp, k, field=0)
+    e_ref = ref.l2_error_modes(u, amp, k, 0.0)
+    print(f"tet L2 error: library {l2:.6e}, oracle {e_ref:.6e}; max {mx:.3e}")
+    assert abs(l2 - e_ref) < 0.02 * e_ref
+    assert mx > 0
+    dev.close()

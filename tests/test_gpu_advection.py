@@ -146,7 +146,43 @@ def test_dormand_prince_reference_error_table_on_gpu():
         u = dev.getTimeIntegrationVector(0)
         err = ref.l2_error_modes(u, PROD3_AMP, PROD3_K, 0.02)
         assert abs(err - expected) < 0.01 * expected, (n, err, expected)
+        # the library's own error norm (computeTotalError on the device)
+        l2, _ = dev.computeErrorModes(PROD3_AMP, PROD3_K, shift=[0.1 * 0.02, 0.2 * 0.02, 0.3 * 0.02])
+        assert abs(l2 - err) < 1e-12 * err, (n, l2, err)
         dev.close()
+
+
+@pytest.mark.parametrize("n,p", [([3, 4], 2), ([3, 3, 3], 2), ([2, 3, 2], 3)])
+def test_max_error_matches_pointwise_evaluation(n, p):
+    """computeMaxError: the largest |f - u_h| at the element rule's (p+1)^dim
+    Gauss points, checked against the oracle's basis evaluated at those
+    points (kernel/Base/HpgemAPISimplified_Impl.h:403-530)."""
+    a = [1.0, 0.5, 0.25][: len(n)]
+    dev, ref = make_pair(n, p, a)
+    amp, kv = [0.8, -0.3], [[1, 2, 1][: len(n)], [2, 1, 1][: len(n)]]
+    u = seeded_state(dev.n_dof, 5) * 0.05
+    dev.setTimeIntegrationVector(0, u)
+    shift = [0.01, 0.02, 0.03][: len(n)]
+    l2, mx = dev.computeErrorModes(amp, kv, shift=shift)
+    mesh = Mesh.structured(n)
+    xg, _ = np.polynomial.legendre.leggauss(p + 1)
+    dim = len(n)
+    grids = np.stack(np.meshgrid(*([xg] * dim), indexing="ij"), -1).reshape(-1, dim)
+    nb = (p + 1) ** dim
+    best = 0.0
+    for e in range(mesh.n_elements):
+        x0 = mesh.coords[e, 0]
+        edges = [mesh.coords[e, 1 << d] - x0 for d in range(dim)]
+        for xi in grids:
+            phi, _ = orc.basis_eval(mesh.shape, p, xi)
+            val = phi @ u[e * nb:(e + 1) * nb]
+            x = x0 + sum((xi[d] + 1) / 2 * edges[d] for d in range(dim))
+            ex = sum(A * np.sin(2 * np.pi * np.dot(k, x - np.asarray(shift))) for A, k in zip(amp, kv))
+            best = max(best, abs(ex - val))
+    print(f"max error {mx:.6e} vs pointwise {best:.6e}; L2 {l2:.4e}")
+    assert abs(mx - best) < 1e-12 * max(best, 1.0)
+    assert l2 > 0
+    dev.close()
 
 
 def test_scale_and_add_matches_numpy():
