@@ -202,6 +202,34 @@ class HpgemAPIDevice:
         check(self.lib.hgdg_rk_steps(self.h, self.solutionVectorId_, aux, t.stages, dptr(a), dptr(b), dt, n_steps))
         return time + n_steps * dt
 
+    def computeErrorAndNormOfUpdate(self, dt):
+        """(error estimate, solution size) of the last step of an embedded tableau."""
+        t = self.tableau
+        if not t.has_error_estimate:
+            raise ValueError(f"{t.name} has no embedded error estimate")
+        aux = (ctypes.c_int * t.stages)(*self.auxiliaryVectorIds_)
+        e = np.ascontiguousarray(t.error, dtype=np.float64)
+        err, nrm = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.hgdg_step_error(self.h, self.solutionVectorId_, aux, t.stages, dptr(e), dt, ctypes.byref(err),
+                                       ctypes.byref(nrm)))
+        return err.value, nrm.value
+
+    def computeOneTimeStepAdaptive(self, time, maximum_relative_error, dt_max, dt_initial):
+        """One adaptive step (computeOneTimeStep(time, maximumRelativeError, dtMax));
+        returns (new time, accepted dt, error estimate, solution size)."""
+        t = self.tableau
+        if not t.has_error_estimate:
+            raise ValueError(f"{t.name} has no embedded error estimate")
+        aux = (ctypes.c_int * t.stages)(*self.auxiliaryVectorIds_)
+        a = np.ascontiguousarray(t.a_matrix(), dtype=np.float64)
+        b = np.ascontiguousarray(t.b, dtype=np.float64)
+        e = np.ascontiguousarray(t.error, dtype=np.float64)
+        tm, dt, err, nrm = ctypes.c_double(time), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(self.lib.hgdg_rk_adaptive_step(self.h, self.solutionVectorId_, aux, t.stages, dptr(a), dptr(b), dptr(e),
+                                             t.order, maximum_relative_error, dt_max, dt_initial, ctypes.byref(tm),
+                                             ctypes.byref(dt), ctypes.byref(err), ctypes.byref(nrm)))
+        return tm.value, dt.value, err.value, nrm.value
+
     def solve(self, start_time, end_time, dt):
         n = int(round((end_time - start_time) / dt))
         return self.computeOneTimeStep(start_time, dt, n)

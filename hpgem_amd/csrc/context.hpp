@@ -41,6 +41,7 @@ struct SimplexMesh {
     DevBuf<double> d_tables;                     // Dr, proj, lift of every order
     DevBuf<float> d_tables32;                    // the same in FP32 (mixed path)
     DevBuf<int32_t> d_tab_index;                 // offsets and column lists into d_tables
+    DevBuf<int64_t> d_elem_off;                  // [E + 1] first DOF of each device element
     DevBuf<double> d_geo;                        // J^{-1} [9][E], then c_inv [E], sigma [4][E], normal [3][4][E]
     DevBuf<int32_t> d_face;                      // [4][E]: neighbour device element or -1, then packed info [4][E]
     DevBuf<double> d_absdet;                     // [E]
@@ -72,6 +73,7 @@ struct hgdg_ctx {
     hgdg::GraphKey graph_key{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    double dt_estimate = 0;  // adaptive stepping: next step size (0 = not started)
 
     bool simplex() const { return shape == HGDG_SHAPE_TETRAHEDRON; }
     int64_t n_dof() const { return dof; }

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <string>
 #include <vector>
@@ -219,11 +220,34 @@ void launch_axpy(hgdg_ctx* c, int to, const std::vector<int>& add, const std::ve
     HGDG_HIP(hipGetLastError());
 }
 
-void rk_steps(hgdg_ctx* c, int solution, const int* aux, int stages, const double* a, const double* b, double dt,
-              int n_steps) {
+// One step of an explicit tableau enqueued on the context's stream: each
+// stage's input combination, RHS and inverse mass in one launch (zero tableau
+// entries contribute exactly nothing and are skipped), then the update.
+void enqueue_rk_step(hgdg_ctx* c, int solution, const int* aux, int stages, const double* a, const double* b,
+                     double dt) {
+    for (int i = 0; i < stages; ++i) {
+        std::vector<int> ids{solution};
+        std::vector<double> co{1.0};
+        for (int j = 0; j < i; ++j)
+            if (a[i * stages + j] != 0.0) {
+                ids.push_back(aux[j]);
+                co.push_back(dt * a[i * stages + j]);
+            }
+        launch_rhs(c, ids.data(), co.data(), int(ids.size()), aux[i]);
+    }
+    std::vector<int> add;
+    std::vector<double> sc;
+    for (int j = 0; j < stages; ++j)
+        if (b[j] != 0.0) {
+            add.push_back(aux[j]);
+            sc.push_back(dt * b[j]);
+        }
+    if (!add.empty()) launch_axpy(c, solution, add, sc);
+}
+
+void check_rk_args(hgdg_ctx* c, int solution, const int* aux, int stages) {
     require(c->has_op, HGDG_ERR_STATE, "set an operator before time stepping");
     require(stages >= 1 && stages <= kMaxIn - 1, HGDG_ERR_ARG, "1..7 stages supported");
-    require(n_steps >= 0, HGDG_ERR_ARG, "negative step count");
     Vec& u = vec_at(c, solution);
     require(!u.is_f32, HGDG_ERR_ARG, "the solution vector must be stored in FP64");
     const bool mixed = c->op.precision == HGDG_PREC_MIXED_FP32;
@@ -232,6 +256,12 @@ void rk_steps(hgdg_ctx* c, int solution, const int* aux, int stages, const doubl
         for (int j = 0; j < i; ++j) require(aux[i] != aux[j], HGDG_ERR_ARG, "stage vectors must be distinct");
         set_storage(c, vec_at(c, aux[i]), mixed);
     }
+}
+
+void rk_steps(hgdg_ctx* c, int solution, const int* aux, int stages, const double* a, const double* b, double dt,
+              int n_steps) {
+    require(n_steps >= 0, HGDG_ERR_ARG, "negative step count");
+    check_rk_args(c, solution, aux, stages);
     GraphKey key{solution, std::vector<int>(aux, aux + stages), std::vector<double>(a, a + stages * stages),
                  std::vector<double>(b, b + stages), dt};
     if (!(c->graph_valid && c->graph_key == key)) {
@@ -239,25 +269,7 @@ void rk_steps(hgdg_ctx* c, int solution, const int* aux, int stages, const doubl
         HGDG_HIP(hipStreamSynchronize(c->stream));
         HGDG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         try {
-            for (int i = 0; i < stages; ++i) {
-                std::vector<int> ids{solution};
-                std::vector<double> co{1.0};
-                // zero tableau entries contribute exactly nothing; they are skipped
-                for (int j = 0; j < i; ++j)
-                    if (a[i * stages + j] != 0.0) {
-                        ids.push_back(aux[j]);
-                        co.push_back(dt * a[i * stages + j]);
-                    }
-                launch_rhs(c, ids.data(), co.data(), int(ids.size()), aux[i]);
-            }
-            std::vector<int> add;
-            std::vector<double> sc;
-            for (int j = 0; j < stages; ++j)
-                if (b[j] != 0.0) {
-                    add.push_back(aux[j]);
-                    sc.push_back(dt * b[j]);
-                }
-            if (!add.empty()) launch_axpy(c, solution, add, sc);
+            enqueue_rk_step(c, solution, aux, stages, a, b, dt);
         } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(c->stream, &g);
@@ -270,6 +282,128 @@ void rk_steps(hgdg_ctx* c, int solution, const int* aux, int stages, const doubl
         c->graph_valid = true;
     }
     for (int s = 0; s < n_steps; ++s) HGDG_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+}
+
+// Error estimate and solution size of the last step
+// (HpgemAPISimplified::computeErrorAndNormOfUpdate, kernel/Base/
+// HpgemAPISimplified_Impl.h:718-745): err2 = sum over all coefficients of
+// (sum_j dt e_j k_j)^2, norm = sum over elements of the Euclidean norm of the
+// element's solution coefficients. One wavefront per element.
+struct StepNormParams {
+    int64_t n_elem;
+    const int64_t* off;  // [n_elem + 1] element DOF offsets, or NULL
+    int64_t block;       // DOF per element when off is NULL
+    const double* u;
+    const void* k[kMaxIn];
+    double c[kMaxIn];
+    uint32_t k_f32;
+    int32_t n_k;
+    double* partial;     // [blocks][2]
+};
+
+__global__ __launch_bounds__(kBlock) void step_error_norm(const StepNormParams p) {
+    constexpr int W = 64;  // wavefront size on CDNA
+    __shared__ double s_part[2][kBlock / W];
+    const int lane = threadIdx.x % W, wave = threadIdx.x / W;
+    const int64_t e = int64_t(blockIdx.x) * (kBlock / W) + wave;
+    double err2 = 0, u2 = 0;
+    if (e < p.n_elem) {
+        const int64_t lo = p.off ? p.off[e] : e * p.block;
+        const int64_t hi = p.off ? p.off[e + 1] : lo + p.block;
+        for (int64_t i = lo + lane; i < hi; i += W) {
+            const double v = p.u[i];
+            u2 += v * v;
+            double d = 0;
+            for (int j = 0; j < p.n_k; ++j)
+                d += p.c[j] * (((p.k_f32 >> j) & 1u) ? double(static_cast<const float*>(p.k[j])[i])
+                                                      : static_cast<const double*>(p.k[j])[i]);
+            err2 += d * d;
+        }
+    }
+    for (int o = W / 2; o > 0; o /= 2) {
+        err2 += __shfl_xor(err2, o, W);
+        u2 += __shfl_xor(u2, o, W);
+    }
+    if (lane == 0) {
+        s_part[0][wave] = err2;
+        s_part[1][wave] = sqrt(u2);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, b = 0;
+        for (int w = 0; w < kBlock / W; ++w) {
+            a += s_part[0][w];
+            b += s_part[1][w];
+        }
+        p.partial[2 * blockIdx.x] = a;
+        p.partial[2 * blockIdx.x + 1] = b;
+    }
+}
+
+void step_error_and_norm(hgdg_ctx* c, int solution, const int* aux, int stages, const double* err_coef, double dt,
+                         double* error, double* norm) {
+    StepNormParams p{};
+    p.n_elem = c->n_elem;
+    p.off = c->simplex() ? c->sm.d_elem_off.ptr : nullptr;
+    p.block = c->dof / c->n_elem;
+    p.u = vec_at(c, solution).f64->ptr;
+    for (int j = 0; j < stages; ++j) {
+        if (err_coef[j] == 0.0) continue;
+        Vec& k = vec_at(c, aux[j]);
+        p.k[p.n_k] = k.ptr();
+        p.c[p.n_k] = dt * err_coef[j];
+        if (k.is_f32) p.k_f32 |= 1u << p.n_k;
+        ++p.n_k;
+    }
+    const int64_t per = kBlock / 64;
+    const int64_t grid = (c->n_elem + per - 1) / per;
+    if (c->d_partial.n < size_t(2 * grid)) c->d_partial.alloc(size_t(2 * grid));
+    p.partial = c->d_partial.ptr;
+    hipLaunchKernelGGL(step_error_norm, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
+    HGDG_HIP(hipGetLastError());
+    std::vector<double> h(size_t(2 * grid));
+    HGDG_HIP(hipMemcpyAsync(h.data(), c->d_partial.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HGDG_HIP(hipStreamSynchronize(c->stream));
+    double e2 = 0, nrm = 0;
+    for (int64_t i = 0; i < grid; ++i) {
+        e2 += h[2 * i];
+        nrm += h[2 * i + 1];
+    }
+    *error = std::sqrt(e2);
+    *norm = nrm;
+}
+
+// HpgemAPISimplified::computeOneTimeStep(time, maximumRelativeError, dtMax)
+// (kernel/Base/HpgemAPISimplified_Impl.h:822-861): steps with the current
+// estimate (capped by dt_max), rewinds and shrinks dt while the embedded
+// error exceeds max_rel_error times the solution size, then updates the
+// estimate. The estimate is kept per context (the reference keeps one
+// function-static value for the whole process).
+void rk_adaptive_step(hgdg_ctx* c, int solution, const int* aux, int stages, const double* a, const double* b,
+                      const double* err_coef, int order, double max_rel_error, double dt_max, double dt_initial,
+                      double* time, double* dt_used, double* error_out, double* norm_out) {
+    require(order >= 1, HGDG_ERR_ARG, "tableau order must be positive");
+    require(max_rel_error > 0 && dt_max > 0 && dt_initial > 0, HGDG_ERR_ARG,
+            "tolerance, dt_max and the initial dt must be positive");
+    check_rk_args(c, solution, aux, stages);
+    if (!(c->dt_estimate > 0)) c->dt_estimate = dt_initial;
+    double dt = std::min(c->dt_estimate, dt_max);
+    double err = std::numeric_limits<double>::infinity(), nrm = 0;
+    for (int attempt = 0; max_rel_error * nrm < err || std::isinf(err); ++attempt) {
+        require(attempt < 64, HGDG_ERR_ARG, "adaptive step: no acceptable step size after 64 attempts");
+        enqueue_rk_step(c, solution, aux, stages, a, b, dt);
+        step_error_and_norm(c, solution, aux, stages, err_coef, dt, &err, &nrm);
+        if (max_rel_error * nrm < err || std::isinf(err)) {
+            for (int j = 0; j < stages; ++j)
+                if (b[j] != 0.0) launch_axpy(c, solution, {aux[j]}, {-dt * b[j]});
+            dt = 0.8 * dt * std::pow(max_rel_error / err * nrm, 1.0 / order);
+        }
+    }
+    c->dt_estimate = 0.9 * dt * std::pow(max_rel_error / err * nrm, 1.0 / order);
+    *time += dt;
+    if (dt_used) *dt_used = dt;
+    if (error_out) *error_out = err;
+    if (norm_out) *norm_out = nrm;
 }
 
 void state_diff(hgdg_ctx* c, int x, int y, double* l2, double* max_abs) {
@@ -445,6 +579,29 @@ int hgdg_state_diff(hgdg_ctx* ctx, int x, int y, double* l2, double* max_abs) {
         require(ctx != nullptr && l2 && max_abs, HGDG_ERR_ARG, "NULL argument");
         HGDG_HIP(hipSetDevice(ctx->device));
         hgdg::state_diff(ctx, x, y, l2, max_abs);
+    });
+}
+
+int hgdg_rk_adaptive_step(hgdg_ctx* ctx, int solution, const int* aux_ids, int stages, const double* a,
+                          const double* b, const double* err_coef, int order, double max_rel_error, double dt_max,
+                          double dt_initial, double* time, double* dt_used, double* error, double* norm) {
+    return run([&] {
+        require(ctx && aux_ids && a && b && err_coef && time, HGDG_ERR_ARG, "NULL argument");
+        HGDG_HIP(hipSetDevice(ctx->device));
+        hgdg::rk_adaptive_step(ctx, solution, aux_ids, stages, a, b, err_coef, order, max_rel_error, dt_max,
+                               dt_initial, time, dt_used, error, norm);
+    });
+}
+
+int hgdg_step_error(hgdg_ctx* ctx, int solution, const int* aux_ids, int stages, const double* err_coef, double dt,
+                    double* error, double* norm) {
+    return run([&] {
+        require(ctx && aux_ids && err_coef && error && norm, HGDG_ERR_ARG, "NULL argument");
+        require(stages >= 1 && stages <= hgdg::kMaxIn, HGDG_ERR_ARG, "1..8 stage vectors");
+        HGDG_HIP(hipSetDevice(ctx->device));
+        require(ctx->has_op, HGDG_ERR_STATE, "set an operator first");
+        require(!hgdg::vec_at(ctx, solution).is_f32, HGDG_ERR_ARG, "the solution vector must be stored in FP64");
+        hgdg::step_error_and_norm(ctx, solution, aux_ids, stages, err_coef, dt, error, norm);
     });
 }
 

@@ -377,6 +377,9 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     }
     c->n_fields = nf;
     c->dof = sm.group_dof[4];
+    std::vector<int64_t> elem_off(sm.dof_off);  // lives until the synchronize at the end
+    elem_off.push_back(c->dof);
+    sm.d_elem_off.upload(elem_off, c->stream);
 
     // geometry (device order): J column k = X_{k+1} - X_0
     sm.jinv.assign(size_t(ne) * 9, 0.0);

@@ -147,6 +147,26 @@ int hgdg_scale_and_add(hgdg_ctx* ctx, int to, int add, double scale);
 int hgdg_rk_steps(hgdg_ctx* ctx, int solution, const int* aux_ids, int stages, const double* a,
                   const double* b, double dt, int n_steps);
 
+/* Error estimate of the last step of an embedded tableau and the size of
+ * the solution: error = || sum_j dt err_coef[j] k_j ||_2 over all
+ * coefficients, norm = sum over elements of the Euclidean norm of the
+ * element's coefficients. Replaces HpgemAPISimplified::
+ * computeErrorAndNormOfUpdate (kernel/Base/HpgemAPISimplified_Impl.h:718-745). */
+int hgdg_step_error(hgdg_ctx* ctx, int solution, const int* aux_ids, int stages, const double* err_coef, double dt,
+                    double* error, double* norm);
+
+/* One adaptive step with an embedded tableau (err_coef = b - b_low, order =
+ * the tableau's order): try dt = min(estimate, dt_max), rewind and retry with
+ * dt *= 0.8 (tol * norm / error)^(1/order) while error > max_rel_error * norm,
+ * then set the next estimate to 0.9 dt (tol * norm / error)^(1/order) and
+ * advance *time by the accepted dt. The first call starts from dt_initial;
+ * the estimate is kept per context. Replaces HpgemAPISimplified::
+ * computeOneTimeStep(time, maximumRelativeError, dtMax) (:822-861).
+ * dt_used, error and norm may be NULL. */
+int hgdg_rk_adaptive_step(hgdg_ctx* ctx, int solution, const int* aux_ids, int stages, const double* a,
+                          const double* b, const double* err_coef, int order, double max_rel_error, double dt_max,
+                          double dt_initial, double* time, double* dt_used, double* error, double* norm);
+
 /* vec = L2 projection of f(x) = sum_m amp[m] sin(2 pi k[m] . x), k given as
  * [n_modes][3] (unused components ignored), with hpGEM's element rule of
  * order 2p+1 on squares and cubes (a collapsed Gauss rule exact to degree

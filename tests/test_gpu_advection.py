@@ -307,3 +307,64 @@ def test_rhs_on_reference_mesh_file(name, p, a):
     dev.computeTimeDerivative([0], [1.0], 1)
     assert_parity(dev.getTimeIntegrationVector(1), ref.time_derivative([u], [1.0]), ref_ld.time_derivative([u], [1.0]))
     dev.close()
+
+
+def test_step_error_and_norm_match_their_definition():
+    """computeErrorAndNormOfUpdate: ||sum_j dt e_j k_j||_2 and the sum of the
+    elements' coefficient norms, against numpy on the downloaded vectors."""
+    dev, _ = make_pair([4, 4, 4], 2, [1.0, 0.7, 0.3], tableau=hpgem_amd.DORMAND_PRINCE)
+    dev.setInitialSolutionModes([0.6, -0.3], [[1, 2, 1], [2, 1, 3]])
+    dt = 0.004
+    dev.computeOneTimeStep(0.0, dt, 1)
+    err, nrm = dev.computeErrorAndNormOfUpdate(dt)
+    t = hpgem_amd.DORMAND_PRINCE
+    ks = [dev.getTimeIntegrationVector(i) for i in dev.auxiliaryVectorIds_]
+    dev_sum = sum(dt * e * k for e, k in zip(t.error, ks))
+    u = dev.getTimeIntegrationVector(0)
+    nb = u.size // dev.n_elements
+    ref_err = np.linalg.norm(dev_sum)
+    ref_nrm = np.linalg.norm(u.reshape(dev.n_elements, nb), axis=1).sum()
+    assert abs(err - ref_err) < 1e-12 * ref_err and abs(nrm - ref_nrm) < 1e-12 * ref_nrm
+    dev.close()
+
+
+def test_adaptive_step_accepts_and_estimates_like_the_reference_loop():
+    """A loose tolerance accepts dt_initial at once: the state equals one fixed
+    Dormand-Prince step and the next estimate is 0.9 dt (tol norm/err)^(1/5)."""
+    a = [1.0, 0.7, 0.3]
+    d1, _ = make_pair([4, 4, 4], 2, a, tableau=hpgem_amd.DORMAND_PRINCE)
+    d2, _ = make_pair([4, 4, 4], 2, a, tableau=hpgem_amd.DORMAND_PRINCE)
+    for d in (d1, d2):
+        d.setInitialSolutionModes([0.6, -0.3], [[1, 2, 1], [2, 1, 3]])
+    t1, dt1, err, nrm = d1.computeOneTimeStepAdaptive(0.0, 1.0, 1.0, 0.01)
+    d2.computeOneTimeStep(0.0, 0.01, 1)
+    assert t1 == 0.01 and dt1 == 0.01 and err <= 1.0 * nrm
+    np.testing.assert_array_equal(d1.getTimeIntegrationVector(0), d2.getTimeIntegrationVector(0))
+    # the next step starts from the estimate (capped by dt_max)
+    t2, dt2, err2, nrm2 = d1.computeOneTimeStepAdaptive(t1, 1.0, 0.015, 0.01)
+    assert dt2 == min(0.015, 0.9 * 0.01 * (1.0 / err * nrm) ** 0.2)
+    assert abs(t2 - (t1 + dt2)) < 1e-15
+    d1.close()
+    d2.close()
+
+
+def test_adaptive_step_rejects_until_the_error_bound_holds():
+    """A tight tolerance forces rejections: the accepted step satisfies
+    error <= tol * norm with a smaller dt, and the run stays accurate."""
+    a = [1.0, 0.7, 0.3]
+    amp, kv = [0.6, -0.3], [[1, 2, 1], [2, 1, 3]]
+    dev, ref = make_pair([6, 6, 6], 3, a, tableau=hpgem_amd.DORMAND_PRINCE)
+    dev.setInitialSolutionModes(amp, kv)
+    tol = 1e-9
+    t, dts = 0.0, []
+    while t < 0.02 - 1e-15:
+        t, dt, err, nrm = dev.computeOneTimeStepAdaptive(t, tol, 0.02 - t, 0.05)
+        assert err <= tol * nrm
+        dts.append(dt)
+    assert dts[0] < 0.05 and len(dts) >= 2
+    l2, _ = dev.computeErrorModes(amp, kv, shift=[x * t for x in a])
+    u0 = ref.project_modes(amp, kv)
+    l2_0 = ref.l2_error_modes(u0, amp, kv, 0.0)
+    print(f"adaptive DP: {len(dts)} steps, dt {dts}, L2 error {l2:.3e} (projection error {l2_0:.3e})")
+    assert l2 < 1.5 * l2_0  # time error negligible next to the spatial one
+    dev.close()
