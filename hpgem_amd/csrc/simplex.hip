@@ -30,15 +30,19 @@ namespace hgdg {
 
 namespace {
 
-// int32 table index per order (p-1): Dr, Mref, then proj and lift offsets for
-// (face, orientation), then (ncol, cols offset) per face; columns follow.
-constexpr int kTixDr = 0;
-constexpr int kTixM = 1;
-constexpr int kTixProj = 2;           // + f * 6 + o
-constexpr int kTixLift = 2 + 24;      // + f * 6 + o
-constexpr int kTixNcol = 2 + 48;      // + f
-constexpr int kTixCols = 2 + 52;      // + f
-constexpr int kTixStride = 2 + 56;
+// int32 table index per order (p-1): offsets into the FP64 (and the FP32)
+// table arrays, and per local face the number of face columns and the offset
+// of their list; after the four orders, the orientation permutations of the
+// face rule points.
+constexpr int kTixDrT = 0;           // [3][nb][nb]: DrT[r][j][i] = Dr_r(i, j)
+constexpr int kTixM = 1;             // [nb][nb]: M_ref
+constexpr int kTixVf = 2;            // + f: [ncol][kFaceQ]: trace of face column c at face point q
+constexpr int kTixLf = 6;            // + f: [kFaceQ][nb]: (M_ref^{-1} phi(face point q))_i
+constexpr int kTixNcol = 10;         // + f
+constexpr int kTixCols = 14;         // + f
+constexpr int kTixStride = 18;
+constexpr int kTixPerm = 4 * kTixStride;  // [6][kFaceQ]: face point index after orientation o
+constexpr int kMaxCols = 15;              // face columns of an order-4 element
 
 // packed per-face info
 __host__ __device__ constexpr int info_lf(int v) { return v & 3; }
@@ -46,6 +50,43 @@ __host__ __device__ constexpr int info_oown(int v) { return (v >> 2) & 7; }
 __host__ __device__ constexpr int info_onbr(int v) { return (v >> 5) & 7; }
 __host__ __device__ constexpr int info_pn(int v) { return (v >> 8) & 7; }
 constexpr int info_pack(int lf, int oown, int onbr, int pn) { return lf | (oown << 2) | (onbr << 5) | (pn << 8); }
+
+struct RefTetRule {
+    int order, n;
+    const double* pts;  // [n][3] on the reference tetrahedron
+    const double* w;
+};
+#include "tet_rules.inc"
+
+// Inverse of a small SPD matrix (Gauss-Jordan with pivoting, long double).
+std::vector<long double> inverse_ld(const std::vector<double>& A, int n) {
+    std::vector<long double> M(A.begin(), A.end()), inv(size_t(n) * n, 0.0L);
+    for (int i = 0; i < n; ++i) inv[size_t(i) * n + i] = 1.0L;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (fabsl(M[size_t(r) * n + c]) > fabsl(M[size_t(piv) * n + c])) piv = r;
+        for (int k = 0; k < n; ++k) {
+            std::swap(M[size_t(c) * n + k], M[size_t(piv) * n + k]);
+            std::swap(inv[size_t(c) * n + k], inv[size_t(piv) * n + k]);
+        }
+        const long double d = M[size_t(c) * n + c];
+        for (int k = 0; k < n; ++k) {
+            M[size_t(c) * n + k] /= d;
+            inv[size_t(c) * n + k] /= d;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const long double f = M[size_t(r) * n + c];
+            if (f == 0.0L) continue;
+            for (int k = 0; k < n; ++k) {
+                M[size_t(r) * n + k] -= f * M[size_t(c) * n + k];
+                inv[size_t(r) * n + k] -= f * inv[size_t(c) * n + k];
+            }
+        }
+    }
+    return inv;
+}
 
 }  // namespace
 
@@ -64,23 +105,23 @@ struct SimplexParams {
     const double* tab;
     const float* tab32;    // the same tables in FP32 (mixed path), same offsets
     const int32_t* tix;
+    const double* face_w;  // [kFaceQ] face rule weights
 };
 
 __host__ __device__ constexpr int simplex_nb(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
 __host__ __device__ constexpr int simplex_tpe(int p) { return simplex_nb(p) > 16 ? simplex_nb(p) : 16; }
-__device__ constexpr int tri_dim(int p) { return (p + 1) * (p + 2) / 2; }
 
 template <int P>
 struct SimplexShape {
     static constexpr int NB = simplex_nb(P);
-    static constexpr int TPE = simplex_tpe(P);  // threads per element (>= face basis size)
+    static constexpr int TPE = simplex_tpe(P);  // threads per element
     static constexpr int EPB = kBlock / TPE;
     // LDS carve-up (doubles)
-    static constexpr int U = 0;                         // [EPB][4][NB] stage input
-    static constexpr int N = U + EPB * 4 * NB;          // [EPB][4][kFaceBasis] neighbour trace columns
-    static constexpr int G = N + EPB * 4 * kFaceBasis;  // [EPB][3][NB] J^{-T}-rotated s
-    static constexpr int J = G + EPB * 3 * NB;          // [EPB][2][kFaceBasis] face flux coefficients
-    static constexpr int TOTAL = J + EPB * 2 * kFaceBasis;
+    static constexpr int U = 0;                             // [EPB][4][NB] stage input
+    static constexpr int G = U + EPB * 4 * NB;              // [EPB][3][NB] J^{-T}-rotated s
+    static constexpr int N = G + EPB * 3 * NB;              // [EPB][4 fields][kMaxCols] one neighbour's face columns
+    static constexpr int F = N + EPB * 4 * kMaxCols;        // [EPB][4 faces][2][kFaceQ] weighted fluxes at face points
+    static constexpr int TOTAL = F + EPB * 8 * kFaceQ;
 };
 
 constexpr int lds_doubles() {
@@ -92,10 +133,19 @@ constexpr int lds_doubles() {
     return m;
 }
 
-// T = double: the FP64 path. T = float: the mixed path; the volume
-// contractions and the face lifts run in FP32 on FP32 tables, the stage
-// input, the face traces and the flux jumps stay FP64, and the per-row sums
-// over faces are compensated.
+// Tetrahedral elements of order P, one block of EPB elements, TPE threads per
+// element, three phases separated by barriers:
+//  1. stage input (all four fields) and J^{-T}-rotated s into LDS;
+//  2. threads of a row: the volume terms; then face by face, the neighbour's
+//     face columns (the basis functions that do not vanish on the shared
+//     face) into LDS and, one thread per face rule point, both traces (the
+//     neighbour's through the orientation permutation) and the central / wall
+//     flux times the weight;
+//  3. threads of a row: the lift of the point fluxes, M_ref^{-1} phi(point).
+// The face rule (order 8) integrates every face term exactly for p <= 4 on
+// both sides, so this is the reference's operator up to rounding. T = double:
+// FP64; T = float: the volume contractions and the lifts in FP32 on FP32
+// tables, compensated per-row sums, traces and fluxes in FP64.
 template <int P, typename T>
 __device__ void simplex_acoustic_block(const SimplexParams& p, int blk, double* lds) {
     using S = SimplexShape<P>;
@@ -109,51 +159,60 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int blk, double* 
     const int64_t E = p.n_elem;
     const int64_t base = p.group_dof[P - 1] + (e - p.group_start[P - 1]) * 4 * NB;
     double* su = lds + S::U + le * 4 * NB;
-    double* sn = lds + S::N + le * 4 * kFaceBasis;
     double* sg = lds + S::G + le * 3 * NB;
-    double* sJ = lds + S::J + le * 2 * kFaceBasis;
+    double* sn = lds + S::N + le * 4 * kMaxCols;
+    double* sf = lds + S::F + le * 8 * kFaceQ;
     const int32_t* tx = p.tix + (P - 1) * kTixStride;
+    const T* tabT = std::is_same<T, float>::value ? reinterpret_cast<const T*>(p.tab32)
+                                                  : reinterpret_cast<const T*>(p.tab);
 
-    // stage input of the element, all four fields
-    if (row_ok)
-        for (int f = 0; f < 4; ++f) su[f * NB + i] = combine(p.io, base + f * NB + i);
-    __syncthreads();
-
+    // ---- phase 1
     double Ji[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     double cinv = 1.0;
     if (elem_ok) {
+#pragma unroll
         for (int q = 0; q < 9; ++q) Ji[q] = p.jinv[q * E + e];
         cinv = p.cinv[e];
     }
-    // g_e = sum_d (J^{-1})_{e,d} s_d: the reference-direction flux of s
-    if (row_ok)
-        for (int r = 0; r < 3; ++r)
-            sg[r * NB + i] = Ji[r * 3 + 0] * su[1 * NB + i] + Ji[r * 3 + 1] * su[2 * NB + i] + Ji[r * 3 + 2] * su[3 * NB + i];
+    if (row_ok) {
+        double u[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            u[f] = combine(p.io, base + f * NB + i);
+            su[f * NB + i] = u[f];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) sg[r * NB + i] = Ji[r * 3] * u[1] + Ji[r * 3 + 1] * u[2] + Ji[r * 3 + 2] * u[3];
+    }
     __syncthreads();
 
-    // volume: k_v = sum_e Dr_e g_e ; k_s_d = (1/c_inv) sum_e (J^{-1})_{e,d} Dr_e v
+    // ---- phase 2a: volume, k_v = sum_r Dr_r g_r ; k_s_d = (1/c_inv) sum_r (J^{-1})_{r,d} Dr_r v
     Acc<T> kv, ks[3];
     if (row_ok) {
-        const T* Dr = (std::is_same<T, float>::value ? reinterpret_cast<const T*>(p.tab32)
-                                                     : reinterpret_cast<const T*>(p.tab)) + tx[kTixDr];
-        T t[3];
-        T vsum = T(0);
-        for (int r = 0; r < 3; ++r) {
-            const T* row = Dr + (r * NB + i) * NB;
-            T a = T(0), b = T(0);
-            for (int j = 0; j < NB; ++j) {
-                a += row[j] * T(sg[r * NB + j]);
-                b += row[j] * T(su[j]);
+        const T* DrT = tabT + tx[kTixDrT];
+        T a[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
+#pragma unroll 2
+        for (int j = 0; j < NB; ++j) {
+            const T v = T(su[j]);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const T d = DrT[(r * NB + j) * NB + i];
+                a[r] += d * T(sg[r * NB + j]);
+                b[r] += d * v;
             }
-            vsum += a;
-            t[r] = b;
         }
-        kv = Acc<T>(vsum);
+        kv = Acc<T>(a[0] + a[1] + a[2]);
+        const T ci = T(1.0 / cinv);
+#pragma unroll
         for (int d = 0; d < 3; ++d)
-            ks[d] = Acc<T>((T(Ji[0 * 3 + d]) * t[0] + T(Ji[1 * 3 + d]) * t[1] + T(Ji[2 * 3 + d]) * t[2]) / T(cinv));
+            ks[d] = Acc<T>((T(Ji[d]) * b[0] + T(Ji[3 + d]) * b[1] + T(Ji[6 + d]) * b[2]) * ci);
     }
 
-    // faces
+    // ---- phase 2b: fluxes at the face points, one face at a time: the
+    // neighbour's face columns (the basis functions that do not vanish on the
+    // shared face) into LDS, then both traces at the points
+    const int32_t* perm = p.tix + kTixPerm;
+#pragma unroll 1
     for (int f = 0; f < 4; ++f) {
         int64_t nb = -1;
         int inf = 0;
@@ -161,81 +220,89 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int blk, double* 
             nb = p.nbr[f * E + e];
             inf = p.info[f * E + e];
         }
-        const int pn = info_pn(inf);
-        const int lfn = info_lf(inf);
-        // (a) neighbour trace columns into LDS
-        if (elem_ok && nb >= 0) {
-            const int32_t* txn = p.tix + (pn - 1) * kTixStride;
-            const int ncn = txn[kTixNcol + lfn];
-            const int32_t* colsn = p.tix + txn[kTixCols + lfn];
+        const int pn = info_pn(inf), lfn = info_lf(inf);
+        const int32_t* txn = p.tix + (pn > 0 ? pn - 1 : 0) * kTixStride;
+        const int ncn = nb >= 0 ? txn[kTixNcol + lfn] : 0;
+        if (nb >= 0) {
             const int nbn = simplex_nb(pn);
             const int64_t nbase = p.group_dof[pn - 1] + (nb - p.group_start[pn - 1]) * 4 * nbn;
+            const int32_t* colsn = p.tix + txn[kTixCols + lfn];
+#pragma unroll 1
             for (int it = i; it < 4 * ncn; it += S::TPE) {
-                const int fld = it / ncn, col = it % ncn;
-                sn[fld * kFaceBasis + col] = combine(p.io, nbase + fld * nbn + colsn[col]);
+                const int fld = it / ncn, c = it % ncn;
+                sn[fld * kMaxCols + c] = combine(p.io, nbase + fld * nbn + colsn[c]);
             }
         }
         __syncthreads();
-        // (b) face-basis coefficients of both traces and of the flux
-        const int nF = tri_dim(nb >= 0 && pn > P ? pn : P);
-        double n[3] = {0, 0, 0};
-        double sig = 0;
-        if (elem_ok) {
-            for (int d = 0; d < 3; ++d) n[d] = p.normal[(d * 4 + f) * E + e];
-            sig = p.sigma[f * E + e];
-        }
-        if (elem_ok && i < kFaceBasis) {
-            double Jv = 0, Js = 0;
-            if (i < nF) {
-                const int nco = tx[kTixNcol + f];
-                const int32_t* colso = p.tix + tx[kTixCols + f];
-                const double* Po = p.tab + tx[kTixProj + f * 6 + info_oown(inf)] + i * nco;
-                double ov = 0, os[3] = {0, 0, 0};
-                for (int c = 0; c < nco; ++c) {
-                    const int col = colso[c];
-                    ov += Po[c] * su[col];
-                    for (int d = 0; d < 3; ++d) os[d] += Po[c] * su[(1 + d) * NB + col];
-                }
-                if (nb >= 0) {
-                    const int32_t* txn = p.tix + (pn - 1) * kTixStride;
-                    const int ncn = txn[kTixNcol + lfn];
-                    const double* Pn = p.tab + txn[kTixProj + lfn * 6 + info_onbr(inf)] + i * ncn;
-                    double nv = 0, ns[3] = {0, 0, 0};
-                    for (int c = 0; c < ncn; ++c) {
-                        nv += Pn[c] * sn[c];
-                        for (int d = 0; d < 3; ++d) ns[d] += Pn[c] * sn[(1 + d) * kFaceBasis + c];
-                    }
-                    Jv = -0.5 * (n[0] * (os[0] - ns[0]) + n[1] * (os[1] - ns[1]) + n[2] * (os[2] - ns[2]));
-                    Js = -0.5 * (ov - nv);
-                } else {
-                    Jv = -(n[0] * os[0] + n[1] * os[1] + n[2] * os[2]);
-                    Js = 0;
-                }
+        if (elem_ok && i < kFaceQ) {
+            const int q = i;
+            const double n0 = p.normal[(0 * 4 + f) * E + e], n1 = p.normal[(1 * 4 + f) * E + e],
+                         n2 = p.normal[(2 * 4 + f) * E + e];
+            const int qo = perm[info_oown(inf) * kFaceQ + q];
+            const int nco = tx[kTixNcol + f];
+            const int32_t* colso = p.tix + tx[kTixCols + f];
+            const double* Vo = p.tab + tx[kTixVf + f];
+            double ov = 0, os0 = 0, os1 = 0, os2 = 0;
+#pragma unroll 1
+            for (int c = 0; c < nco; ++c) {
+                const double phi = Vo[c * kFaceQ + qo];
+                const int col = colso[c];
+                ov += phi * su[col];
+                os0 += phi * su[NB + col];
+                os1 += phi * su[2 * NB + col];
+                os2 += phi * su[3 * NB + col];
             }
-            sJ[i] = Jv;
-            sJ[kFaceBasis + i] = Js;
-        }
-        __syncthreads();
-        // (c) lift back: k += sigma M_ref^{-1} P^T J
-        if (row_ok) {
-            const T* L = (std::is_same<T, float>::value ? reinterpret_cast<const T*>(p.tab32)
-                                                        : reinterpret_cast<const T*>(p.tab)) +
-                         tx[kTixLift + f * 6 + info_oown(inf)] + i * kFaceBasis;
-            T lv = T(0), ls = T(0);
-            for (int m = 0; m < nF; ++m) {
-                lv += L[m] * T(sJ[m]);
-                ls += L[m] * T(sJ[kFaceBasis + m]);
+            double Jv, Js;
+            if (nb >= 0) {
+                const int qn = perm[info_onbr(inf) * kFaceQ + q];
+                const double* Vn = p.tab + txn[kTixVf + lfn];
+                double tv = 0, t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll 1
+                for (int c = 0; c < ncn; ++c) {
+                    const double phi = Vn[c * kFaceQ + qn];
+                    tv += phi * sn[c];
+                    t0 += phi * sn[kMaxCols + c];
+                    t1 += phi * sn[2 * kMaxCols + c];
+                    t2 += phi * sn[3 * kMaxCols + c];
+                }
+                Jv = -0.5 * (n0 * (os0 - t0) + n1 * (os1 - t1) + n2 * (os2 - t2));
+                Js = -0.5 * (ov - tv);
+            } else {
+                Jv = -(n0 * os0 + n1 * os1 + n2 * os2);
+                Js = 0;
             }
-            kv.add(T(sig) * lv);
-            const T c = T(sig) * ls / T(cinv);
-            for (int d = 0; d < 3; ++d) ks[d].add(c * T(n[d]));
+            const double w = p.face_w[q];
+            sf[(f * 2) * kFaceQ + q] = w * Jv;
+            sf[(f * 2 + 1) * kFaceQ + q] = w * Js;
         }
         __syncthreads();
     }
 
+    // ---- phase 3: lift, k += sigma_f sum_q (M_ref^{-1} phi(point q))_i flux(q)
     if (row_ok) {
+        const T ci = T(1.0 / cinv);
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            const int inf = p.info[f * E + e];
+            const int oo = info_oown(inf);
+            const T* Lf = tabT + tx[kTixLf + f];
+            T lv = T(0), ls = T(0);
+#pragma unroll 4
+            for (int q = 0; q < kFaceQ; ++q) {
+                const T l = Lf[perm[oo * kFaceQ + q] * NB + i];
+                lv += l * T(sf[(f * 2) * kFaceQ + q]);
+                ls += l * T(sf[(f * 2 + 1) * kFaceQ + q]);
+            }
+            const T sig = T(p.sigma[f * E + e]);
+            kv.add(sig * lv);
+            const T c = sig * ls * ci;
+            ks[0].add(c * T(p.normal[(0 * 4 + f) * E + e]));
+            ks[1].add(c * T(p.normal[(1 * 4 + f) * E + e]));
+            ks[2].add(c * T(p.normal[(2 * 4 + f) * E + e]));
+        }
         const double vals[4] = {double(kv.value()), double(ks[0].value()), double(ks[1].value()),
                                 double(ks[2].value())};
+#pragma unroll
         for (int f = 0; f < 4; ++f) {
             const int64_t o = base + f * NB + i;
             if (p.io.out_f32) static_cast<float*>(p.io.out)[o] = float(vals[f]);
@@ -480,36 +547,68 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     c->n_faces = n_total;
     c->n_internal = n_internal;
 
-    // reference tables of every order
+    // reference tables of every order (see the kernel): DrT, M_ref, and per
+    // local face the face columns, their traces at the face rule's points and
+    // the lift rows M_ref^{-1} phi(point); then the orientation permutations
     sm.ops.clear();
     for (int q = 1; q <= kMaxSimplexOrder; ++q) sm.ops.push_back(simplex_ops(q));
     std::vector<double> tab;
-    std::vector<int32_t> tix(size_t(kTixStride) * kMaxSimplexOrder, 0);
+    std::vector<int32_t> tix(size_t(kTixPerm) + 6 * kFaceQ, 0);
     std::vector<int32_t> cols;
     for (int q = 1; q <= kMaxSimplexOrder; ++q) {
         const SimplexOps& ops = sm.ops[q - 1];
+        const int nb = ops.nb;
         int32_t* t = tix.data() + (q - 1) * kTixStride;
-        t[kTixDr] = int32_t(tab.size());
-        tab.insert(tab.end(), ops.Dr.begin(), ops.Dr.end());
+        t[kTixDrT] = int32_t(tab.size());
+        for (int r = 0; r < 3; ++r)
+            for (int j = 0; j < nb; ++j)
+                for (int i = 0; i < nb; ++i) tab.push_back(ops.Dr[(size_t(r) * nb + i) * nb + j]);
         t[kTixM] = int32_t(tab.size());
         tab.insert(tab.end(), ops.Mref.begin(), ops.Mref.end());
-        for (int f = 0; f < 4; ++f)
-            for (int o = 0; o < 6; ++o) {
-                t[kTixProj + f * 6 + o] = int32_t(tab.size());
-                tab.insert(tab.end(), ops.proj[f][o].begin(), ops.proj[f][o].end());
-                t[kTixLift + f * 6 + o] = int32_t(tab.size());
-                tab.insert(tab.end(), ops.lift[f][o].begin(), ops.lift[f][o].end());
-            }
+        const std::vector<long double> Minv = inverse_ld(ops.Mref, nb);
         for (int f = 0; f < 4; ++f) {
+            std::vector<long double> phi(size_t(kFaceQ) * nb), grad(size_t(nb) * 3);
+            for (int qq = 0; qq < kFaceQ; ++qq) {
+                const long double tp[2] = {kFaceRulePts[qq][0], kFaceRulePts[qq][1]};
+                long double x[3];
+                tet_face_map(f, tp, x);
+                tet_basis(q, x, &phi[size_t(qq) * nb], grad.data());
+            }
+            require(int(ops.cols[f].size()) <= kMaxCols, HGDG_ERR_ARG, "too many face columns");
+            t[kTixVf + f] = int32_t(tab.size());
+            for (int col : ops.cols[f])
+                for (int qq = 0; qq < kFaceQ; ++qq) tab.push_back(double(phi[size_t(qq) * nb + col]));
+            t[kTixLf + f] = int32_t(tab.size());
+            for (int qq = 0; qq < kFaceQ; ++qq)
+                for (int i = 0; i < nb; ++i) {
+                    long double sum = 0;
+                    for (int col : ops.cols[f]) sum += Minv[size_t(i) * nb + col] * phi[size_t(qq) * nb + col];
+                    tab.push_back(double(sum));
+                }
             t[kTixNcol + f] = int32_t(ops.cols[f].size());
             t[kTixCols + f] = int32_t(cols.size());  // relative, fixed below
             cols.insert(cols.end(), ops.cols[f].begin(), ops.cols[f].end());
         }
     }
+    // face point q after orientation o of the reference triangle
+    for (int o = 0; o < 6; ++o)
+        for (int qq = 0; qq < kFaceQ; ++qq) {
+            const long double tp[2] = {kFaceRulePts[qq][0], kFaceRulePts[qq][1]};
+            long double m[2];
+            tri_orientation_map(o, tp, m);
+            int hit = -1;
+            for (int k = 0; k < kFaceQ; ++k)
+                if (fabsl(m[0] - kFaceRulePts[k][0]) < 1e-12L && fabsl(m[1] - kFaceRulePts[k][1]) < 1e-12L) hit = k;
+            require(hit >= 0, HGDG_ERR_ARG, "face rule is not symmetric under the orientation maps");
+            tix[size_t(kTixPerm) + o * kFaceQ + qq] = hit;
+        }
     const int32_t col_base = int32_t(tix.size());
     for (int q = 0; q < kMaxSimplexOrder; ++q)
         for (int f = 0; f < 4; ++f) tix[q * kTixStride + kTixCols + f] += col_base;
     tix.insert(tix.end(), cols.begin(), cols.end());
+    // face rule weights after the tables
+    sm.face_w_off = int64_t(tab.size());
+    tab.insert(tab.end(), kFaceRuleW, kFaceRuleW + kFaceQ);
 
     sm.d_tables.upload(tab, c->stream);
     const std::vector<float> tab32(tab.begin(), tab.end());  // lives until the synchronize below
@@ -559,6 +658,7 @@ void simplex_rhs(hgdg_ctx* c, const StageIO& io) {
     p.tab = sm.d_tables.ptr;
     p.tab32 = sm.d_tables32.ptr;
     p.tix = sm.d_tab_index.ptr;
+    p.face_w = sm.d_tables.ptr + sm.face_w_off;
     const int grid = p.group_block[4];
     if (grid > 0) {
         if (c->op.precision == HGDG_PREC_MIXED_FP32)
@@ -604,8 +704,17 @@ void simplex_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, d
 
 namespace {
 
-// L2 projector of order p on the reference tetrahedron: points of a
-// collapsed Gauss rule exact to degree 2p+3 and R[i][q] = sum_j
+// hpGEM's rule for a requested order (AllGaussQuadratureRules::getRule).
+void ref_tet_rule(int requested, std::vector<long double>& pts, std::vector<long double>& w) {
+    require(requested >= 0 && requested <= kRefTetRuleMaxOrder, HGDG_ERR_UNSUPPORTED,
+            "no tetrahedral rule of order " + std::to_string(requested));
+    const RefTetRule& r = kRefTetRules[requested];
+    pts.assign(r.pts, r.pts + 3 * r.n);
+    w.assign(r.w, r.w + r.n);
+}
+
+// L2 projector of order p on the reference tetrahedron with hpGEM's element
+// rule of order 2p+1 (as setInitialSolution integrates): R[i][q] = sum_j
 // (M_ref^{-1})_{ij} w_q phi_j(xi_q), so that the coefficients of the
 // projection of f on an affine element are sum_q R[i][q] f(x(xi_q)).
 struct TetProjector {
@@ -618,7 +727,7 @@ TetProjector tet_projector(int p) {
     TetProjector t;
     t.nb = tet_nb(p);
     std::vector<long double> pts, w;
-    tet_rule(2 * p + 3, pts, w);
+    ref_tet_rule(2 * p + 1, pts, w);
     t.nq = int(w.size());
     const int nb = t.nb, nq = t.nq;
     std::vector<long double> phi(size_t(nq) * nb), grad(size_t(nb) * 3), M(size_t(nb) * nb, 0.0L);
@@ -709,9 +818,8 @@ void simplex_project_host(hgdg_ctx* c, Vec& v, int field, int n_modes, const dou
 }
 
 // Error against f(x - shift) for sine modes f, as tensor_error_modes: L2 with
-// a rule exact to degree 2p+2, maximum over the points of a rule exact to
-// degree 2p+1 (collapsed Gauss rules, so the figures agree with hpGEM's
-// tetrahedral rules to quadrature accuracy). Host side, after a download.
+// hpGEM's tetrahedral rule of order 2p+2, maximum over the points of the
+// element rule (order 2p+1). Host side, after a download.
 void simplex_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, const double* amp, const double* k,
                          const double* shift, double* l2, double* max_abs) {
     SimplexMesh& sm = c->sm;
@@ -724,7 +832,7 @@ void simplex_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, cons
     auto make = [](int p, int degree) {
         Pts t;
         std::vector<long double> pts, w;
-        tet_rule(degree, pts, w);
+        ref_tet_rule(degree, pts, w);
         t.nq = int(w.size());
         t.nb = tet_nb(p);
         t.x.assign(pts.begin(), pts.end());

@@ -126,7 +126,6 @@ __device__ __forceinline__ double face_jump(uint32_t m, double own, double other
 template <int DIM, int N1, typename T>
 __device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk) {
     using S = Pencil<DIM, N1>;
-    constexpr int NF = 2 * DIM;
     __shared__ double s_w[S::EPB * S::NB];  // FP64 stage input of the block's elements
     __shared__ T s_D[N1 * N1];
     __shared__ T s_Mi[2 * N1];  // s_Mi[s * N1 + a] = Mi(a, s)
@@ -135,58 +134,23 @@ __device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk)
     if (tid < 2 * N1) s_Mi[tid] = T(p.Mi[(tid % N1) * N1 + tid / N1]);
     const int le = tid / S::NP;
     const int pc = tid % S::NP;
-    const int64_t first = blk * S::EPB;
-    const int64_t e = first + le;
+    const int64_t e = blk * S::EPB + le;
     const bool active = (le < S::EPB) && (e < p.n_elem);
     const int ib = pc % N1;                    // y index of this pencil
     const int ic = DIM == 3 ? pc / N1 : 0;     // z index of this pencil
     double* sw = s_w + le * S::NB;
-
-    // Every HBM read of the thread is issued before the first wait: the
-    // upwind modes, neighbour ids and geometry, the own pencil, and the
-    // inflow traces of neighbours outside this block (those inside are read
-    // from LDS after the barrier). Per face f = 2d + s the trace needed is
-    // coefficient 1-s along d: one value for x faces (ox), a pencil for y/z.
-    uint32_t md = 0;
-    int64_t nb[NF];
-    bool from_lds[NF];
     double w[N1];
-    double ox[2] = {0.0, 0.0};
-    double oyz[NF - 2][N1];
-    T ah[DIM];
     if (active) {
-        md = p.mode[e];
-#pragma unroll
-        for (int f = 0; f < NF; ++f) nb[f] = p.nbr[f * p.n_elem + e];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) ah[d] = T(p.ahat[d * p.n_elem + e]);
         combine_pencil<N1>(p.io, e * S::NB + pc * N1, w);
-#pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            const uint32_t m = (md >> (3 * f)) & 7u;
-            const bool need = (m == MODE_NBR || m == MODE_AVG);
-            const int64_t lnb = nb[f] - first;
-            from_lds[f] = need && lnb >= 0 && lnb < S::EPB;
-            if (!need || from_lds[f]) continue;
-            const int s = f & 1;
-            if (f < 2) {
-                const int64_t idx = nb[f] * S::NB + pc * N1 + (1 - s);
-                double v = p.io.coef[0] * load_in(p.io, 0, idx);
-                for (int i = 1; i < kMaxIn; ++i) {
-                    if (i >= p.io.n_in) break;
-                    v += p.io.coef[i] * load_in(p.io, i, idx);
-                }
-                ox[s] = v;
-            } else {
-                const int nbr_pc = f < 4 ? (1 - s) + N1 * ic : ib + N1 * (1 - s);
-                combine_pencil<N1>(p.io, nb[f] * S::NB + nbr_pc * N1, oyz[f - 2]);
-            }
-        }
 #pragma unroll
         for (int a = 0; a < N1; ++a) sw[pc * N1 + a] = w[a];
     }
     __syncthreads();
     if (!active) return;
+
+    T ah[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) ah[d] = T(p.ahat[d * p.n_elem + e]);
 
     // volume (strong form): -sum_d ahat_d (Dr along d) w
     T vol[N1];
@@ -218,12 +182,26 @@ __device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk)
     for (int a = 0; a < N1; ++a) acc[a] = Acc<T>(vol[a]);
 
     // faces: -(2s-1) ahat_d Mi(idx_d, s) (T_{d,s} - own_{d,s}); jumps in FP64
+    const uint32_t md = p.mode[e];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {  // x faces: one trace value per pencil
         const uint32_t m = (md >> (3 * s)) & 7u;
         if (m == MODE_OWN) continue;
-        double other = ox[s];
-        if (from_lds[s]) other = s_w[(nb[s] - first) * S::NB + pc * N1 + (1 - s)];
+        double other = 0.0;
+        if (m == MODE_NBR || m == MODE_AVG) {
+            const int64_t nb = p.nbr[s * p.n_elem + e];
+            const int64_t lnb = nb - blk * S::EPB;
+            if (lnb >= 0 && lnb < S::EPB) {
+                other = s_w[lnb * S::NB + pc * N1 + (1 - s)];  // neighbour in this block
+            } else {
+                const int64_t idx = nb * S::NB + pc * N1 + (1 - s);
+                other = p.io.coef[0] * load_in(p.io, 0, idx);
+                for (int i = 1; i < kMaxIn; ++i) {
+                    if (i >= p.io.n_in) break;
+                    other += p.io.coef[i] * load_in(p.io, i, idx);
+                }
+            }
+        }
         const T jump = T(face_jump(m, w[s], other));
         const T sc = (s ? T(-1) : T(1)) * ah[0] * jump;
 #pragma unroll
@@ -239,11 +217,18 @@ __device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk)
             const int own_pc = d == 1 ? s + N1 * ic : ib + N1 * s;
             const int nbr_pc = d == 1 ? (1 - s) + N1 * ic : ib + N1 * (1 - s);
             double other[N1];
+            if (m == MODE_NBR || m == MODE_AVG) {
+                const int64_t nb = p.nbr[f * p.n_elem + e];
+                const int64_t lnb = nb - blk * S::EPB;
+                if (lnb >= 0 && lnb < S::EPB) {
 #pragma unroll
-            for (int a = 0; a < N1; ++a) other[a] = (m == MODE_NBR || m == MODE_AVG) ? oyz[f - 2][a] : 0.0;
-            if (from_lds[f]) {
+                    for (int a = 0; a < N1; ++a) other[a] = s_w[lnb * S::NB + nbr_pc * N1 + a];
+                } else {
+                    combine_pencil<N1>(p.io, nb * S::NB + nbr_pc * N1, other);
+                }
+            } else {
 #pragma unroll
-                for (int a = 0; a < N1; ++a) other[a] = s_w[(nb[f] - first) * S::NB + nbr_pc * N1 + a];
+                for (int a = 0; a < N1; ++a) other[a] = 0.0;
             }
             const int my = d == 1 ? ib : ic;
             const T sc = (s ? T(-1) : T(1)) * ah[d] * s_Mi[s * N1 + my];

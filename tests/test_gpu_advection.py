@@ -152,7 +152,7 @@ def test_dormand_prince_reference_error_table_on_gpu():
         dev.close()
 
 
-@pytest.mark.parametrize("n,p", [([3, 4], 2), ([3, 3, 3], 2), ([2, 3, 2], 3)])
+@pytest.mark.parametrize("n,p", [([3, 4], 2), ([3, 3, 3], 2), ([3, 4, 3], 3)])
 def test_max_error_matches_pointwise_evaluation(n, p):
     """computeMaxError: the largest |f - u_h| at the element rule's (p+1)^dim
     Gauss points, checked against the oracle's basis evaluated at those
