@@ -31,18 +31,17 @@ namespace hgdg {
 namespace {
 
 // int32 table index per order (p-1): offsets into the FP64 (and the FP32)
-// table arrays, and per local face the number of face columns and the offset
-// of their list; after the four orders, the orientation permutations of the
-// face rule points.
+// table arrays and, per local face, the offset of its list of face columns.
+// The face tables are stored once per orientation o of the face rule points
+// (the points permuted by the orientation map), so no lookup sits between a
+// thread and its table reads.
 constexpr int kTixDrT = 0;           // [3][nb][nb]: DrT[r][j][i] = Dr_r(i, j)
 constexpr int kTixM = 1;             // [nb][nb]: M_ref
-constexpr int kTixVf = 2;            // + f: [ncol][kFaceQ]: trace of face column c at face point q
-constexpr int kTixLf = 6;            // + f: [kFaceQ][nb]: (M_ref^{-1} phi(face point q))_i
-constexpr int kTixNcol = 10;         // + f
-constexpr int kTixCols = 14;         // + f
-constexpr int kTixStride = 18;
-constexpr int kTixPerm = 4 * kTixStride;  // [6][kFaceQ]: face point index after orientation o
-constexpr int kMaxCols = 15;              // face columns of an order-4 element
+constexpr int kTixVf = 2;            // + f: [6][ncol][kFaceQ]: trace of face column c at point o(q)
+constexpr int kTixLf = 6;            // + f: [6][kFaceQ][nb]: (M_ref^{-1} phi(point o(q)))_i
+constexpr int kTixCols = 10;         // + f
+constexpr int kTixStride = 14;
+constexpr int kMaxCols = 15;         // face columns of an order-4 element
 
 // packed per-face info
 __host__ __device__ constexpr int info_lf(int v) { return v & 3; }
@@ -96,6 +95,7 @@ struct SimplexParams {
     int64_t group_start[5];
     int64_t group_dof[5];
     int32_t group_block[5];
+    const int4* blocks;    // per block: order, first device element, element count, unused
     const double* jinv;    // [9][E]: (J^{-1})_{e,d} at [(3e + d) E + elem]
     const double* cinv;    // [E]
     const double* sigma;   // [4][E]: face area ratio / |det J|
@@ -105,22 +105,25 @@ struct SimplexParams {
     const double* tab;
     const float* tab32;    // the same tables in FP32 (mixed path), same offsets
     const int32_t* tix;
-    const double* face_w;  // [kFaceQ] face rule weights
+    double face_w[kFaceQ]; // face rule weights
 };
 
 __host__ __device__ constexpr int simplex_nb(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
-__host__ __device__ constexpr int simplex_tpe(int p) { return simplex_nb(p) > 16 ? simplex_nb(p) : 16; }
+__host__ __device__ constexpr int simplex_ncol(int p) { return (p + 1) * (p + 2) / 2; }
+// threads per element: 32 up to order 3 (a wavefront holds two elements), nb at order 4
+__host__ __device__ constexpr int simplex_tpe(int p) { return simplex_nb(p) > 32 ? simplex_nb(p) : 32; }
 
 template <int P>
 struct SimplexShape {
     static constexpr int NB = simplex_nb(P);
+    static constexpr int NCOL = simplex_ncol(P);
     static constexpr int TPE = simplex_tpe(P);  // threads per element
     static constexpr int EPB = kBlock / TPE;
     // LDS carve-up (doubles)
     static constexpr int U = 0;                             // [EPB][4][NB] stage input
     static constexpr int G = U + EPB * 4 * NB;              // [EPB][3][NB] J^{-T}-rotated s
-    static constexpr int N = G + EPB * 3 * NB;              // [EPB][4 fields][kMaxCols] one neighbour's face columns
-    static constexpr int F = N + EPB * 4 * kMaxCols;        // [EPB][4 faces][2][kFaceQ] weighted fluxes at face points
+    static constexpr int N = G + EPB * 3 * NB;              // [EPB][4 faces][4 fields][kMaxCols] neighbours' face columns
+    static constexpr int F = N + EPB * 16 * kMaxCols;       // [EPB][4 faces][2][kFaceQ] weighted fluxes at face points
     static constexpr int TOTAL = F + EPB * 8 * kFaceQ;
 };
 
@@ -133,34 +136,50 @@ constexpr int lds_doubles() {
     return m;
 }
 
-// Tetrahedral elements of order P, one block of EPB elements, TPE threads per
-// element, three phases separated by barriers:
-//  1. stage input (all four fields) and J^{-T}-rotated s into LDS;
-//  2. threads of a row: the volume terms; then face by face, the neighbour's
+// Traces of the four fields of a neighbour of order PN at face point q, from
+// its face columns in LDS and its face table for orientation o.
+template <int PN>
+__device__ __forceinline__ void nbr_traces(const SimplexParams& p, int lfn, int o, int q, const double* cols4,
+                                           double (&t)[4]) {
+    constexpr int NC = simplex_ncol(PN);
+    const double* V = p.tab + p.tix[(PN - 1) * kTixStride + kTixVf + lfn] + (o * NC) * kFaceQ + q;
+    t[0] = t[1] = t[2] = t[3] = 0;
+#pragma unroll 3
+    for (int c = 0; c < NC; ++c) {
+        const double phi = V[c * kFaceQ];
+#pragma unroll
+        for (int fld = 0; fld < 4; ++fld) t[fld] += phi * cols4[fld * kMaxCols + c];
+    }
+}
+
+// Tetrahedral elements of order P: one block of up to EPB consecutive device
+// elements, TPE threads per element, three phases separated by barriers:
+//  1. the stage input (all four fields) and J^{-T}-rotated s into LDS, and the
 //     face columns (the basis functions that do not vanish on the shared
-//     face) into LDS and, one thread per face rule point, both traces (the
-//     neighbour's through the orientation permutation) and the central / wall
-//     flux times the weight;
+//     face) of all four neighbours, all loads in flight together;
+//  2. threads of a row: the volume terms; all threads over (face, point)
+//     items: both traces at the face rule's points (tables pre-permuted per
+//     orientation) and the central / wall flux times the weight;
 //  3. threads of a row: the lift of the point fluxes, M_ref^{-1} phi(point).
 // The face rule (order 8) integrates every face term exactly for p <= 4 on
 // both sides, so this is the reference's operator up to rounding. T = double:
 // FP64; T = float: the volume contractions and the lifts in FP32 on FP32
 // tables, compensated per-row sums, traces and fluxes in FP64.
 template <int P, typename T>
-__device__ void simplex_acoustic_block(const SimplexParams& p, int blk, double* lds) {
+__device__ void simplex_acoustic_block(const SimplexParams& p, int64_t first, int count, double* lds) {
     using S = SimplexShape<P>;
     constexpr int NB = S::NB;
     const int tid = threadIdx.x;
     const int le = tid / S::TPE;
     const int i = tid % S::TPE;
-    const int64_t e = p.group_start[P - 1] + int64_t(blk) * S::EPB + le;
-    const bool elem_ok = (le < S::EPB) && (e < p.group_start[P]);
+    const int64_t e = first + le;
+    const bool elem_ok = le < count;
     const bool row_ok = elem_ok && (i < NB);
     const int64_t E = p.n_elem;
     const int64_t base = p.group_dof[P - 1] + (e - p.group_start[P - 1]) * 4 * NB;
     double* su = lds + S::U + le * 4 * NB;
     double* sg = lds + S::G + le * 3 * NB;
-    double* sn = lds + S::N + le * 4 * kMaxCols;
+    double* sn = lds + S::N + le * 16 * kMaxCols;
     double* sf = lds + S::F + le * 8 * kFaceQ;
     const int32_t* tx = p.tix + (P - 1) * kTixStride;
     const T* tabT = std::is_same<T, float>::value ? reinterpret_cast<const T*>(p.tab32)
@@ -169,7 +188,13 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int blk, double* 
     // ---- phase 1
     double Ji[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     double cinv = 1.0;
+    int32_t nbf[4] = {-1, -1, -1, -1}, inf[4] = {0, 0, 0, 0};
     if (elem_ok) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            nbf[f] = p.nbr[f * E + e];
+            inf[f] = p.info[f * E + e];
+        }
 #pragma unroll
         for (int q = 0; q < 9; ++q) Ji[q] = p.jinv[q * E + e];
         cinv = p.cinv[e];
@@ -183,6 +208,21 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int blk, double* 
         }
 #pragma unroll
         for (int r = 0; r < 3; ++r) sg[r * NB + i] = Ji[r * 3] * u[1] + Ji[r * 3 + 1] * u[2] + Ji[r * 3 + 2] * u[3];
+    }
+    if (elem_ok) {
+        // items (face, field, column): 4 x 4 x kMaxCols
+#pragma unroll 1
+        for (int it = i; it < 16 * kMaxCols; it += S::TPE) {
+            const int f = it / (4 * kMaxCols), fld = (it / kMaxCols) % 4, c = it % kMaxCols;
+            const int32_t nb = f == 0 ? nbf[0] : f == 1 ? nbf[1] : f == 2 ? nbf[2] : nbf[3];
+            const int fi = f == 0 ? inf[0] : f == 1 ? inf[1] : f == 2 ? inf[2] : inf[3];
+            const int pn = info_pn(fi);
+            if (nb < 0 || c >= simplex_ncol(pn)) continue;
+            const int nbn = simplex_nb(pn);
+            const int col = p.tix[p.tix[(pn - 1) * kTixStride + kTixCols + info_lf(fi)] + c];
+            const int64_t nbase = p.group_dof[pn - 1] + (nb - p.group_start[pn - 1]) * 4 * nbn;
+            sn[(f * 4 + fld) * kMaxCols + c] = combine(p.io, nbase + fld * nbn + col);
+        }
     }
     __syncthreads();
 
@@ -208,88 +248,59 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int blk, double* 
             ks[d] = Acc<T>((T(Ji[d]) * b[0] + T(Ji[3 + d]) * b[1] + T(Ji[6 + d]) * b[2]) * ci);
     }
 
-    // ---- phase 2b: fluxes at the face points, one face at a time: the
-    // neighbour's face columns (the basis functions that do not vanish on the
-    // shared face) into LDS, then both traces at the points
-    const int32_t* perm = p.tix + kTixPerm;
+    // ---- phase 2b: fluxes at the face points, items (face, point)
+    if (elem_ok) {
 #pragma unroll 1
-    for (int f = 0; f < 4; ++f) {
-        int64_t nb = -1;
-        int inf = 0;
-        if (elem_ok) {
-            nb = p.nbr[f * E + e];
-            inf = p.info[f * E + e];
-        }
-        const int pn = info_pn(inf), lfn = info_lf(inf);
-        const int32_t* txn = p.tix + (pn > 0 ? pn - 1 : 0) * kTixStride;
-        const int ncn = nb >= 0 ? txn[kTixNcol + lfn] : 0;
-        if (nb >= 0) {
-            const int nbn = simplex_nb(pn);
-            const int64_t nbase = p.group_dof[pn - 1] + (nb - p.group_start[pn - 1]) * 4 * nbn;
-            const int32_t* colsn = p.tix + txn[kTixCols + lfn];
-#pragma unroll 1
-            for (int it = i; it < 4 * ncn; it += S::TPE) {
-                const int fld = it / ncn, c = it % ncn;
-                sn[fld * kMaxCols + c] = combine(p.io, nbase + fld * nbn + colsn[c]);
-            }
-        }
-        __syncthreads();
-        if (elem_ok && i < kFaceQ) {
-            const int q = i;
+        for (int it = i; it < 4 * kFaceQ; it += S::TPE) {
+            const int f = it / kFaceQ, q = it % kFaceQ;
+            const int32_t nb = f == 0 ? nbf[0] : f == 1 ? nbf[1] : f == 2 ? nbf[2] : nbf[3];
+            const int fi = f == 0 ? inf[0] : f == 1 ? inf[1] : f == 2 ? inf[2] : inf[3];
             const double n0 = p.normal[(0 * 4 + f) * E + e], n1 = p.normal[(1 * 4 + f) * E + e],
                          n2 = p.normal[(2 * 4 + f) * E + e];
-            const int qo = perm[info_oown(inf) * kFaceQ + q];
-            const int nco = tx[kTixNcol + f];
+            const double* Vo = p.tab + tx[kTixVf + f] + (info_oown(fi) * S::NCOL) * kFaceQ + q;
             const int32_t* colso = p.tix + tx[kTixCols + f];
-            const double* Vo = p.tab + tx[kTixVf + f];
-            double ov = 0, os0 = 0, os1 = 0, os2 = 0;
-#pragma unroll 1
-            for (int c = 0; c < nco; ++c) {
-                const double phi = Vo[c * kFaceQ + qo];
+            double o4[4] = {0, 0, 0, 0};
+#pragma unroll 3
+            for (int c = 0; c < S::NCOL; ++c) {
+                const double phi = Vo[c * kFaceQ];
                 const int col = colso[c];
-                ov += phi * su[col];
-                os0 += phi * su[NB + col];
-                os1 += phi * su[2 * NB + col];
-                os2 += phi * su[3 * NB + col];
+#pragma unroll
+                for (int fld = 0; fld < 4; ++fld) o4[fld] += phi * su[fld * NB + col];
             }
             double Jv, Js;
             if (nb >= 0) {
-                const int qn = perm[info_onbr(inf) * kFaceQ + q];
-                const double* Vn = p.tab + txn[kTixVf + lfn];
-                double tv = 0, t0 = 0, t1 = 0, t2 = 0;
-#pragma unroll 1
-                for (int c = 0; c < ncn; ++c) {
-                    const double phi = Vn[c * kFaceQ + qn];
-                    tv += phi * sn[c];
-                    t0 += phi * sn[kMaxCols + c];
-                    t1 += phi * sn[2 * kMaxCols + c];
-                    t2 += phi * sn[3 * kMaxCols + c];
+                double t4[4];
+                const double* cols4 = sn + f * 4 * kMaxCols;
+                const int lfn = info_lf(fi), on = info_onbr(fi);
+                switch (info_pn(fi)) {
+                    case 1: nbr_traces<1>(p, lfn, on, q, cols4, t4); break;
+                    case 2: nbr_traces<2>(p, lfn, on, q, cols4, t4); break;
+                    case 3: nbr_traces<3>(p, lfn, on, q, cols4, t4); break;
+                    default: nbr_traces<4>(p, lfn, on, q, cols4, t4); break;
                 }
-                Jv = -0.5 * (n0 * (os0 - t0) + n1 * (os1 - t1) + n2 * (os2 - t2));
-                Js = -0.5 * (ov - tv);
+                Jv = -0.5 * (n0 * (o4[1] - t4[1]) + n1 * (o4[2] - t4[2]) + n2 * (o4[3] - t4[3]));
+                Js = -0.5 * (o4[0] - t4[0]);
             } else {
-                Jv = -(n0 * os0 + n1 * os1 + n2 * os2);
+                Jv = -(n0 * o4[1] + n1 * o4[2] + n2 * o4[3]);
                 Js = 0;
             }
             const double w = p.face_w[q];
             sf[(f * 2) * kFaceQ + q] = w * Jv;
             sf[(f * 2 + 1) * kFaceQ + q] = w * Js;
         }
-        __syncthreads();
     }
+    __syncthreads();
 
-    // ---- phase 3: lift, k += sigma_f sum_q (M_ref^{-1} phi(point q))_i flux(q)
+    // ---- phase 3: lift, k += sigma_f sum_q (M_ref^{-1} phi(point o(q)))_i flux(q)
     if (row_ok) {
         const T ci = T(1.0 / cinv);
-#pragma unroll
+#pragma unroll 1
         for (int f = 0; f < 4; ++f) {
-            const int inf = p.info[f * E + e];
-            const int oo = info_oown(inf);
-            const T* Lf = tabT + tx[kTixLf + f];
+            const T* Lf = tabT + tx[kTixLf + f] + (info_oown(inf[f]) * kFaceQ) * NB + i;
             T lv = T(0), ls = T(0);
 #pragma unroll 4
             for (int q = 0; q < kFaceQ; ++q) {
-                const T l = Lf[perm[oo * kFaceQ + q] * NB + i];
+                const T l = Lf[q * NB];
                 lv += l * T(sf[(f * 2) * kFaceQ + q]);
                 ls += l * T(sf[(f * 2 + 1) * kFaceQ + q]);
             }
@@ -311,14 +322,21 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int blk, double* 
     }
 }
 
+// Blocks follow a schedule built on the host: the mesh is cut into slabs of
+// consecutive elements and, slab after slab, each order's elements of the
+// slab fill blocks of that order. With the XCD renumbering, an XCD works
+// through neighbouring slabs, so an element's neighbours of every order are
+// read while they are still in its L2.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void simplex_acoustic_rhs(const SimplexParams p) {
     __shared__ double lds[lds_doubles()];
-    const int b = int(xcd_block(blockIdx.x, gridDim.x));
-    if (b < p.group_block[1]) simplex_acoustic_block<1, T>(p, b - p.group_block[0], lds);
-    else if (b < p.group_block[2]) simplex_acoustic_block<2, T>(p, b - p.group_block[1], lds);
-    else if (b < p.group_block[3]) simplex_acoustic_block<3, T>(p, b - p.group_block[2], lds);
-    else simplex_acoustic_block<4, T>(p, b - p.group_block[3], lds);
+    const int4 bl = p.blocks[xcd_block(blockIdx.x, gridDim.x)];
+    switch (bl.x) {
+        case 1: simplex_acoustic_block<1, T>(p, bl.y, bl.z, lds); break;
+        case 2: simplex_acoustic_block<2, T>(p, bl.y, bl.z, lds); break;
+        case 3: simplex_acoustic_block<3, T>(p, bl.y, bl.z, lds); break;
+        default: simplex_acoustic_block<4, T>(p, bl.y, bl.z, lds); break;
+    }
 }
 
 // Per-element |det J| sum_f dx_f^T M_ref dx_f (plain L2 of all fields) and
@@ -548,12 +566,24 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     c->n_internal = n_internal;
 
     // reference tables of every order (see the kernel): DrT, M_ref, and per
-    // local face the face columns, their traces at the face rule's points and
-    // the lift rows M_ref^{-1} phi(point); then the orientation permutations
+    // local face and orientation o the traces of the face columns at the
+    // permuted face rule points and the lift rows M_ref^{-1} phi(point)
+    int perm[6][kFaceQ];  // face point q after orientation o of the reference triangle
+    for (int o = 0; o < 6; ++o)
+        for (int qq = 0; qq < kFaceQ; ++qq) {
+            const long double tp[2] = {kFaceRulePts[qq][0], kFaceRulePts[qq][1]};
+            long double m[2];
+            tri_orientation_map(o, tp, m);
+            int hit = -1;
+            for (int k = 0; k < kFaceQ; ++k)
+                if (fabsl(m[0] - kFaceRulePts[k][0]) < 1e-12L && fabsl(m[1] - kFaceRulePts[k][1]) < 1e-12L) hit = k;
+            require(hit >= 0, HGDG_ERR_ARG, "face rule is not symmetric under the orientation maps");
+            perm[o][qq] = hit;
+        }
     sm.ops.clear();
     for (int q = 1; q <= kMaxSimplexOrder; ++q) sm.ops.push_back(simplex_ops(q));
     std::vector<double> tab;
-    std::vector<int32_t> tix(size_t(kTixPerm) + 6 * kFaceQ, 0);
+    std::vector<int32_t> tix(size_t(kTixStride) * kMaxSimplexOrder, 0);
     std::vector<int32_t> cols;
     for (int q = 1; q <= kMaxSimplexOrder; ++q) {
         const SimplexOps& ops = sm.ops[q - 1];
@@ -574,41 +604,56 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
                 tet_face_map(f, tp, x);
                 tet_basis(q, x, &phi[size_t(qq) * nb], grad.data());
             }
-            require(int(ops.cols[f].size()) <= kMaxCols, HGDG_ERR_ARG, "too many face columns");
+            require(int(ops.cols[f].size()) == simplex_ncol(q), HGDG_ERR_ARG, "unexpected face column count");
             t[kTixVf + f] = int32_t(tab.size());
-            for (int col : ops.cols[f])
-                for (int qq = 0; qq < kFaceQ; ++qq) tab.push_back(double(phi[size_t(qq) * nb + col]));
+            for (int o = 0; o < 6; ++o)
+                for (int col : ops.cols[f])
+                    for (int qq = 0; qq < kFaceQ; ++qq) tab.push_back(double(phi[size_t(perm[o][qq]) * nb + col]));
             t[kTixLf + f] = int32_t(tab.size());
-            for (int qq = 0; qq < kFaceQ; ++qq)
-                for (int i = 0; i < nb; ++i) {
-                    long double sum = 0;
-                    for (int col : ops.cols[f]) sum += Minv[size_t(i) * nb + col] * phi[size_t(qq) * nb + col];
-                    tab.push_back(double(sum));
-                }
-            t[kTixNcol + f] = int32_t(ops.cols[f].size());
+            for (int o = 0; o < 6; ++o)
+                for (int qq = 0; qq < kFaceQ; ++qq)
+                    for (int i = 0; i < nb; ++i) {
+                        long double sum = 0;
+                        for (int col : ops.cols[f])
+                            sum += Minv[size_t(i) * nb + col] * phi[size_t(perm[o][qq]) * nb + col];
+                        tab.push_back(double(sum));
+                    }
             t[kTixCols + f] = int32_t(cols.size());  // relative, fixed below
             cols.insert(cols.end(), ops.cols[f].begin(), ops.cols[f].end());
         }
     }
-    // face point q after orientation o of the reference triangle
-    for (int o = 0; o < 6; ++o)
-        for (int qq = 0; qq < kFaceQ; ++qq) {
-            const long double tp[2] = {kFaceRulePts[qq][0], kFaceRulePts[qq][1]};
-            long double m[2];
-            tri_orientation_map(o, tp, m);
-            int hit = -1;
-            for (int k = 0; k < kFaceQ; ++k)
-                if (fabsl(m[0] - kFaceRulePts[k][0]) < 1e-12L && fabsl(m[1] - kFaceRulePts[k][1]) < 1e-12L) hit = k;
-            require(hit >= 0, HGDG_ERR_ARG, "face rule is not symmetric under the orientation maps");
-            tix[size_t(kTixPerm) + o * kFaceQ + qq] = hit;
-        }
     const int32_t col_base = int32_t(tix.size());
     for (int q = 0; q < kMaxSimplexOrder; ++q)
         for (int f = 0; f < 4; ++f) tix[q * kTixStride + kTixCols + f] += col_base;
     tix.insert(tix.end(), cols.begin(), cols.end());
-    // face rule weights after the tables
-    sm.face_w_off = int64_t(tab.size());
-    tab.insert(tab.end(), kFaceRuleW, kFaceRuleW + kFaceQ);
+
+    // block schedule: slabs of consecutive mesh elements, each order's
+    // elements of a slab in blocks of that order's size
+    {
+        const int64_t slab = std::max<int64_t>(1024, (ne + 255) / 256);
+        static const int epb[4] = {SimplexShape<1>::EPB, SimplexShape<2>::EPB, SimplexShape<3>::EPB,
+                                   SimplexShape<4>::EPB};
+        std::vector<int32_t> blocks;
+        int64_t cursor[4] = {sm.group_start[0], sm.group_start[1], sm.group_start[2], sm.group_start[3]};
+        for (int64_t lo = 0; lo < ne; lo += slab) {
+            const int64_t hi = std::min(ne, lo + slab);
+            for (int g = 0; g < 4; ++g) {
+                // device elements of order g+1 with mesh index below hi (mesh order inside a group)
+                int64_t end = cursor[g];
+                while (end < sm.group_start[g + 1] && sm.mesh_of[end] < hi) ++end;
+                for (int64_t b = cursor[g]; b < end; b += epb[g]) {
+                    blocks.push_back(g + 1);
+                    blocks.push_back(int32_t(b));
+                    blocks.push_back(int32_t(std::min<int64_t>(epb[g], end - b)));
+                    blocks.push_back(0);
+                }
+                cursor[g] = end;
+            }
+        }
+        sm.n_blocks = int64_t(blocks.size() / 4);
+        sm.d_blocks.upload(blocks, c->stream);
+        HGDG_HIP(hipStreamSynchronize(c->stream));  // `blocks` goes out of scope
+    }
 
     sm.d_tables.upload(tab, c->stream);
     const std::vector<float> tab32(tab.begin(), tab.end());  // lives until the synchronize below
@@ -658,8 +703,9 @@ void simplex_rhs(hgdg_ctx* c, const StageIO& io) {
     p.tab = sm.d_tables.ptr;
     p.tab32 = sm.d_tables32.ptr;
     p.tix = sm.d_tab_index.ptr;
-    p.face_w = sm.d_tables.ptr + sm.face_w_off;
-    const int grid = p.group_block[4];
+    p.blocks = reinterpret_cast<const int4*>(sm.d_blocks.ptr);
+    for (int q = 0; q < kFaceQ; ++q) p.face_w[q] = kFaceRuleW[q];
+    const int grid = int(sm.n_blocks);
     if (grid > 0) {
         if (c->op.precision == HGDG_PREC_MIXED_FP32)
             hipLaunchKernelGGL(simplex_acoustic_rhs<float>, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
