@@ -34,6 +34,14 @@ N_CELLS = 32
 P = 3
 VELOCITY = [1.0, 0.7, 0.3]
 SEED = 1
+# HBM bytes per launch of tensor_advection_rhs<3,4,T> on this workload, from
+# `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of
+# `bench.py --profile-workload`, each scaled by what the counter reads for a
+# plain copy of one 16 MiB vector (FETCH_SIZE x 1.9977, WRITE_SIZE x 1); the
+# commands and per-kernel figures are in
+# profiles/rhs_kernels_32cube_p3_and_tet24.md. Counters cannot be read from
+# inside this process, so the figure is reported as measured there.
+MEASURED_TRAFFIC = {"double": 57.61e6, "float": 37.31e6}
 # dt = CFL h / |a|. At CFL 0.1 the RK4 step of this p = 3 operator has
 # spectral radius ~1.47 (power iteration on the oracle, 4^3 periodic mesh), so
 # round-off grows ~1e8-fold over 50 steps; 0.05 lies inside the stability
@@ -382,7 +390,8 @@ def main():
     mixed_ok = gpu["mixed"]["rel_l2_err_vs_fp64"] <= 1e-6
     head = gpu["mixed"] if mixed_ok else gpu["fp64"]
     roof = {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": head["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
+            "frac": head["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": MEASURED_TRAFFIC["float" if mixed_ok else "double"],
             "kernel": "tensor_advection_rhs<3,4," + ("float" if mixed_ok else "double") + ">",
             "bytes_alg_per_launch": head["bytes_alg_per_rhs"]}
     out = {
