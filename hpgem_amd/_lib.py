@@ -65,6 +65,8 @@ SIGNATURES = [
     ("hgdg_get_stream", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     ("hgdg_basis_size", ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     ("hgdg_basis_eval", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _D]),
+    ("hgdg_hcurl_basis_size", ctypes.c_int, [ctypes.c_int]),
+    ("hgdg_hcurl_basis_eval", ctypes.c_int, [ctypes.c_int, _D, _D, _D]),
     ("hgdg_structured_mesh", ctypes.c_int, [ctypes.c_int, _I32, _D, _D, _I32, ctypes.c_int, _I64, _I32, _I32,
                                             _D, _I64]),
 ]

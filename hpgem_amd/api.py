@@ -87,6 +87,16 @@ def basis_eval(shape, p, point):
     return out
 
 
+def hcurl_basis_eval(p, point):
+    """Values and curls ([n][3] each) of hpGEM's Ainsworth-Coyle set of order p."""
+    lib = _lib.load()
+    n = lib.hgdg_hcurl_basis_size(int(p))
+    v, c = np.zeros((n, 3)), np.zeros((n, 3))
+    x = np.ascontiguousarray(point, dtype=np.float64)
+    check(lib.hgdg_hcurl_basis_eval(int(p), dptr(x), dptr(v), dptr(c)))
+    return v, c
+
+
 class HpgemAPIDevice:
     """Explicit DG solver whose RHS, mass solve and RK loop run on one GPU.
 

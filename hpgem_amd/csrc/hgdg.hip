@@ -745,6 +745,23 @@ int hgdg_structured_mesh(int dim, const int32_t* n, const double* lo, const doub
     });
 }
 
+int hgdg_hcurl_basis_size(int p) { return hgdg::hcurl_tet_size(p); }
+
+int hgdg_hcurl_basis_eval(int p, const double* point, double* values, double* curls) {
+    return run([&] {
+        require(point && values && curls, HGDG_ERR_ARG, "NULL argument");
+        require(p >= 1 && p <= 2, HGDG_ERR_UNSUPPORTED, "H(curl) tetrahedral orders 1 and 2");
+        const int n = hgdg::hcurl_tet_size(p);
+        const long double x[3] = {point[0], point[1], point[2]};
+        std::vector<long double> v(size_t(n) * 3), c(size_t(n) * 3);
+        hgdg::hcurl_tet_basis(p, x, v.data(), c.data());
+        for (int i = 0; i < 3 * n; ++i) {
+            values[i] = double(v[i]);
+            curls[i] = double(c[i]);
+        }
+    });
+}
+
 int hgdg_mesh_read(const char* path, hgdg_mesh_file_info* info, double* coords, int64_t* node_ids, int32_t* zones,
                    int64_t* face_elements, int32_t* face_local, char* zone_names) {
     return run([&] {

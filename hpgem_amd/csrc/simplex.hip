@@ -42,6 +42,7 @@ constexpr int kTixLf = 6;            // + f: [6][kFaceQ][nb]: (M_ref^{-1} phi(po
 constexpr int kTixCols = 10;         // + f
 constexpr int kTixStride = 14;
 constexpr int kMaxCols = 15;         // face columns of an order-4 element
+constexpr int kTixMax = 4 * kTixStride + 4 * 4 * kMaxCols;  // whole index table (LDS copy in the kernel)
 
 // packed per-face info
 __host__ __device__ constexpr int info_lf(int v) { return v & 3; }
@@ -105,6 +106,7 @@ struct SimplexParams {
     const double* tab;
     const float* tab32;    // the same tables in FP32 (mixed path), same offsets
     const int32_t* tix;
+    int32_t n_tix;
     double face_w[kFaceQ]; // face rule weights
 };
 
@@ -139,10 +141,10 @@ constexpr int lds_doubles() {
 // Traces of the four fields of a neighbour of order PN at face point q, from
 // its face columns in LDS and its face table for orientation o.
 template <int PN>
-__device__ __forceinline__ void nbr_traces(const SimplexParams& p, int lfn, int o, int q, const double* cols4,
-                                           double (&t)[4]) {
+__device__ __forceinline__ void nbr_traces(const SimplexParams& p, const int32_t* tix, int lfn, int o, int q,
+                                           const double* cols4, double (&t)[4]) {
     constexpr int NC = simplex_ncol(PN);
-    const double* V = p.tab + p.tix[(PN - 1) * kTixStride + kTixVf + lfn] + (o * NC) * kFaceQ + q;
+    const double* V = p.tab + tix[(PN - 1) * kTixStride + kTixVf + lfn] + (o * NC) * kFaceQ + q;
     t[0] = t[1] = t[2] = t[3] = 0;
 #pragma unroll 3
     for (int c = 0; c < NC; ++c) {
@@ -166,7 +168,8 @@ __device__ __forceinline__ void nbr_traces(const SimplexParams& p, int lfn, int 
 // FP64; T = float: the volume contractions and the lifts in FP32 on FP32
 // tables, compensated per-row sums, traces and fluxes in FP64.
 template <int P, typename T>
-__device__ void simplex_acoustic_block(const SimplexParams& p, int64_t first, int count, double* lds) {
+__device__ void simplex_acoustic_block(const SimplexParams& p, const int32_t* tix, int64_t first, int count,
+                                       double* lds) {
     using S = SimplexShape<P>;
     constexpr int NB = S::NB;
     const int tid = threadIdx.x;
@@ -181,7 +184,7 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int64_t first, in
     double* sg = lds + S::G + le * 3 * NB;
     double* sn = lds + S::N + le * 16 * kMaxCols;
     double* sf = lds + S::F + le * 8 * kFaceQ;
-    const int32_t* tx = p.tix + (P - 1) * kTixStride;
+    const int32_t* tx = tix + (P - 1) * kTixStride;
     const T* tabT = std::is_same<T, float>::value ? reinterpret_cast<const T*>(p.tab32)
                                                   : reinterpret_cast<const T*>(p.tab);
 
@@ -211,7 +214,7 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int64_t first, in
     }
     if (elem_ok) {
         // items (face, field, column): 4 x 4 x kMaxCols
-#pragma unroll 1
+#pragma unroll 4
         for (int it = i; it < 16 * kMaxCols; it += S::TPE) {
             const int f = it / (4 * kMaxCols), fld = (it / kMaxCols) % 4, c = it % kMaxCols;
             const int32_t nb = f == 0 ? nbf[0] : f == 1 ? nbf[1] : f == 2 ? nbf[2] : nbf[3];
@@ -219,7 +222,7 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int64_t first, in
             const int pn = info_pn(fi);
             if (nb < 0 || c >= simplex_ncol(pn)) continue;
             const int nbn = simplex_nb(pn);
-            const int col = p.tix[p.tix[(pn - 1) * kTixStride + kTixCols + info_lf(fi)] + c];
+            const int col = tix[tix[(pn - 1) * kTixStride + kTixCols + info_lf(fi)] + c];
             const int64_t nbase = p.group_dof[pn - 1] + (nb - p.group_start[pn - 1]) * 4 * nbn;
             sn[(f * 4 + fld) * kMaxCols + c] = combine(p.io, nbase + fld * nbn + col);
         }
@@ -258,7 +261,7 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int64_t first, in
             const double n0 = p.normal[(0 * 4 + f) * E + e], n1 = p.normal[(1 * 4 + f) * E + e],
                          n2 = p.normal[(2 * 4 + f) * E + e];
             const double* Vo = p.tab + tx[kTixVf + f] + (info_oown(fi) * S::NCOL) * kFaceQ + q;
-            const int32_t* colso = p.tix + tx[kTixCols + f];
+            const int32_t* colso = tix + tx[kTixCols + f];
             double o4[4] = {0, 0, 0, 0};
 #pragma unroll 3
             for (int c = 0; c < S::NCOL; ++c) {
@@ -273,10 +276,10 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int64_t first, in
                 const double* cols4 = sn + f * 4 * kMaxCols;
                 const int lfn = info_lf(fi), on = info_onbr(fi);
                 switch (info_pn(fi)) {
-                    case 1: nbr_traces<1>(p, lfn, on, q, cols4, t4); break;
-                    case 2: nbr_traces<2>(p, lfn, on, q, cols4, t4); break;
-                    case 3: nbr_traces<3>(p, lfn, on, q, cols4, t4); break;
-                    default: nbr_traces<4>(p, lfn, on, q, cols4, t4); break;
+                    case 1: nbr_traces<1>(p, tix, lfn, on, q, cols4, t4); break;
+                    case 2: nbr_traces<2>(p, tix, lfn, on, q, cols4, t4); break;
+                    case 3: nbr_traces<3>(p, tix, lfn, on, q, cols4, t4); break;
+                    default: nbr_traces<4>(p, tix, lfn, on, q, cols4, t4); break;
                 }
                 Jv = -0.5 * (n0 * (o4[1] - t4[1]) + n1 * (o4[2] - t4[2]) + n2 * (o4[3] - t4[3]));
                 Js = -0.5 * (o4[0] - t4[0]);
@@ -294,7 +297,7 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int64_t first, in
     // ---- phase 3: lift, k += sigma_f sum_q (M_ref^{-1} phi(point o(q)))_i flux(q)
     if (row_ok) {
         const T ci = T(1.0 / cinv);
-#pragma unroll 1
+#pragma unroll 2
         for (int f = 0; f < 4; ++f) {
             const T* Lf = tabT + tx[kTixLf + f] + (info_oown(inf[f]) * kFaceQ) * NB + i;
             T lv = T(0), ls = T(0);
@@ -330,12 +333,15 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, int64_t first, in
 template <typename T>
 __global__ __launch_bounds__(kBlock) void simplex_acoustic_rhs(const SimplexParams p) {
     __shared__ double lds[lds_doubles()];
+    __shared__ int32_t s_tix[kTixMax];  // table offsets and face column lists, read in every phase
+    for (int k = threadIdx.x; k < p.n_tix; k += kBlock) s_tix[k] = p.tix[k];
     const int4 bl = p.blocks[xcd_block(blockIdx.x, gridDim.x)];
+    __syncthreads();
     switch (bl.x) {
-        case 1: simplex_acoustic_block<1, T>(p, bl.y, bl.z, lds); break;
-        case 2: simplex_acoustic_block<2, T>(p, bl.y, bl.z, lds); break;
-        case 3: simplex_acoustic_block<3, T>(p, bl.y, bl.z, lds); break;
-        default: simplex_acoustic_block<4, T>(p, bl.y, bl.z, lds); break;
+        case 1: simplex_acoustic_block<1, T>(p, s_tix, bl.y, bl.z, lds); break;
+        case 2: simplex_acoustic_block<2, T>(p, s_tix, bl.y, bl.z, lds); break;
+        case 3: simplex_acoustic_block<3, T>(p, s_tix, bl.y, bl.z, lds); break;
+        default: simplex_acoustic_block<4, T>(p, s_tix, bl.y, bl.z, lds); break;
     }
 }
 
@@ -626,6 +632,7 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     for (int q = 0; q < kMaxSimplexOrder; ++q)
         for (int f = 0; f < 4; ++f) tix[q * kTixStride + kTixCols + f] += col_base;
     tix.insert(tix.end(), cols.begin(), cols.end());
+    require(tix.size() <= size_t(kTixMax), HGDG_ERR_ARG, "table index larger than its LDS copy");
 
     // block schedule: slabs of consecutive mesh elements, each order's
     // elements of a slab in blocks of that order's size
@@ -703,6 +710,7 @@ void simplex_rhs(hgdg_ctx* c, const StageIO& io) {
     p.tab = sm.d_tables.ptr;
     p.tab32 = sm.d_tables32.ptr;
     p.tix = sm.d_tab_index.ptr;
+    p.n_tix = int32_t(sm.d_tab_index.n);
     p.blocks = reinterpret_cast<const int4*>(sm.d_blocks.ptr);
     for (int q = 0; q < kFaceQ; ++q) p.face_w[q] = kFaceRuleW[q];
     const int grid = int(sm.n_blocks);

@@ -293,6 +293,110 @@ void tet_rule(int k, std::vector<long double>& pts, std::vector<long double>& w)
             }
 }
 
+namespace {
+
+// Barycentric coordinate lambda_v of the reference tetrahedron and its
+// (constant) gradient: lambda_0 = 1 - x - y - z, lambda_k = x_{k-1}.
+long double bary(int v, const long double* x) { return v == 0 ? 1 - x[0] - x[1] - x[2] : x[v - 1]; }
+void bary_grad(int v, long double* g) {
+    for (int d = 0; d < 3; ++d) g[d] = v == 0 ? -1.0L : (d == v - 1 ? 1.0L : 0.0L);
+}
+void cross(const long double* a, const long double* b, long double* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Edge function of degree k on edge (o, i): with phi0 = l_i grad l_o - l_o grad l_i
+// (Whitney) and phi1 = l_i grad l_o + l_o grad l_i = grad(l_o l_i),
+// k = 0: phi0, k = 1: phi1, k >= 2: -(k-1)/k P_{k-2}(s) phi0 + (2k-1)/k P_{k-1}(s) phi1, s = l_i - l_o.
+void ac_edge(int k, int o, int i, const long double* x, long double* val, long double* curl) {
+    long double go[3], gi[3], phi0[3], phi1[3], gio[3];
+    bary_grad(o, go);
+    bary_grad(i, gi);
+    const long double lo = bary(o, x), li = bary(i, x);
+    for (int d = 0; d < 3; ++d) {
+        phi0[d] = li * go[d] - lo * gi[d];
+        phi1[d] = li * go[d] + lo * gi[d];
+    }
+    cross(gi, go, gio);  // curl phi0 = 2 grad l_i x grad l_o, curl phi1 = 0
+    if (k == 0) {
+        for (int d = 0; d < 3; ++d) {
+            val[d] = phi0[d];
+            curl[d] = 2 * gio[d];
+        }
+        return;
+    }
+    if (k == 1) {
+        for (int d = 0; d < 3; ++d) {
+            val[d] = phi1[d];
+            curl[d] = 0;
+        }
+        return;
+    }
+    long double pa, dpa, dda, pb, dpb, ddb;
+    legendre3(k - 2, li - lo, pa, dpa, dda);
+    legendre3(k - 1, li - lo, pb, dpb, ddb);
+    const long double ca = -(k - 1.0L) / k, cb = (2.0L * k - 1) / k;
+    long double gs[3], x0[3], x1[3];
+    for (int d = 0; d < 3; ++d) gs[d] = gi[d] - go[d];
+    cross(gs, phi0, x0);
+    cross(gs, phi1, x1);
+    // curl(f phi) = grad f x phi + f curl phi
+    for (int d = 0; d < 3; ++d) {
+        val[d] = ca * pa * phi0[d] + cb * pb * phi1[d];
+        curl[d] = ca * (dpa * x0[d] + pa * 2 * gio[d]) + cb * dpb * x1[d];
+    }
+}
+
+// Edge-based face function of degree k >= 2 on the face opposite vertex v,
+// attached to vertex c: with (a, b) the face's other two vertices (a < b),
+// grad l_c l_a l_b P_{k-2}(l_b - l_a).
+void ac_edge_face(int k, int v, int c, const long double* x, long double* val, long double* curl) {
+    int a = 0;
+    while (a == c || a == v) ++a;
+    int b = a + 1;
+    while (b == c || b == v) ++b;
+    long double ga[3], gb[3], gc[3];
+    bary_grad(a, ga);
+    bary_grad(b, gb);
+    bary_grad(c, gc);
+    const long double la = bary(a, x), lb = bary(b, x);
+    long double P, dP, ddP;
+    legendre3(k - 2, lb - la, P, dP, ddP);
+    long double ac_[3], bc[3];
+    cross(ga, gc, ac_);
+    cross(gb, gc, bc);
+    // f = l_a l_b P(l_b - l_a): grad f = (l_b P - l_a l_b P') grad l_a + (l_a P + l_a l_b P') grad l_b
+    const long double fa = lb * P - la * lb * dP, fb = la * P + la * lb * dP;
+    for (int d = 0; d < 3; ++d) {
+        val[d] = gc[d] * la * lb * P;
+        curl[d] = fa * ac_[d] + fb * bc[d];
+    }
+}
+
+}  // namespace
+
+int hcurl_tet_size(int p) { return p < 1 ? 0 : 6 * (p + 1) + (p >= 2 ? 12 * (p - 1) : 0); }
+
+void hcurl_tet_basis(int p, const long double* x, long double* val, long double* curl) {
+    if (p < 1 || p > 2) throw std::invalid_argument("the H(curl) tetrahedral set is built for orders 1 and 2");
+    static const int edges[6][2] = {{0, 1}, {0, 2}, {0, 3}, {2, 3}, {1, 3}, {1, 2}};
+    static const int faces[12][2] = {{3, 0}, {3, 1}, {3, 2}, {2, 0}, {2, 1}, {2, 3},
+                                     {1, 0}, {1, 2}, {1, 3}, {0, 1}, {0, 2}, {0, 3}};
+    int n = 0;
+    for (int k = 0; k <= p; ++k)
+        for (const auto& ed : edges) {
+            ac_edge(k, ed[0], ed[1], x, val + 3 * n, curl + 3 * n);
+            ++n;
+        }
+    for (int k = 2; k <= p; ++k)
+        for (const auto& fc : faces) {
+            ac_edge_face(k, fc[0], fc[1], x, val + 3 * n, curl + 3 * n);
+            ++n;
+        }
+}
+
 SimplexOps simplex_ops(int p) {
     if (p < 1 || p > kMaxSimplexOrder) throw std::invalid_argument("tetrahedral order must be in 1..4");
     SimplexOps ops;

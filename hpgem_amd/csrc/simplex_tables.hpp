@@ -50,6 +50,14 @@ struct SimplexOps {
 // Built in long double with exact collapsed Gauss rules.
 SimplexOps simplex_ops(int p);
 
+// hpGEM's Ainsworth-Coyle H(curl) DG set on the reference tetrahedron
+// (createDGBasisFunctionSet3DAinsworthCoyle, BasisFunctions3DAinsworthCoyle
+// .cpp:417-505) for orders 1 and 2: per degree k = 0..p the six edge
+// functions on edges (0,1), (0,2), (0,3), (2,3), (1,3), (1,2), then for
+// k >= 2 the twelve edge-based face functions; values and curls, [n][3] each.
+int hcurl_tet_size(int p);
+void hcurl_tet_basis(int p, const long double* x, long double* val, long double* curl);
+
 // Collapsed Gauss rule on the reference tetrahedron exact for total degree k.
 void tet_rule(int k, std::vector<long double>& pts /*[n][3]*/, std::vector<long double>& w);
 

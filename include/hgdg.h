@@ -273,6 +273,13 @@ int hgdg_structured_mesh(int dim, const int32_t* n, const double* lo, const doub
                          const int32_t* periodic, int simplices, int64_t* n_elements,
                          int32_t* n_vertices, int32_t* shape, double* coords, int64_t* node_ids);
 
+/* hpGEM's Ainsworth-Coyle H(curl) DG set on the reference tetrahedron
+ * (createDGBasisFunctionSet3DAinsworthCoyle, kernel/FE/BasisFunctions3D
+ * AinsworthCoyle.cpp:417-505), orders 1 and 2: size, and values and curls
+ * ([n][3] each) at one reference point. */
+int hgdg_hcurl_basis_size(int p);
+int hgdg_hcurl_basis_eval(int p, const double* point, double* values, double* curls);
+
 /* Sizes of a .hpgem mesh file (format version 1 or 2, one partition). */
 typedef struct hgdg_mesh_file_info {
     int32_t version, dim, shape, n_vertices; /* shape: HGDG_SHAPE_*, one type per file */

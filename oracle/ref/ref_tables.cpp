@@ -17,6 +17,7 @@
 #include "FE/BasisFunctions2DH1ConformingTriangle.h"
 #include "FE/BasisFunctions3DH1ConformingCube.h"
 #include "FE/BasisFunctions3DH1ConformingTetrahedron.h"
+#include "FE/BasisFunctions3DAinsworthCoyle.h"
 #include "Geometry/PointReference.h"
 #include "Geometry/ReferenceCube.h"
 #include "Geometry/ReferenceLine.h"
@@ -136,6 +137,47 @@ void dump_basis(const char* shape, FE::BasisFunctionSet* (*make)(std::size_t),
                 g.push_back(set->evalDeriv(i, d, pt));
         if (k) std::printf(",");
         print_vec(g);
+    }
+    std::printf("]}");
+    delete set;
+}
+
+// Vector-valued H(curl) set on the reference tetrahedron: values and curls
+// at the sample points.
+void dump_hcurl_basis(const char* name, FE::BasisFunctionSet* (*make)(std::size_t), std::size_t p) {
+    FE::BasisFunctionSet* set = make(p);
+    auto pts = sample_points(3, true);
+    sep();
+    std::printf("{\"kind\":\"hcurl_basis\",\"set\":\"%s\",\"p\":%zu,\"n\":%zu,\"points\":[", name, p,
+                set->size());
+    for (std::size_t k = 0; k < pts.size(); ++k) {
+        if (k) std::printf(",");
+        print_vec(pts[k]);
+    }
+    std::printf("],\"values\":[");
+    for (std::size_t k = 0; k < pts.size(); ++k) {
+        Geometry::PointReference<3> pt;
+        for (std::size_t d = 0; d < 3; ++d) pt[d] = pts[k][d];
+        std::vector<double> v;
+        for (std::size_t i = 0; i < set->size(); ++i) {
+            LinearAlgebra::SmallVector<3> r;
+            set->eval(i, pt, r);
+            for (std::size_t d = 0; d < 3; ++d) v.push_back(r[d]);
+        }
+        if (k) std::printf(",");
+        print_vec(v);
+    }
+    std::printf("],\"curls\":[");
+    for (std::size_t k = 0; k < pts.size(); ++k) {
+        Geometry::PointReference<3> pt;
+        for (std::size_t d = 0; d < 3; ++d) pt[d] = pts[k][d];
+        std::vector<double> v;
+        for (std::size_t i = 0; i < set->size(); ++i) {
+            LinearAlgebra::SmallVector<3> r = set->evalCurl(i, pt);
+            for (std::size_t d = 0; d < 3; ++d) v.push_back(r[d]);
+        }
+        if (k) std::printf(",");
+        print_vec(v);
     }
     std::printf("]}");
     delete set;
@@ -269,6 +311,8 @@ int main() {
         dump_basis<3>("tetrahedron",
                       &FE::createDGBasisFunctionSet3DH1Tetrahedron, p, true);
     }
+    for (std::size_t p = 1; p <= 3; ++p)
+        dump_hcurl_basis("ainsworth_coyle", &FE::createDGBasisFunctionSet3DAinsworthCoyle, p);
     dump_geometry<2>("square", Geometry::ReferenceSquare::Instance(), false);
     dump_geometry<3>("cube", Geometry::ReferenceCube::Instance(), false);
     dump_geometry<2>("triangle", Geometry::ReferenceTriangle::Instance(), true);

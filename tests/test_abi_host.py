@@ -70,3 +70,17 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "libhgdg.so"))
     with pytest.raises(ImportError):
         _lib.load()
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_hcurl_basis_matches_reference_fixture(p):
+    """The library's Ainsworth-Coyle set against the values and curls the
+    reference's own BasisFunctions3DAinsworthCoyle.cpp printed."""
+    from hpgem_amd.api import hcurl_basis_eval
+    t = [x for x in orc.fixture_tables() if x["kind"] == "hcurl_basis" and x["p"] == p][0]
+    n = t["n"]
+    for pt, vals, curls in zip(t["points"], t["values"], t["curls"]):
+        v, c = hcurl_basis_eval(p, pt)
+        assert v.shape == (n, 3)
+        np.testing.assert_allclose(v.ravel(), vals, rtol=0, atol=1e-14)
+        np.testing.assert_allclose(c.ravel(), curls, rtol=0, atol=1e-13)
