@@ -1313,6 +1313,238 @@ using namespace orc;
         return 1;                          \
     }
 
+// ------------------------------------------------------------ Maxwell IP (DG-Max)
+// Interior-penalty DG curl-curl operator of DG-Max
+// (applications/DG-Max/Algorithms/DGMaxDiscretization.cpp:193-260 element
+// matrices, :344-410 face matrices) with hpGEM's Ainsworth-Coyle H(curl) set
+// (kernel/FE/BasisFunctions3DAinsworthCoyle.cpp) and the covariant transform
+// of HCurlConformingTransformation (kernel/Base/HCurlConformingTransformation.h):
+// phi = J^{-T} phi_ref, curl phi = J curl_ref / det J.
+namespace mx {
+
+Real bary(int node, const Real* p) { return node == 0 ? 1 - p[0] - p[1] - p[2] : p[node - 1]; }
+void bary_grad(int node, Real* g) {
+    for (int d = 0; d < 3; ++d) g[d] = node == 0 ? -1 : (d == node - 1 ? 1 : 0);
+}
+void outer(const Real* a, const Real* b, Real* r) {  // OuterProduct: a x b
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+}
+// LegendrePolynomial / LegendrePolynomialDerivative (FE/helperFunctions.cpp:110-137)
+Real legendre(int n, Real x) {
+    if (n == 0) return 1;
+    if (n == 1) return x;
+    return (2.0 * n - 1) / n * x * legendre(n - 1, x) - (n - 1.0) / n * legendre(n - 2, x);
+}
+Real legendre_d(int n, Real x) {
+    if (n == 0) return 0;
+    if (n == 1) return 1;
+    return (2.0 * n - 1) / n * (x * legendre_d(n - 1, x) + legendre(n - 1, x)) - (n - 1.0) / n * legendre_d(n - 2, x);
+}
+
+// BasisCurlEdgeAinsworthCoyle(deg, o, i)::eval / evalCurl (:80-165)
+void edge(int deg, int o, int i, const Real* p, Real* val, Real* curl) {
+    Real go[3], gi[3];
+    bary_grad(o, go);
+    bary_grad(i, gi);
+    const Real vi = bary(i, p), vo = bary(o, p);
+    Real ret[3], dum[3];
+    for (int d = 0; d < 3; ++d) {
+        ret[d] = go[d] * vi;
+        dum[d] = gi[d] * vo;
+    }
+    if (deg == 0) {
+        for (int d = 0; d < 3; ++d) val[d] = ret[d] - dum[d];
+        outer(gi, go, curl);
+        for (int d = 0; d < 3; ++d) curl[d] *= 2;
+        return;
+    }
+    if (deg == 1) {
+        for (int d = 0; d < 3; ++d) {
+            val[d] = ret[d] + dum[d];
+            curl[d] = 0;
+        }
+        return;
+    }
+    const Real s = vi - vo;
+    Real phi0[3], phi1[3];
+    for (int d = 0; d < 3; ++d) {
+        phi0[d] = ret[d] - dum[d];
+        phi1[d] = phi0[d] + 2 * dum[d];
+    }
+    const Real ca = -Real(deg - 1) / deg * legendre(deg - 2, s);
+    const Real cb = Real(2 * deg - 1) / deg * legendre(deg - 1, s);
+    for (int d = 0; d < 3; ++d) val[d] = ca * phi0[d] + cb * phi1[d];
+    Real gs[3], c0[3], x1[3], x0[3];
+    for (int d = 0; d < 3; ++d) gs[d] = gi[d] - go[d];
+    outer(gi, go, c0);
+    outer(gs, phi1, x1);
+    outer(gs, phi0, x0);
+    for (int d = 0; d < 3; ++d)
+        curl[d] = ca * 2 * c0[d] + Real(2 * deg - 1) / deg * legendre_d(deg - 1, s) * x1[d] -
+                  Real(deg - 1) / deg * legendre_d(deg - 2, s) * x0[d];
+}
+
+// BasisCurlEdgeFaceAinsworthCoyle(deg, opposing, special)::eval / evalCurl (:167-215)
+void edge_face(int deg, int opp, int c, const Real* p, Real* val, Real* curl) {
+    int a = 0;
+    while (c == a || opp == a) ++a;
+    int b = a + 1;
+    while (c == b || opp == b) ++b;
+    Real ga[3], gb[3], gc[3];
+    bary_grad(a, ga);
+    bary_grad(b, gb);
+    bary_grad(c, gc);
+    const Real va = bary(a, p), vb = bary(b, p);
+    const Real P = legendre(deg - 2, vb - va), dP = legendre_d(deg - 2, vb - va);
+    for (int d = 0; d < 3; ++d) val[d] = gc[d] * va * vb * P;
+    Real r1[3], r2[3];
+    outer(ga, gc, r1);
+    outer(gb, gc, r2);
+    for (int d = 0; d < 3; ++d) curl[d] = r1[d] * (vb * P - va * vb * dP) + r2[d] * (va * P + va * vb * dP);
+}
+
+int size(int p) { return 6 * (p + 1) + (p >= 2 ? 12 * (p - 1) : 0); }
+
+// createDGBasisFunctionSet3DAinsworthCoyle (:417-450), orders 1 and 2
+void eval(int p, const Real* x, Real* val, Real* curl) {
+    if (p < 1 || p > 2) throw std::runtime_error("Ainsworth-Coyle set restated for orders 1 and 2");
+    static const int E[6][2] = {{0, 1}, {0, 2}, {0, 3}, {2, 3}, {1, 3}, {1, 2}};
+    static const int F[12][2] = {{3, 0}, {3, 1}, {3, 2}, {2, 0}, {2, 1}, {2, 3},
+                                 {1, 0}, {1, 2}, {1, 3}, {0, 1}, {0, 2}, {0, 3}};
+    int n = 0;
+    for (int k = 0; k <= p; ++k)
+        for (auto& e : E) {
+            edge(k, e[0], e[1], x, val + 3 * n, curl + 3 * n);
+            ++n;
+        }
+    for (int k = 2; k <= p; ++k)
+        for (auto& f : F) {
+            edge_face(k, f[0], f[1], x, val + 3 * n, curl + 3 * n);
+            ++n;
+        }
+}
+
+// physical values / curls at a reference point of an affine tetrahedron
+void physical(int p, const Real* J, const Real* xi, std::vector<Real>& val, std::vector<Real>& curl) {
+    const int n = size(p);
+    std::vector<Real> vr(size_t(3 * n)), cr(size_t(3 * n));
+    eval(p, xi, vr.data(), cr.data());
+    const Real dJ = det(3, J);
+    val.assign(size_t(3 * n), 0);
+    curl.assign(size_t(3 * n), 0);
+    for (int i = 0; i < n; ++i) {
+        Real A[9], rhs[3];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) A[r * 3 + c] = J[c * 3 + r];  // J^T
+        for (int d = 0; d < 3; ++d) rhs[d] = vr[i * 3 + d];
+        solve_small(3, A, rhs);
+        for (int d = 0; d < 3; ++d) {
+            val[i * 3 + d] = rhs[d];
+            Real s = 0;
+            for (int k = 0; k < 3; ++k) s += J[d * 3 + k] * cr[i * 3 + k];
+            curl[i * 3 + d] = s / dJ;
+        }
+    }
+}
+
+struct Out {
+    int n = 0;
+    std::vector<std::vector<Real>> S, M;                 // per element, column-major n x n
+    std::vector<std::array<std::vector<Real>, 4>> F;     // per face: (test side, trial side) blocks
+};
+
+void assemble(Problem& P, int p, const Real* eps, Real stab, Out& out) {
+    if (P.shape != TETRAHEDRON) throw std::runtime_error("Maxwell assembly on tetrahedra");
+    const int n = size(p);
+    out.n = n;
+    out.S.assign(P.nE, std::vector<Real>(size_t(n) * n, 0));
+    out.M.assign(P.nE, std::vector<Real>(size_t(n) * n, 0));
+    std::vector<Real> v, c;
+    for (int64_t e = 0; e < P.nE; ++e) {
+        Rule r = get_rule(TETRAHEDRON, 2 * p + 1);
+        for (int q = 0; q < r.n(); ++q) {
+            Real x[3], J[9];
+            elem_map(TETRAHEDRON, P.Xe(e), &r.pts[q * 3], x, J);
+            const Real w = r.w[q] * std::fabs(det(3, J));
+            physical(p, J, &r.pts[q * 3], v, c);
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) {
+                    Real cc = 0, vv = 0;
+                    for (int d = 0; d < 3; ++d) {
+                        cc += c[i * 3 + d] * c[j * 3 + d];
+                        vv += v[i * 3 + d] * v[j * 3 + d];
+                    }
+                    out.S[e][j + size_t(i) * n] += w * cc;
+                    out.M[e][j + size_t(i) * n] += w * eps[e] * vv;
+                }
+        }
+    }
+    out.F.assign(P.faces.size(), {});
+    for (size_t fi = 0; fi < P.faces.size(); ++fi) {
+        const FaceRec& f = P.faces[fi];
+        const int ns = f.internal ? 2 : 1;
+        for (int b = 0; b < 4; ++b) out.F[fi][b].assign((b == 0 || f.internal) ? size_t(n) * n : 0, 0);
+        // face diameter: largest distance between the face's vertices (FaceGeometry.cpp:176-194)
+        const int* fn = face_nodes(TETRAHEDRON, f.fL);
+        Real diam2 = 0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = a + 1; b < 3; ++b) {
+                Real d2 = 0;
+                for (int d = 0; d < 3; ++d) {
+                    const Real dd = P.Xe(f.L)[fn[a] * 3 + d] - P.Xe(f.L)[fn[b] * 3 + d];
+                    d2 += dd * dd;
+                }
+                diam2 = std::max(diam2, d2);
+            }
+        const Real local_stab = stab / std::sqrt(diam2);
+        const Real factor = f.internal ? -0.5 : -1.0;
+        Rule r = get_rule(TRIANGLE, 2 * p + 1);
+        for (int q = 0; q < r.n(); ++q) {
+            FacePoint fp;
+            face_point(P, f, &r.pts[q * 2], r.w[q], fp);  // weight * |n| and the unit normal
+            std::vector<Real> val[2], curl[2];
+            Real peL[3], xL[3], JL[9];
+            face_map(TETRAHEDRON, f.fL, &r.pts[q * 2], peL, nullptr, nullptr);
+            elem_map(TETRAHEDRON, P.Xe(f.L), peL, xL, JL);
+            physical(p, JL, peL, val[0], curl[0]);
+            if (f.internal) {
+                Real pf2[2], peR[3], xR[3], JR[9];
+                orientation_map(TRIANGLE, f.orient, &r.pts[q * 2], pf2);
+                face_map(TETRAHEDRON, f.fR, pf2, peR, nullptr, nullptr);
+                elem_map(TETRAHEDRON, P.Xe(f.R), peR, xR, JR);
+                physical(p, JR, peR, val[1], curl[1]);
+            }
+            // unit normal x phi, negated for the right element's functions
+            // (PhysicalFace_Impl.h:454-489)
+            std::vector<Real> T[2];
+            for (int sd = 0; sd < ns; ++sd) {
+                T[sd].assign(size_t(3 * n), 0);
+                for (int i = 0; i < n; ++i) {
+                    outer(fp.nunit, &val[sd][i * 3], &T[sd][i * 3]);
+                    if (sd == 1)
+                        for (int d = 0; d < 3; ++d) T[sd][i * 3 + d] *= -1;
+                }
+            }
+            for (int su = 0; su < ns; ++su)      // side of trial function i
+                for (int st = 0; st < ns; ++st)  // side of test function j
+                    for (int i = 0; i < n; ++i)
+                        for (int j = 0; j < n; ++j) {
+                            Real ct = 0, tc = 0, tt = 0;
+                            for (int d = 0; d < 3; ++d) {
+                                ct += curl[su][i * 3 + d] * T[st][j * 3 + d];
+                                tc += T[su][i * 3 + d] * curl[st][j * 3 + d];
+                                tt += T[su][i * 3 + d] * T[st][j * 3 + d];
+                            }
+                            out.F[fi][st * 2 + su][j + size_t(i) * n] += fp.w * (factor * (ct + tc) + local_stab * tt);
+                        }
+        }
+    }
+}
+
+}  // namespace mx
+
 extern "C" {
 
 const char* orc_last_error() { return g_err.c_str(); }
@@ -1541,6 +1773,40 @@ int orc_l2_error_modes(void* h, const double* u, int n_modes, const double* amp,
         m.amp = ar.data();
         m.k = kr.data();
         *err = double(l2_error(*P, ur.data(), m, Real(t)));
+    })
+}
+
+int orc_hcurl_eval(int p, const double* pt, double* val, double* curl) {
+    ORC_TRY({
+        const int n = mx::size(p);
+        Real x[3] = {pt[0], pt[1], pt[2]};
+        std::vector<Real> v(size_t(3 * n)), c(size_t(3 * n));
+        mx::eval(p, x, v.data(), c.data());
+        for (int i = 0; i < 3 * n; ++i) {
+            val[i] = double(v[i]);
+            curl[i] = double(c[i]);
+        }
+    })
+}
+
+// DG-Max element (S: curl-curl, M: permittivity mass) and face matrices,
+// column-major n x n each; F per face: blocks (test side, trial side) LL, LR,
+// RL, RR (boundary faces: LL only, the rest left zero).
+int orc_maxwell_assemble(void* h, int p, const double* eps, double stab, double* S, double* M, double* F) {
+    ORC_TRY({
+        Problem* P = static_cast<Problem*>(h);
+        std::vector<Real> er(eps, eps + P->nE);
+        mx::Out o;
+        mx::assemble(*P, p, er.data(), Real(stab), o);
+        const size_t nn = size_t(o.n) * o.n;
+        for (int64_t e = 0; e < P->nE; ++e)
+            for (size_t k = 0; k < nn; ++k) {
+                S[e * nn + k] = double(o.S[e][k]);
+                M[e * nn + k] = double(o.M[e][k]);
+            }
+        for (size_t fi = 0; fi < o.F.size(); ++fi)
+            for (int b = 0; b < 4; ++b)
+                for (size_t k = 0; k < nn; ++k) F[(fi * 4 + b) * nn + k] = k < o.F[fi][b].size() ? double(o.F[fi][b][k]) : 0.0;
     })
 }
 

@@ -58,6 +58,10 @@ def load(extended=False):
     lib.orc_project_modes.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D]
     lib.orc_l2_error_modes.argtypes = [ctypes.c_void_p, _D, ctypes.c_int, _D, _D, ctypes.c_double, _D]
     lib.orc_mass_matrix.argtypes = [ctypes.c_void_p, ctypes.c_int64, _D]
+    lib.orc_hcurl_eval.restype = ctypes.c_int
+    lib.orc_hcurl_eval.argtypes = [ctypes.c_int, _D, _D, _D]
+    lib.orc_maxwell_assemble.restype = ctypes.c_int
+    lib.orc_maxwell_assemble.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_double, _D, _D, _D]
     _libs[extended] = lib
     _load_simplex_rules(lib)
     return lib
@@ -138,6 +142,15 @@ def orientation_index(fshape, l1, l2):
     if r < 0:
         check(1)
     return r
+
+
+def hcurl_eval(p, pt):
+    """Values and curls ([n][3]) of the oracle's Ainsworth-Coyle set."""
+    lib = load()
+    n = 6 * (p + 1) + (12 * (p - 1) if p >= 2 else 0)
+    v, c = np.zeros(3 * n), np.zeros(3 * n)
+    check(lib.orc_hcurl_eval(p, _dp(np.ascontiguousarray(pt, dtype=np.float64)), _dp(v), _dp(c)))
+    return v.reshape(n, 3), c.reshape(n, 3)
 
 
 def structured_mesh(n, lo=None, hi=None, periodic=None, simplices=False):
@@ -242,6 +255,23 @@ class Problem:
         err = ctypes.c_double()
         check(self.lib.orc_l2_error_modes(self.h, _dp(u), len(amp), _dp(amp), _dp(kk), t, ctypes.byref(err)))
         return err.value
+
+    def maxwell(self, p, eps, stab):
+        """DG-Max element matrices S (curl-curl), M (permittivity mass) and face
+        blocks F[face][(test side, trial side)], each n x n with [row, col] =
+        [test, trial]."""
+        n = 6 * (p + 1) + (12 * (p - 1) if p >= 2 else 0)
+        nf = self.n_faces
+        S = np.zeros(self.n_elem * n * n)
+        M = np.zeros(self.n_elem * n * n)
+        F = np.zeros(nf * 4 * n * n)
+        e = np.ascontiguousarray(eps, dtype=np.float64)
+        check(self.lib.orc_maxwell_assemble(self.h, p, _dp(e), stab, _dp(S), _dp(M), _dp(F)))
+        # column-major blocks -> [row][col]
+        S = S.reshape(self.n_elem, n, n).transpose(0, 2, 1)
+        M = M.reshape(self.n_elem, n, n).transpose(0, 2, 1)
+        F = F.reshape(nf, 4, n, n).transpose(0, 1, 3, 2)
+        return S, M, F
 
     def mass_matrix(self, e):
         nb = self.n_dof // self.n_elem if np.all(self.orders == self.orders[0]) else None
