@@ -142,3 +142,13 @@ SIGNATURES += [
     ("hgdg_mesh_read", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(MeshFileInfo), _D, _I64, _I32, _I64, _I32,
                                       ctypes.c_char_p]),
 ]
+
+
+class MaxwellDesc(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_int32), ("epsilon", _D), ("stab", ctypes.c_double)]
+
+
+SIGNATURES += [
+    ("hgdg_maxwell_assemble", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MaxwellDesc), _I64, _I32, _I64, _I64, _D,
+                                             _D, _D]),
+]

@@ -97,6 +97,35 @@ def hcurl_basis_eval(p, point):
     return v, c
 
 
+def maxwell_assemble(mesh, p, epsilon, stab, device=0):
+    """Interior-penalty Maxwell stiffness (block-CSR) and permittivity mass of
+    a tetrahedral mesh on the GPU (DG-Max, DGMaxDiscretization.cpp:193-410)."""
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    check(lib.hgdg_ctx_create(ctypes.byref(h), device))
+    try:
+        order = np.ones(mesh.n_elements, dtype=np.int32)
+        desc = _lib.MeshDesc(mesh.shape, mesh.dim, mesh.n_elements, dptr(mesh.coords),
+                             mesh.node_ids.ctypes.data_as(_lib._I64), order.ctypes.data_as(_lib._I32), 1)
+        check(lib.hgdg_mesh_upload(h, ctypes.byref(desc)))
+        eps = np.ascontiguousarray(epsilon, dtype=np.float64)
+        md = _lib.MaxwellDesc(int(p), dptr(eps), float(stab))
+        nblk, bs, ms = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_double()
+        check(lib.hgdg_maxwell_assemble(h, ctypes.byref(md), ctypes.byref(nblk), ctypes.byref(bs), None, None, None,
+                                        None, None))
+        n = bs.value
+        row_ptr = np.zeros(mesh.n_elements + 1, dtype=np.int64)
+        col_idx = np.zeros(nblk.value, dtype=np.int64)
+        K = np.zeros((nblk.value, n, n))
+        M = np.zeros((mesh.n_elements, n, n))
+        check(lib.hgdg_maxwell_assemble(h, ctypes.byref(md), ctypes.byref(nblk), ctypes.byref(bs),
+                                        row_ptr.ctypes.data_as(_lib._I64), col_idx.ctypes.data_as(_lib._I64),
+                                        dptr(K), dptr(M), ctypes.byref(ms)))
+        return {"row_ptr": row_ptr, "col_idx": col_idx, "K": K, "M": M, "block_size": n, "kernel_ms": ms.value}
+    finally:
+        check(lib.hgdg_ctx_destroy(h))
+
+
 class HpgemAPIDevice:
     """Explicit DG solver whose RHS, mass solve and RK loop run on one GPU.
 

@@ -76,6 +76,7 @@ struct hgdg_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     double dt_estimate = 0;  // adaptive stepping: next step size (0 = not started)
+    double last_kernel_ms = 0;  // duration of the last assembly kernel (HIP events)
 
     bool simplex() const { return shape == HGDG_SHAPE_TETRAHEDRON; }
     int64_t n_dof() const { return dof; }
@@ -131,6 +132,10 @@ struct AxpyItem {
     int32_t n_x;
 };
 void axpy_batch_launch(hipStream_t stream, const AxpyItem* items_dev, const int64_t* off, int n, int64_t blocks);
+
+// maxwell.hip
+void maxwell_assemble(hgdg_ctx* c, int order, const double* eps, double stab, int64_t* n_blocks, int32_t* block_size,
+                      int64_t* row_ptr, int64_t* col_idx, double* K, double* M);
 
 // simplex.hip
 void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m);

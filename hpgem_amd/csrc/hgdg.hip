@@ -745,6 +745,16 @@ int hgdg_structured_mesh(int dim, const int32_t* n, const double* lo, const doub
     });
 }
 
+int hgdg_maxwell_assemble(hgdg_ctx* ctx, const hgdg_maxwell_desc* d, int64_t* n_blocks, int32_t* block_size,
+                          int64_t* row_ptr, int64_t* col_idx, double* K, double* M, double* kernel_ms) {
+    return run([&] {
+        require(ctx && d && n_blocks && block_size, HGDG_ERR_ARG, "NULL argument");
+        HGDG_HIP(hipSetDevice(ctx->device));
+        hgdg::maxwell_assemble(ctx, d->order, d->epsilon, d->stab, n_blocks, block_size, row_ptr, col_idx, K, M);
+        if (kernel_ms) *kernel_ms = row_ptr ? ctx->last_kernel_ms : 0.0;
+    });
+}
+
 int hgdg_hcurl_basis_size(int p) { return hgdg::hcurl_tet_size(p); }
 
 int hgdg_hcurl_basis_eval(int p, const double* point, double* values, double* curls) {

@@ -40,9 +40,11 @@ constexpr int kTixM = 1;             // [nb][nb]: M_ref
 constexpr int kTixVf = 2;            // + f: [6][ncol][kFaceQ]: trace of face column c at point o(q)
 constexpr int kTixLf = 6;            // + f: [6][kFaceQ][nb]: (M_ref^{-1} phi(point o(q)))_i
 constexpr int kTixCols = 10;         // + f
-constexpr int kTixStride = 14;
+constexpr int kTixLf0 = 14;          // [4][kFaceQ][nb]: the lift rows at the unpermuted face points
+constexpr int kTixStride = 15;
+constexpr int kTixPerm = 4 * kTixStride;  // [6][16 face points]: face point index after orientation o
 constexpr int kMaxCols = 15;         // face columns of an order-4 element
-constexpr int kTixMax = 4 * kTixStride + 4 * 4 * kMaxCols;  // whole index table (LDS copy in the kernel)
+constexpr int kTixMax = kTixPerm + 6 * 16 + 4 * 4 * kMaxCols;  // whole index table (LDS copy in the kernel)
 
 // packed per-face info
 __host__ __device__ constexpr int info_lf(int v) { return v & 3; }
@@ -57,6 +59,7 @@ struct RefTetRule {
     const double* w;
 };
 #include "tet_rules.inc"
+static_assert(kFaceQ == 16, "the table index reserves 16 face points");
 
 // Inverse of a small SPD matrix (Gauss-Jordan with pivoting, long double).
 std::vector<long double> inverse_ld(const std::vector<double>& A, int n) {
@@ -121,12 +124,17 @@ struct SimplexShape {
     static constexpr int NCOL = simplex_ncol(P);
     static constexpr int TPE = simplex_tpe(P);  // threads per element
     static constexpr int EPB = kBlock / TPE;
-    // LDS carve-up (doubles)
+    // LDS carve-up (doubles). The work area W after the stage input holds, in
+    // turn, the derivative tables (volume), then the neighbours' face columns
+    // N and the point fluxes F (faces); the lift tables reuse U and N.
     static constexpr int U = 0;                             // [EPB][4][NB] stage input
-    static constexpr int G = U + EPB * 4 * NB;              // [EPB][3][NB] J^{-T}-rotated s
-    static constexpr int N = G + EPB * 3 * NB;              // [EPB][4 faces][4 fields][kMaxCols] neighbours' face columns
+    static constexpr int W = U + EPB * 4 * NB;
+    static constexpr int N = W;                             // [EPB][4 faces][4 fields][kMaxCols] neighbours' face columns
     static constexpr int F = N + EPB * 16 * kMaxCols;       // [EPB][4 faces][2][kFaceQ] weighted fluxes at face points
-    static constexpr int TOTAL = F + EPB * 8 * kFaceQ;
+    static constexpr int FACE_END = F + EPB * 8 * kFaceQ;
+    static constexpr int DR_END = W + 3 * NB * NB;          // DrT [3][NB][NB]
+    static constexpr int TOTAL = FACE_END > DR_END ? FACE_END : DR_END;
+    static_assert(4 * kFaceQ * NB <= F, "lift tables must fit in U and N");
 };
 
 constexpr int lds_doubles() {
@@ -181,14 +189,14 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, const int32_t* ti
     const int64_t E = p.n_elem;
     const int64_t base = p.group_dof[P - 1] + (e - p.group_start[P - 1]) * 4 * NB;
     double* su = lds + S::U + le * 4 * NB;
-    double* sg = lds + S::G + le * 3 * NB;
     double* sn = lds + S::N + le * 16 * kMaxCols;
     double* sf = lds + S::F + le * 8 * kFaceQ;
+    T* sDr = reinterpret_cast<T*>(lds + S::W);
     const int32_t* tx = tix + (P - 1) * kTixStride;
     const T* tabT = std::is_same<T, float>::value ? reinterpret_cast<const T*>(p.tab32)
                                                   : reinterpret_cast<const T*>(p.tab);
 
-    // ---- phase 1
+    // ---- phase 1: geometry, stage input, the order's derivative tables
     double Ji[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     double cinv = 1.0;
     int32_t nbf[4] = {-1, -1, -1, -1}, inf[4] = {0, 0, 0, 0};
@@ -203,15 +211,42 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, const int32_t* ti
         cinv = p.cinv[e];
     }
     if (row_ok) {
-        double u[4];
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            u[f] = combine(p.io, base + f * NB + i);
-            su[f * NB + i] = u[f];
-        }
-#pragma unroll
-        for (int r = 0; r < 3; ++r) sg[r * NB + i] = Ji[r * 3] * u[1] + Ji[r * 3 + 1] * u[2] + Ji[r * 3 + 2] * u[3];
+        for (int f = 0; f < 4; ++f) su[f * NB + i] = combine(p.io, base + f * NB + i);
     }
+    {
+        const T* src = tabT + tx[kTixDrT];
+#pragma unroll 4
+        for (int k = tid; k < 3 * NB * NB; k += kBlock) sDr[k] = src[k];
+    }
+    __syncthreads();
+
+    // ---- phase 2a: volume, k_v = sum_r Dr_r g_r ; k_s_d = (1/c_inv) sum_r (J^{-1})_{r,d} Dr_r v,
+    // g_r = sum_d (J^{-1})_{r,d} s_d formed on the fly
+    Acc<T> kv, ks[3];
+    if (row_ok) {
+        T a[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
+#pragma unroll 2
+        for (int j = 0; j < NB; ++j) {
+            const double s0 = su[NB + j], s1 = su[2 * NB + j], s2 = su[3 * NB + j];
+            const T v = T(su[j]);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const T g = T(Ji[r * 3] * s0 + Ji[r * 3 + 1] * s1 + Ji[r * 3 + 2] * s2);
+                const T d = sDr[(r * NB + j) * NB + i];
+                a[r] += d * g;
+                b[r] += d * v;
+            }
+        }
+        kv = Acc<T>(a[0] + a[1] + a[2]);
+        const T ci = T(1.0 / cinv);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            ks[d] = Acc<T>((T(Ji[d]) * b[0] + T(Ji[3 + d]) * b[1] + T(Ji[6 + d]) * b[2]) * ci);
+    }
+    __syncthreads();  // the derivative tables give way to the face data
+
+    // ---- the neighbours' face columns
     if (elem_ok) {
         // items (face, field, column): 4 x 4 x kMaxCols
 #pragma unroll 4
@@ -228,28 +263,6 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, const int32_t* ti
         }
     }
     __syncthreads();
-
-    // ---- phase 2a: volume, k_v = sum_r Dr_r g_r ; k_s_d = (1/c_inv) sum_r (J^{-1})_{r,d} Dr_r v
-    Acc<T> kv, ks[3];
-    if (row_ok) {
-        const T* DrT = tabT + tx[kTixDrT];
-        T a[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
-#pragma unroll 2
-        for (int j = 0; j < NB; ++j) {
-            const T v = T(su[j]);
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                const T d = DrT[(r * NB + j) * NB + i];
-                a[r] += d * T(sg[r * NB + j]);
-                b[r] += d * v;
-            }
-        }
-        kv = Acc<T>(a[0] + a[1] + a[2]);
-        const T ci = T(1.0 / cinv);
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-            ks[d] = Acc<T>((T(Ji[d]) * b[0] + T(Ji[3 + d]) * b[1] + T(Ji[6 + d]) * b[2]) * ci);
-    }
 
     // ---- phase 2b: fluxes at the face points, items (face, point)
     if (elem_ok) {
@@ -294,16 +307,26 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, const int32_t* ti
     }
     __syncthreads();
 
-    // ---- phase 3: lift, k += sigma_f sum_q (M_ref^{-1} phi(point o(q)))_i flux(q)
+    // ---- phase 3: the order's lift rows into LDS (over U and N, no longer
+    // needed), then k += sigma_f sum_q (M_ref^{-1} phi(point o(q)))_i flux(q)
+    T* sL = reinterpret_cast<T*>(lds + S::U);
+    {
+        const T* src = tabT + tx[kTixLf0];
+#pragma unroll 4
+        for (int k = tid; k < 4 * kFaceQ * NB; k += kBlock) sL[k] = src[k];
+    }
+    __syncthreads();
     if (row_ok) {
         const T ci = T(1.0 / cinv);
-#pragma unroll 2
+        const int32_t* perm = tix + kTixPerm;
+#pragma unroll 1
         for (int f = 0; f < 4; ++f) {
-            const T* Lf = tabT + tx[kTixLf + f] + (info_oown(inf[f]) * kFaceQ) * NB + i;
+            const int32_t* pf = perm + info_oown(inf[f]) * kFaceQ;
+            const T* Lf = sL + f * kFaceQ * NB + i;
             T lv = T(0), ls = T(0);
-#pragma unroll 4
+#pragma unroll 8
             for (int q = 0; q < kFaceQ; ++q) {
-                const T l = Lf[q * NB];
+                const T l = Lf[pf[q] * NB];
                 lv += l * T(sf[(f * 2) * kFaceQ + q]);
                 ls += l * T(sf[(f * 2 + 1) * kFaceQ + q]);
             }
@@ -589,7 +612,9 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     sm.ops.clear();
     for (int q = 1; q <= kMaxSimplexOrder; ++q) sm.ops.push_back(simplex_ops(q));
     std::vector<double> tab;
-    std::vector<int32_t> tix(size_t(kTixStride) * kMaxSimplexOrder, 0);
+    std::vector<int32_t> tix(size_t(kTixPerm) + 6 * kFaceQ, 0);
+    for (int o = 0; o < 6; ++o)
+        for (int qq = 0; qq < kFaceQ; ++qq) tix[size_t(kTixPerm) + o * kFaceQ + qq] = perm[o][qq];
     std::vector<int32_t> cols;
     for (int q = 1; q <= kMaxSimplexOrder; ++q) {
         const SimplexOps& ops = sm.ops[q - 1];
@@ -626,6 +651,23 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
                     }
             t[kTixCols + f] = int32_t(cols.size());  // relative, fixed below
             cols.insert(cols.end(), ops.cols[f].begin(), ops.cols[f].end());
+        }
+        // the lift rows at the unpermuted points, all four faces: [f][q][i]
+        t[kTixLf0] = int32_t(tab.size());
+        for (int f = 0; f < 4; ++f) {
+            std::vector<long double> phi(size_t(kFaceQ) * nb), grad(size_t(nb) * 3);
+            for (int qq = 0; qq < kFaceQ; ++qq) {
+                const long double tp[2] = {kFaceRulePts[qq][0], kFaceRulePts[qq][1]};
+                long double x[3];
+                tet_face_map(f, tp, x);
+                tet_basis(q, x, &phi[size_t(qq) * nb], grad.data());
+            }
+            for (int qq = 0; qq < kFaceQ; ++qq)
+                for (int i = 0; i < nb; ++i) {
+                    long double sum = 0;
+                    for (int col : ops.cols[f]) sum += Minv[size_t(i) * nb + col] * phi[size_t(qq) * nb + col];
+                    tab.push_back(double(sum));
+                }
         }
     }
     const int32_t col_base = int32_t(tix.size());

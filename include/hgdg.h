@@ -273,6 +273,30 @@ int hgdg_structured_mesh(int dim, const int32_t* n, const double* lo, const doub
                          const int32_t* periodic, int simplices, int64_t* n_elements,
                          int32_t* n_vertices, int32_t* shape, double* coords, int64_t* node_ids);
 
+/* ---- interior-penalty Maxwell assembly on tetrahedra ---------------------
+ * DG-Max's curl-curl stiffness and permittivity mass
+ * (applications/DG-Max/Algorithms/DGMaxDiscretization.cpp:193-260 element
+ * matrices, :344-410 face matrices: -1/2 ({curl u}.[v]_T + [u]_T.{curl v})
+ * + stab/diam [u]_T.[v]_T inside, the same with factor -1 on the boundary)
+ * with hpGEM's Ainsworth-Coyle H(curl) set. Replaces
+ * DGMaxDiscretization::computeElementIntegrands/computeFaceIntegrals with
+ * the assembly into a global matrix (kernel/Utilities/GlobalMatrix.cpp:235-366). */
+typedef struct hgdg_maxwell_desc {
+    int32_t order;            /* 1 or 2                                          */
+    const double* epsilon;    /* [n_elements] relative permittivity, mesh order  */
+    double stab;              /* penalty: stab / (face diameter)                 */
+} hgdg_maxwell_desc;
+
+/* Needs a tetrahedral mesh on the context (the element orders of the mesh
+ * description are not used). First call with row_ptr == NULL for n_blocks and
+ * block_size (n = 12 or 30); then row_ptr [n_elements + 1], col_idx
+ * [n_blocks] (block rows and columns are mesh elements, columns ascending),
+ * K [n_blocks][n][n] and M [n_elements][n][n], row = test function of the row
+ * element, column = trial function of the column element; kernel_ms (may be
+ * NULL) receives the assembly kernel's duration. */
+int hgdg_maxwell_assemble(hgdg_ctx* ctx, const hgdg_maxwell_desc* d, int64_t* n_blocks, int32_t* block_size,
+                          int64_t* row_ptr, int64_t* col_idx, double* K, double* M, double* kernel_ms);
+
 /* hpGEM's Ainsworth-Coyle H(curl) DG set on the reference tetrahedron
  * (createDGBasisFunctionSet3DAinsworthCoyle, kernel/FE/BasisFunctions3D
  * AinsworthCoyle.cpp:417-505), orders 1 and 2: size, and values and curls
