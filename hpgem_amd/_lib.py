@@ -39,6 +39,8 @@ SIGNATURES = [
     ("hgdg_mesh_upload", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MeshDesc)]),
     ("hgdg_mesh_info", ctypes.c_int, [ctypes.c_void_p, _I64, _I64, _I64, _I64]),
     ("hgdg_operator_set", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OpDesc)]),
+    ("hgdg_set_advection_tiles", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hgdg_get_advection_tiles", ctypes.c_int, [ctypes.c_void_p, _INT, _D]),
     ("hgdg_state_alloc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hgdg_state_set", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
     ("hgdg_state_get", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),

@@ -178,6 +178,19 @@ class HpgemAPIDevice:
         except Exception:
             pass
 
+    # ---- launch shape of the tensor advection kernel ----------------------
+    def setAdvectionTiles(self, tiles):
+        """Tiles of elements per workgroup (1, 2, 4, 8), or 0 to time every
+        choice and keep the fastest (done by default when the operator is set)."""
+        check(self.lib.hgdg_set_advection_tiles(self.h, int(tiles)))
+
+    def getAdvectionTiles(self):
+        """(tiles in use, timed ms per launch of candidates 1, 2, 4, 8)."""
+        t = ctypes.c_int()
+        ms = np.zeros(4)
+        check(self.lib.hgdg_get_advection_tiles(self.h, ctypes.byref(t), dptr(ms)))
+        return t.value, ms
+
     # ---- time-integration vectors (reference layout) ----------------------
     def setTimeIntegrationVector(self, vec, values):
         v = np.ascontiguousarray(values, dtype=np.float64)

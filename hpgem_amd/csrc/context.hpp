@@ -77,6 +77,11 @@ struct hgdg_ctx {
     hipGraphExec_t graph_exec = nullptr;
     double dt_estimate = 0;  // adaptive stepping: next step size (0 = not started)
     double last_kernel_ms = 0;  // duration of the last assembly kernel (HIP events)
+    // tensor advection: tiles of elements each workgroup takes (1 = one tile,
+    // no prefetch); chosen by timing at operator set-up unless fixed by the caller
+    int adv_tiles = 1;
+    int adv_tiles_fixed = 0;
+    double adv_tile_ms[4] = {0, 0, 0, 0};  // timed candidates 1, 2, 4, 8 (0 = not timed)
 
     bool simplex() const { return shape == HGDG_SHAPE_TETRAHEDRON; }
     int64_t n_dof() const { return dof; }
@@ -103,6 +108,9 @@ void tensor_project_device(hgdg_ctx* c, Vec& v, int field, int n_modes, const do
                            const double* gauss = nullptr);
 void tensor_operator_set(hgdg_ctx* c, const hgdg_op_desc* op);
 void tensor_rhs(hgdg_ctx* c, const StageIO& io);
+// Chooses c->adv_tiles for the advection operator by timing each candidate
+// on scratch vectors of the mesh's size (or takes c->adv_tiles_fixed).
+void tensor_tune(hgdg_ctx* c);
 void tensor_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* l2, double* max_abs);
 void tensor_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, const double* amp, const double* k,
                         const double* shift, double* l2, double* max_abs);

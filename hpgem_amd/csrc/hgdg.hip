@@ -179,6 +179,7 @@ void operator_set(hgdg_ctx* c, const hgdg_op_desc* op) {
     c->op.c_inv = nullptr;
     c->has_op = true;
     c->drop_graph();
+    if (!c->simplex()) tensor_tune(c);
     if (old_fields != c->n_fields) c->vecs.clear();  // the field count changed the vector size
 }
 
@@ -495,6 +496,29 @@ int hgdg_operator_set(hgdg_ctx* ctx, const hgdg_op_desc* op) {
         require(ctx != nullptr, HGDG_ERR_ARG, "ctx is NULL");
         HGDG_HIP(hipSetDevice(ctx->device));
         hgdg::operator_set(ctx, op);
+    });
+}
+
+int hgdg_set_advection_tiles(hgdg_ctx* ctx, int tiles) {
+    return run([&] {
+        require(ctx != nullptr, HGDG_ERR_ARG, "ctx is NULL");
+        require(tiles == 0 || tiles == 1 || tiles == 2 || tiles == 4 || tiles == 8, HGDG_ERR_ARG,
+                "tiles must be 0 (timed choice), 1, 2, 4 or 8");
+        HGDG_HIP(hipSetDevice(ctx->device));
+        ctx->adv_tiles_fixed = tiles;
+        if (ctx->has_op && !ctx->simplex()) {
+            ctx->drop_graph();
+            hgdg::tensor_tune(ctx);
+        }
+    });
+}
+
+int hgdg_get_advection_tiles(hgdg_ctx* ctx, int* tiles, double* ms) {
+    return run([&] {
+        require(ctx != nullptr && tiles != nullptr, HGDG_ERR_ARG, "ctx and tiles must be non-NULL");
+        *tiles = ctx->adv_tiles;
+        if (ms)
+            for (int i = 0; i < 4; ++i) ms[i] = ctx->adv_tile_ms[i];
     });
 }
 

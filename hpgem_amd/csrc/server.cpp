@@ -143,6 +143,7 @@ hgdg_ctx* build_solve(int device, bool mixed, const Request& r) {
     } else {
         for (int i = 0; i < 3; ++i) op.velocity[i] = d.velocity[i];
     }
+    c->adv_tiles_fixed = 1;  // requests run in the batched kernels; no launch-shape timing per request
     operator_set(c.get(), &op);
     c->vecs.resize(5);
     set_storage(c.get(), c->vecs[0], false);

@@ -31,16 +31,17 @@ namespace hgdg {
 namespace {
 
 // int32 table index per order (p-1): offsets into the FP64 (and the FP32)
-// table arrays and, per local face, the offset of its list of face columns;
-// after the four orders, the permutation of the face rule points under each
-// orientation map. The kernel copies the whole index into LDS and stages the
-// tables of a block's order there, so table reads are LDS reads.
+// table arrays and, per local face, the offset of its list of face columns.
+// The face tables are stored once per orientation o of the face rule points
+// (the points permuted by the orientation map), so no lookup sits between a
+// thread and its table reads.
 constexpr int kTixDrT = 0;           // [3][nb][nb]: DrT[r][j][i] = Dr_r(i, j)
 constexpr int kTixM = 1;             // [nb][nb]: M_ref
+constexpr int kTixVf = 2;            // + f: [6][ncol][kFaceQ]: trace of face column c at point o(q)
+constexpr int kTixLf = 6;            // + f: [6][kFaceQ][nb]: (M_ref^{-1} phi(point o(q)))_i
 constexpr int kTixCols = 10;         // + f
 constexpr int kTixLf0 = 14;          // [4][kFaceQ][nb]: the lift rows at the unpermuted face points
-constexpr int kTixVt0 = 15;          // [4][ncol][kFaceQ]: face column traces at the unpermuted points
-constexpr int kTixStride = 16;
+constexpr int kTixStride = 15;
 constexpr int kTixPerm = 4 * kTixStride;  // [6][16 face points]: face point index after orientation o
 constexpr int kMaxCols = 15;         // face columns of an order-4 element
 constexpr int kTixMax = kTixPerm + 6 * 16 + 4 * 4 * kMaxCols;  // whole index table (LDS copy in the kernel)
@@ -132,9 +133,7 @@ struct SimplexShape {
     static constexpr int F = N + EPB * 16 * kMaxCols;       // [EPB][4 faces][2][kFaceQ] weighted fluxes at face points
     static constexpr int FACE_END = F + EPB * 8 * kFaceQ;
     static constexpr int DR_END = W + 3 * NB * NB;          // DrT [3][NB][NB]
-    static constexpr int VT = FACE_END;                     // face column traces of every order, [4 orders] ...
-    static constexpr int VT_END = VT + 4 * 16 * (3 + 6 + 10 + 15);
-    static constexpr int TOTAL = VT_END > DR_END ? VT_END : DR_END;
+    static constexpr int TOTAL = FACE_END > DR_END ? FACE_END : DR_END;
     static_assert(4 * kFaceQ * NB <= F, "lift tables must fit in U and N");
 };
 
@@ -149,16 +148,11 @@ constexpr int lds_doubles() {
 
 // Traces of the four fields of a neighbour of order PN at face point q, from
 // its face columns in LDS and its face table for orientation o.
-// Offset of the unpermuted trace table of order p, local face f, inside the
-// all-orders block (orders ascending, faces ascending, [ncol][kFaceQ] each).
-__host__ __device__ constexpr int vt_offset(int p, int f) {
-    return 4 * 16 * (p > 1 ? 3 : 0) + 4 * 16 * (p > 2 ? 6 : 0) + 4 * 16 * (p > 3 ? 10 : 0) + f * simplex_ncol(p) * 16;
-}
-
 template <int PN>
-__device__ __forceinline__ void nbr_traces(const double* vt, int lfn, int qn, const double* cols4, double (&t)[4]) {
+__device__ __forceinline__ void nbr_traces(const SimplexParams& p, const int32_t* tix, int lfn, int o, int q,
+                                           const double* cols4, double (&t)[4]) {
     constexpr int NC = simplex_ncol(PN);
-    const double* V = vt + vt_offset(PN, lfn) + qn;
+    const double* V = p.tab + tix[(PN - 1) * kTixStride + kTixVf + lfn] + (o * NC) * kFaceQ + q;
     t[0] = t[1] = t[2] = t[3] = 0;
 #pragma unroll 3
     for (int c = 0; c < NC; ++c) {
@@ -252,13 +246,7 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, const int32_t* ti
     }
     __syncthreads();  // the derivative tables give way to the face data
 
-    // ---- the face column traces of every order, and the neighbours' face columns
-    double* svt = lds + S::VT;
-    {
-        const double* src = p.tab + tix[kTixVt0];  // order 1 first, all orders contiguous
-#pragma unroll 4
-        for (int k = tid; k < S::VT_END - S::VT; k += kBlock) svt[k] = src[k];
-    }
+    // ---- the neighbours' face columns
     if (elem_ok) {
         // items (face, field, column): 4 x 4 x kMaxCols
 #pragma unroll 4
@@ -285,8 +273,7 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, const int32_t* ti
             const int fi = f == 0 ? inf[0] : f == 1 ? inf[1] : f == 2 ? inf[2] : inf[3];
             const double n0 = p.normal[(0 * 4 + f) * E + e], n1 = p.normal[(1 * 4 + f) * E + e],
                          n2 = p.normal[(2 * 4 + f) * E + e];
-            const int32_t* perm = tix + kTixPerm;
-            const double* Vo = svt + vt_offset(P, f) + perm[info_oown(fi) * kFaceQ + q];
+            const double* Vo = p.tab + tx[kTixVf + f] + (info_oown(fi) * S::NCOL) * kFaceQ + q;
             const int32_t* colso = tix + tx[kTixCols + f];
             double o4[4] = {0, 0, 0, 0};
 #pragma unroll 3
@@ -300,12 +287,12 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, const int32_t* ti
             if (nb >= 0) {
                 double t4[4];
                 const double* cols4 = sn + f * 4 * kMaxCols;
-                const int lfn = info_lf(fi), qn = perm[info_onbr(fi) * kFaceQ + q];
+                const int lfn = info_lf(fi), on = info_onbr(fi);
                 switch (info_pn(fi)) {
-                    case 1: nbr_traces<1>(svt, lfn, qn, cols4, t4); break;
-                    case 2: nbr_traces<2>(svt, lfn, qn, cols4, t4); break;
-                    case 3: nbr_traces<3>(svt, lfn, qn, cols4, t4); break;
-                    default: nbr_traces<4>(svt, lfn, qn, cols4, t4); break;
+                    case 1: nbr_traces<1>(p, tix, lfn, on, q, cols4, t4); break;
+                    case 2: nbr_traces<2>(p, tix, lfn, on, q, cols4, t4); break;
+                    case 3: nbr_traces<3>(p, tix, lfn, on, q, cols4, t4); break;
+                    default: nbr_traces<4>(p, tix, lfn, on, q, cols4, t4); break;
                 }
                 Jv = -0.5 * (n0 * (o4[1] - t4[1]) + n1 * (o4[2] - t4[2]) + n2 * (o4[3] - t4[3]));
                 Js = -0.5 * (o4[0] - t4[0]);
@@ -607,9 +594,9 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     c->n_faces = n_total;
     c->n_internal = n_internal;
 
-    // reference tables of every order (see the kernel): DrT, M_ref, the face
-    // columns of each local face, the lift rows M_ref^{-1} phi(point) and the
-    // face column traces at the face rule points
+    // reference tables of every order (see the kernel): DrT, M_ref, and per
+    // local face and orientation o the traces of the face columns at the
+    // permuted face rule points and the lift rows M_ref^{-1} phi(point)
     int perm[6][kFaceQ];  // face point q after orientation o of the reference triangle
     for (int o = 0; o < 6; ++o)
         for (int qq = 0; qq < kFaceQ; ++qq) {
@@ -649,6 +636,19 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
                 tet_basis(q, x, &phi[size_t(qq) * nb], grad.data());
             }
             require(int(ops.cols[f].size()) == simplex_ncol(q), HGDG_ERR_ARG, "unexpected face column count");
+            t[kTixVf + f] = int32_t(tab.size());
+            for (int o = 0; o < 6; ++o)
+                for (int col : ops.cols[f])
+                    for (int qq = 0; qq < kFaceQ; ++qq) tab.push_back(double(phi[size_t(perm[o][qq]) * nb + col]));
+            t[kTixLf + f] = int32_t(tab.size());
+            for (int o = 0; o < 6; ++o)
+                for (int qq = 0; qq < kFaceQ; ++qq)
+                    for (int i = 0; i < nb; ++i) {
+                        long double sum = 0;
+                        for (int col : ops.cols[f])
+                            sum += Minv[size_t(i) * nb + col] * phi[size_t(perm[o][qq]) * nb + col];
+                        tab.push_back(double(sum));
+                    }
             t[kTixCols + f] = int32_t(cols.size());  // relative, fixed below
             cols.insert(cols.end(), ops.cols[f].begin(), ops.cols[f].end());
         }
@@ -668,27 +668,6 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
                     for (int col : ops.cols[f]) sum += Minv[size_t(i) * nb + col] * phi[size_t(qq) * nb + col];
                     tab.push_back(double(sum));
                 }
-        }
-    }
-    // unpermuted face column traces of every order, contiguous (vt_offset)
-    {
-        const int32_t vt0 = int32_t(tab.size());
-        for (int q = 1; q <= kMaxSimplexOrder; ++q) {
-            const SimplexOps& ops = sm.ops[q - 1];
-            const int nb = ops.nb;
-            tix[size_t(q - 1) * kTixStride + kTixVt0] = vt0;
-            for (int f = 0; f < 4; ++f) {
-                require(int32_t(tab.size()) == vt0 + vt_offset(q, f), HGDG_ERR_ARG, "trace table layout");
-                std::vector<long double> phi(size_t(kFaceQ) * nb), grad(size_t(nb) * 3);
-                for (int qq = 0; qq < kFaceQ; ++qq) {
-                    const long double tp[2] = {kFaceRulePts[qq][0], kFaceRulePts[qq][1]};
-                    long double x[3];
-                    tet_face_map(f, tp, x);
-                    tet_basis(q, x, &phi[size_t(qq) * nb], grad.data());
-                }
-                for (int col : ops.cols[f])
-                    for (int qq = 0; qq < kFaceQ; ++qq) tab.push_back(double(phi[size_t(qq) * nb + col]));
-            }
         }
     }
     const int32_t col_base = int32_t(tix.size());

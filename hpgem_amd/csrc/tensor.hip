@@ -123,34 +123,53 @@ __device__ __forceinline__ double face_jump(uint32_t m, double own, double other
     return (m == MODE_NBR) ? other - own : 0.5 * (other - own);
 }
 
-template <int DIM, int N1, typename T>
-__device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk) {
+// Per-element data of the face terms: upwind modes, scaled velocity, and the
+// neighbour of every face. It is loaded together with the stage input, so
+// after the block's barrier only the neighbours' traces are still to come.
+template <int DIM>
+struct ElemMeta {
+    uint32_t md;
+    double ah[DIM];
+    int32_t nb[2 * DIM];
+};
+
+template <int DIM, int N1>
+__device__ __forceinline__ bool advection_load(const RhsParams& p, int64_t blk, double (&w)[N1], ElemMeta<DIM>& m) {
     using S = Pencil<DIM, N1>;
-    __shared__ double s_w[S::EPB * S::NB];  // FP64 stage input of the block's elements
-    __shared__ T s_D[N1 * N1];
-    __shared__ T s_Mi[2 * N1];  // s_Mi[s * N1 + a] = Mi(a, s)
+    const int le = threadIdx.x / S::NP, pc = threadIdx.x % S::NP;
+    const int64_t e = blk * S::EPB + le;
+    const bool active = (le < S::EPB) && (e < p.n_elem);
+    if (active) {
+        m.md = p.mode[e];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) m.ah[d] = p.ahat[d * p.n_elem + e];
+#pragma unroll
+        for (int f = 0; f < 2 * DIM; ++f) m.nb[f] = p.nbr[f * p.n_elem + e];
+        combine_pencil<N1>(p.io, e * S::NB + pc * N1, w);
+    }
+    return active;
+}
+
+// Everything after the block's stage input is in LDS: this thread's pencil w
+// (also in registers), the block's elements in s_w.
+template <int DIM, int N1, typename T>
+__device__ __forceinline__ void advection_finish(const RhsParams& p, int64_t blk, const double (&w)[N1],
+                                                 const ElemMeta<DIM>& meta, const double* s_w, const T* s_D,
+                                                 const T* s_Mi) {
+    using S = Pencil<DIM, N1>;
     const int tid = threadIdx.x;
-    if (tid < N1 * N1) s_D[tid] = T(p.Dr[tid]);
-    if (tid < 2 * N1) s_Mi[tid] = T(p.Mi[(tid % N1) * N1 + tid / N1]);
     const int le = tid / S::NP;
     const int pc = tid % S::NP;
     const int64_t e = blk * S::EPB + le;
     const bool active = (le < S::EPB) && (e < p.n_elem);
     const int ib = pc % N1;                    // y index of this pencil
     const int ic = DIM == 3 ? pc / N1 : 0;     // z index of this pencil
-    double* sw = s_w + le * S::NB;
-    double w[N1];
-    if (active) {
-        combine_pencil<N1>(p.io, e * S::NB + pc * N1, w);
-#pragma unroll
-        for (int a = 0; a < N1; ++a) sw[pc * N1 + a] = w[a];
-    }
-    __syncthreads();
+    const double* sw = s_w + le * S::NB;
     if (!active) return;
 
     T ah[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) ah[d] = T(p.ahat[d * p.n_elem + e]);
+    for (int d = 0; d < DIM; ++d) ah[d] = T(meta.ah[d]);
 
     // volume (strong form): -sum_d ahat_d (Dr along d) w
     T vol[N1];
@@ -182,14 +201,14 @@ __device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk)
     for (int a = 0; a < N1; ++a) acc[a] = Acc<T>(vol[a]);
 
     // faces: -(2s-1) ahat_d Mi(idx_d, s) (T_{d,s} - own_{d,s}); jumps in FP64
-    const uint32_t md = p.mode[e];
+    const uint32_t md = meta.md;
 #pragma unroll
     for (int s = 0; s < 2; ++s) {  // x faces: one trace value per pencil
         const uint32_t m = (md >> (3 * s)) & 7u;
         if (m == MODE_OWN) continue;
         double other = 0.0;
         if (m == MODE_NBR || m == MODE_AVG) {
-            const int64_t nb = p.nbr[s * p.n_elem + e];
+            const int64_t nb = meta.nb[s];
             const int64_t lnb = nb - blk * S::EPB;
             if (lnb >= 0 && lnb < S::EPB) {
                 other = s_w[lnb * S::NB + pc * N1 + (1 - s)];  // neighbour in this block
@@ -218,7 +237,7 @@ __device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk)
             const int nbr_pc = d == 1 ? (1 - s) + N1 * ic : ib + N1 * (1 - s);
             double other[N1];
             if (m == MODE_NBR || m == MODE_AVG) {
-                const int64_t nb = p.nbr[f * p.n_elem + e];
+                const int64_t nb = meta.nb[f];
                 const int64_t lnb = nb - blk * S::EPB;
                 if (lnb >= 0 && lnb < S::EPB) {
 #pragma unroll
@@ -249,8 +268,78 @@ __device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk)
 }
 
 template <int DIM, int N1, typename T>
-__global__ __launch_bounds__(kBlock) void tensor_advection_rhs(const RhsParams p) {
+__device__ __forceinline__ void advection_tables(const RhsParams& p, T* s_D, T* s_Mi) {
+    const int tid = threadIdx.x;
+    if (tid < N1 * N1) s_D[tid] = T(p.Dr[tid]);
+    if (tid < 2 * N1) s_Mi[tid] = T(p.Mi[(tid % N1) * N1 + tid / N1]);  // s_Mi[s * N1 + a] = Mi(a, s)
+}
+
+template <int DIM, int N1, typename T>
+__device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk) {
+    using S = Pencil<DIM, N1>;
+    __shared__ double s_w[S::EPB * S::NB];  // FP64 stage input of the block's elements
+    __shared__ T s_D[N1 * N1];
+    __shared__ T s_Mi[2 * N1];
+    advection_tables<DIM, N1, T>(p, s_D, s_Mi);
+    double w[N1];
+    ElemMeta<DIM> meta;
+    if (advection_load<DIM, N1>(p, blk, w, meta)) {
+        const int le = threadIdx.x / S::NP, pc = threadIdx.x % S::NP;
+#pragma unroll
+        for (int a = 0; a < N1; ++a) s_w[le * S::NB + pc * N1 + a] = w[a];
+    }
+    __syncthreads();
+    advection_finish<DIM, N1, T>(p, blk, w, meta, s_w, s_D, s_Mi);
+}
+
+// Minimum waves per SIMD the compiler is asked to keep for the one-tile
+// kernel: 3D p=3 in FP64 needs a few registers beyond the 72 of seven waves
+// otherwise, and fits them without spilling.
+template <int DIM, int N1, typename T>
+constexpr int adv_min_waves() {
+    return (DIM == 3 && N1 == 4 && sizeof(T) == 8) ? 7 : 1;
+}
+
+template <int DIM, int N1, typename T>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(adv_min_waves<DIM, N1, T>())))
+void tensor_advection_rhs(const RhsParams p) {
     advection_block<DIM, N1, T>(p, xcd_block(blockIdx.x, gridDim.x));
+}
+
+// The same work with each block taking `tiles` consecutive tiles of EPB
+// elements: while a tile is computed, the next tile's stage input is already
+// being loaded into registers, so the loads of one tile overlap the
+// arithmetic and neighbour reads of the previous one, and a launch can be
+// sized to fill the GPU in one round. Per element the arithmetic is exactly
+// that of tensor_advection_rhs.
+template <int DIM, int N1, typename T>
+__global__ __launch_bounds__(kBlock) void tensor_advection_rhs_tiled(const RhsParams p, int tiles) {
+    using S = Pencil<DIM, N1>;
+    __shared__ double s_w[S::EPB * S::NB];
+    __shared__ T s_D[N1 * N1];
+    __shared__ T s_Mi[2 * N1];
+    advection_tables<DIM, N1, T>(p, s_D, s_Mi);
+    const int64_t n_tiles = (p.n_elem + S::EPB - 1) / S::EPB;
+    const int64_t t0 = xcd_block(blockIdx.x, gridDim.x) * tiles;
+    const int64_t t1 = t0 + tiles < n_tiles ? t0 + tiles : n_tiles;
+    const int le = threadIdx.x / S::NP, pc = threadIdx.x % S::NP;
+    double wn[N1];
+    ElemMeta<DIM> mn;
+    bool act_n = t0 < t1 && advection_load<DIM, N1>(p, t0, wn, mn);
+    for (int64_t t = t0; t < t1; ++t) {
+        double w[N1];
+#pragma unroll
+        for (int a = 0; a < N1; ++a) w[a] = wn[a];
+        const ElemMeta<DIM> meta = mn;
+        const bool act = act_n;
+        __syncthreads();  // the previous tile's readers of s_w are done
+        if (act)
+#pragma unroll
+            for (int a = 0; a < N1; ++a) s_w[le * S::NB + pc * N1 + a] = w[a];
+        __syncthreads();
+        act_n = t + 1 < t1 && advection_load<DIM, N1>(p, t + 1, wn, mn);
+        advection_finish<DIM, N1, T>(p, t, w, meta, s_w, s_D, s_Mi);
+    }
 }
 
 // Block b of a batched launch belongs to problem s with off[s] <= b < off[s+1].
@@ -1076,6 +1165,102 @@ void tensor_operator_set(hgdg_ctx* c, const hgdg_op_desc* op) {
     HGDG_HIP(hipStreamSynchronize(c->stream));
 }
 
+static RhsParams advection_params(hgdg_ctx* c, const StageIO& io) {
+    RhsParams p{};
+    p.n_elem = c->n_elem;
+    p.io = io;
+    p.ahat = c->tm.d_geo.ptr;
+    p.nbr = c->tm.d_nbr.ptr;
+    p.mode = c->tm.d_mode.ptr;
+    p.Dr = Dr_ptr(c);
+    p.Mi = Mi_ptr(c);
+    return p;
+}
+
+static void launch_advection(hgdg_ctx* c, const RhsParams& p, int tiles) {
+    const bool mixed = c->op.precision == HGDG_PREC_MIXED_FP32;
+    dispatch_tensor(c->dim, c->tm.n1, [&](auto D, auto N) {
+        constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
+        using S = Pencil<DIMc, N1c>;
+        const int64_t n_tiles = (c->n_elem + S::EPB - 1) / S::EPB;
+        if (tiles > 1) {
+            const int64_t grid = (n_tiles + tiles - 1) / tiles;
+            if (mixed)
+                hipLaunchKernelGGL((tensor_advection_rhs_tiled<DIMc, N1c, float>), dim3(unsigned(grid)), dim3(kBlock),
+                                   0, c->stream, p, tiles);
+            else
+                hipLaunchKernelGGL((tensor_advection_rhs_tiled<DIMc, N1c, double>), dim3(unsigned(grid)),
+                                   dim3(kBlock), 0, c->stream, p, tiles);
+            return;
+        }
+        if (mixed)
+            hipLaunchKernelGGL((tensor_advection_rhs<DIMc, N1c, float>), dim3(unsigned(n_tiles)), dim3(kBlock), 0,
+                               c->stream, p);
+        else
+            hipLaunchKernelGGL((tensor_advection_rhs<DIMc, N1c, double>), dim3(unsigned(n_tiles)), dim3(kBlock), 0,
+                               c->stream, p);
+    });
+    HGDG_HIP(hipGetLastError());
+}
+
+void tensor_tune(hgdg_ctx* c) {
+    for (double& t : c->adv_tile_ms) t = 0;
+    if (c->op.kind != HGDG_OP_ADVECTION_UPWIND) return;
+    if (c->adv_tiles_fixed > 0) {
+        c->adv_tiles = c->adv_tiles_fixed;
+        return;
+    }
+    // One RK stage's shape: u + a k -> k', with the stage vectors in the
+    // storage type of the operator's precision. Values do not change the
+    // timing (no data-dependent branches), so the inputs are zero.
+    const bool mixed = c->op.precision == HGDG_PREC_MIXED_FP32;
+    const size_t n = size_t(c->n_dof());
+    DevBuf<double> u, k, out;
+    u.alloc(n);
+    k.alloc(n);
+    out.alloc(n);
+    HGDG_HIP(hipMemsetAsync(u.ptr, 0, n * sizeof(double), c->stream));
+    HGDG_HIP(hipMemsetAsync(k.ptr, 0, n * sizeof(double), c->stream));
+    StageIO io{};
+    io.n_in = 2;
+    io.in[0] = u.ptr;
+    io.coef[0] = 1.0;
+    io.in[1] = k.ptr;
+    io.coef[1] = 0.5;
+    io.in_f32 = mixed ? 2u : 0u;
+    io.out = out.ptr;
+    io.out_f32 = mixed ? 1 : 0;
+    const RhsParams p = advection_params(c, io);
+    hipEvent_t ev[2];
+    HGDG_HIP(hipEventCreate(&ev[0]));
+    HGDG_HIP(hipEventCreate(&ev[1]));
+    constexpr int kCand[4] = {1, 2, 4, 8};
+    constexpr int kReps = 20;
+    double best = 0;
+    int best_tiles = 1;
+    for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < 4; ++i) {
+            for (int r = 0; r < 3; ++r) launch_advection(c, p, kCand[i]);
+            HGDG_HIP(hipEventRecord(ev[0], c->stream));
+            for (int r = 0; r < kReps; ++r) launch_advection(c, p, kCand[i]);
+            HGDG_HIP(hipEventRecord(ev[1], c->stream));
+            HGDG_HIP(hipEventSynchronize(ev[1]));
+            float ms = 0;
+            HGDG_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            const double t = double(ms) / kReps;
+            if (pass == 0 || t < c->adv_tile_ms[i]) c->adv_tile_ms[i] = t;
+        }
+    for (int i = 0; i < 4; ++i)
+        if (i == 0 || c->adv_tile_ms[i] < best) {
+            best = c->adv_tile_ms[i];
+            best_tiles = kCand[i];
+        }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    HGDG_HIP(hipStreamSynchronize(c->stream));
+    c->adv_tiles = best_tiles;
+}
+
 void tensor_rhs(hgdg_ctx* c, const StageIO& io) {
     const bool mixed = c->op.precision == HGDG_PREC_MIXED_FP32;
     if (c->op.kind == HGDG_OP_ACOUSTIC_CENTRAL) {
@@ -1101,26 +1286,7 @@ void tensor_rhs(hgdg_ctx* c, const StageIO& io) {
         HGDG_HIP(hipGetLastError());
         return;
     }
-    RhsParams p{};
-    p.n_elem = c->n_elem;
-    p.io = io;
-    p.ahat = c->tm.d_geo.ptr;
-    p.nbr = c->tm.d_nbr.ptr;
-    p.mode = c->tm.d_mode.ptr;
-    p.Dr = Dr_ptr(c);
-    p.Mi = Mi_ptr(c);
-    dispatch_tensor(c->dim, c->tm.n1, [&](auto D, auto N) {
-        constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
-        using S = Pencil<DIMc, N1c>;
-        const int64_t grid = (c->n_elem + S::EPB - 1) / S::EPB;
-        if (mixed)
-            hipLaunchKernelGGL((tensor_advection_rhs<DIMc, N1c, float>), dim3(unsigned(grid)), dim3(kBlock), 0,
-                               c->stream, p);
-        else
-            hipLaunchKernelGGL((tensor_advection_rhs<DIMc, N1c, double>), dim3(unsigned(grid)), dim3(kBlock), 0,
-                               c->stream, p);
-    });
-    HGDG_HIP(hipGetLastError());
+    launch_advection(c, advection_params(c, io), c->adv_tiles);
 }
 
 void tensor_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, double* max_abs) {

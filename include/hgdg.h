@@ -114,6 +114,19 @@ int hgdg_mesh_info(hgdg_ctx* ctx, int64_t* n_elements, int64_t* n_faces, int64_t
  * (kernel/Base/HpgemAPILinear_Impl.h:131-139,250-288). */
 int hgdg_operator_set(hgdg_ctx* ctx, const hgdg_op_desc* op);
 
+/* Launch shape of the tensor-product advection kernel: each workgroup takes
+ * `tiles` consecutive tiles of elements (1, 2, 4 or 8), loading the next
+ * tile's stage input while it computes the current one. 0 (the default)
+ * lets hgdg_operator_set time every choice on the mesh and keep the fastest.
+ * Results are bitwise the same for every choice. No counterpart in the
+ * reference (a launch parameter of this implementation); applies to the next
+ * hgdg_operator_set, or at once if an operator is set. */
+int hgdg_set_advection_tiles(hgdg_ctx* ctx, int tiles);
+
+/* The tile count in use and the timed average launch (ms) of each candidate
+ * 1, 2, 4, 8 (0 where not timed). ms may be NULL. */
+int hgdg_get_advection_tiles(hgdg_ctx* ctx, int* tiles, double* ms);
+
 /* Allocate n time-integration vectors on the device (all zero).
  * Replaces setNumberOfTimeIntegrationVectorsGlobally (HpgemAPIBase_Impl.h:141-156). */
 int hgdg_state_alloc(hgdg_ctx* ctx, int n_vectors);
