@@ -31,20 +31,22 @@ namespace hgdg {
 namespace {
 
 // int32 table index per order (p-1): offsets into the FP64 (and the FP32)
-// table arrays and, per local face, the offset of its list of face columns.
-// The face tables are stored once per orientation o of the face rule points
-// (the points permuted by the orientation map), so no lookup sits between a
-// thread and its table reads.
+// table arrays and, per local face, the offset of its list of face columns
+// (the basis functions whose trace on that face does not vanish). Face terms
+// work with an orthonormal hierarchical basis chi_m of P_4 on the reference
+// triangle (degree by degree): the trace tables give the chi-coefficients of
+// each face column's trace when the face is parametrised through orientation
+// o, the lift tables map flux coefficients back to the element.
 constexpr int kTixDrT = 0;           // [3][nb][nb]: DrT[r][j][i] = Dr_r(i, j)
 constexpr int kTixM = 1;             // [nb][nb]: M_ref
-constexpr int kTixVf = 2;            // + f: [6][ncol][kFaceQ]: trace of face column c at point o(q)
-constexpr int kTixLf = 6;            // + f: [6][kFaceQ][nb]: (M_ref^{-1} phi(point o(q)))_i
-constexpr int kTixCols = 10;         // + f
-constexpr int kTixLf0 = 14;          // [4][kFaceQ][nb]: the lift rows at the unpermuted face points
-constexpr int kTixStride = 15;
-constexpr int kTixPerm = 4 * kTixStride;  // [6][16 face points]: face point index after orientation o
+constexpr int kTixCols = 2;          // + f
+constexpr int kTixProj = 6;          // [4 f][6 o][15 m][ncol]: chi_m-coefficients of the face columns' traces
+constexpr int kTixLift = 7;          // [4 f][ncol m][nb i]: (M_ref^{-1} int_f phi chi_m)_i, own face coordinates
+constexpr int kTixStride = 8;
+constexpr int kTixRel = 4 * kTixStride;  // [6 own][6 neighbour]: orientation of the neighbour's face seen from ours
+constexpr int kTixIdent = kTixRel + 36;  // the identity orientation
 constexpr int kMaxCols = 15;         // face columns of an order-4 element
-constexpr int kTixMax = kTixPerm + 6 * 16 + 4 * 4 * kMaxCols;  // whole index table (LDS copy in the kernel)
+constexpr int kTixMax = kTixIdent + 1 + 4 * 4 * kMaxCols;  // whole index table (LDS copy in the kernel)
 
 // packed per-face info
 __host__ __device__ constexpr int info_lf(int v) { return v & 3; }
@@ -99,7 +101,6 @@ struct SimplexParams {
     int64_t group_start[5];
     int64_t group_dof[5];
     int32_t group_block[5];
-    const int4* blocks;    // per block: order, first device element, element count, unused
     const double* jinv;    // [9][E]: (J^{-1})_{e,d} at [(3e + d) E + elem]
     const double* cinv;    // [E]
     const double* sigma;   // [4][E]: face area ratio / |det J|
@@ -110,7 +111,6 @@ struct SimplexParams {
     const float* tab32;    // the same tables in FP32 (mixed path), same offsets
     const int32_t* tix;
     int32_t n_tix;
-    double face_w[kFaceQ]; // face rule weights
 };
 
 __host__ __device__ constexpr int simplex_nb(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
@@ -124,62 +124,86 @@ struct SimplexShape {
     static constexpr int NCOL = simplex_ncol(P);
     static constexpr int TPE = simplex_tpe(P);  // threads per element
     static constexpr int EPB = kBlock / TPE;
-    // LDS carve-up (doubles). The work area W after the stage input holds, in
-    // turn, the derivative tables (volume), then the neighbours' face columns
-    // N and the point fluxes F (faces); the lift tables reuse U and N.
-    static constexpr int U = 0;                             // [EPB][4][NB] stage input
+    // LDS carve-up of the RHS kernel (in doubles; T-typed regions use the
+    // first half of theirs in FP32). FM, FI and U live through the whole
+    // kernel; the work area W holds in turn the volume data (G, DR) and the
+    // face data (SO, SN, SF).
+    static constexpr int FM = 0;                            // [EPB][4 faces][4]: unit normal, sigma
+    static constexpr int FI = FM + EPB * 16;                // [EPB][4 faces] (int2): neighbour, packed face info
+    static constexpr int U = FI + EPB * 4;                  // [EPB][4][NB] stage input
     static constexpr int W = U + EPB * 4 * NB;
-    static constexpr int N = W;                             // [EPB][4 faces][4 fields][kMaxCols] neighbours' face columns
-    static constexpr int F = N + EPB * 16 * kMaxCols;       // [EPB][4 faces][2][kFaceQ] weighted fluxes at face points
-    static constexpr int FACE_END = F + EPB * 8 * kFaceQ;
-    static constexpr int DR_END = W + 3 * NB * NB;          // DrT [3][NB][NB]
-    static constexpr int TOTAL = FACE_END > DR_END ? FACE_END : DR_END;
-    static_assert(4 * kFaceQ * NB <= F, "lift tables must fit in U and N");
+    static constexpr int G = W;                             // [EPB][4][NB] (T): v, then g_r = sum_d (J^{-1})_{r,d} s_d
+    static constexpr bool DR_LDS = P < 4;                   // DrT staged in LDS (order 4: read through the caches)
+    static constexpr int DR = G + EPB * 4 * NB;             // [3][NB][NB] (T): DrT
+    static constexpr int VOL_END = DR + (DR_LDS ? 3 * NB * NB : 0);
+    static constexpr int SO = W;                            // [EPB][4 faces][NCOL] (double2): n.s and v at own face columns
+    static constexpr int SN = SO + EPB * 8 * NCOL;          // [EPB][4 faces][kMaxCols] (double2): the same for the neighbours
+    static constexpr int SF = SN + EPB * 8 * kMaxCols;      // [EPB][4 faces][NCOL] (T pairs): flux coefficients
+    static constexpr int FACE_END = SF + EPB * 8 * NCOL;
+    static constexpr int TOTAL = FACE_END > VOL_END ? FACE_END : VOL_END;
+    static_assert(FI % 2 == 0 && U % 2 == 0 && W % 2 == 0 && SN % 2 == 0 && SF % 2 == 0, "double2 alignment");
 };
 
-constexpr int lds_doubles() {
-    int m = 0;
-    m = SimplexShape<1>::TOTAL > m ? SimplexShape<1>::TOTAL : m;
-    m = SimplexShape<2>::TOTAL > m ? SimplexShape<2>::TOTAL : m;
-    m = SimplexShape<3>::TOTAL > m ? SimplexShape<3>::TOTAL : m;
-    m = SimplexShape<4>::TOTAL > m ? SimplexShape<4>::TOTAL : m;
-    return m;
-}
+template <typename T>
+struct FluxPair {
+    T v, s;  // flux coefficient of the v equation, of the s equations
+};
 
-// Traces of the four fields of a neighbour of order PN at face point q, from
-// its face columns in LDS and its face table for orientation o.
-template <int PN>
-__device__ __forceinline__ void nbr_traces(const SimplexParams& p, const int32_t* tix, int lfn, int o, int q,
-                                           const double* cols4, double (&t)[4]) {
-    constexpr int NC = simplex_ncol(PN);
-    const double* V = p.tab + tix[(PN - 1) * kTixStride + kTixVf + lfn] + (o * NC) * kFaceQ + q;
-    t[0] = t[1] = t[2] = t[3] = 0;
-#pragma unroll 3
+// Sum over c < NC of row[c] * x[c] for both components of x.
+template <int NC>
+__device__ __forceinline__ void dot_pairs(const double* row, const double2* x, double& a, double& b) {
+    a = 0;
+    b = 0;
+#pragma unroll 5
     for (int c = 0; c < NC; ++c) {
-        const double phi = V[c * kFaceQ];
-#pragma unroll
-        for (int fld = 0; fld < 4; ++fld) t[fld] += phi * cols4[fld * kMaxCols + c];
+        const double r = row[c];
+        const double2 v = x[c];
+        a += r * v.x;
+        b += r * v.y;
     }
 }
 
-// Tetrahedral elements of order P: one block of up to EPB consecutive device
-// elements, TPE threads per element, three phases separated by barriers:
-//  1. the stage input (all four fields) and J^{-T}-rotated s into LDS, and the
-//     face columns (the basis functions that do not vanish on the shared
-//     face) of all four neighbours, all loads in flight together;
-//  2. threads of a row: the volume terms; all threads over (face, point)
-//     items: both traces at the face rule's points (tables pre-permuted per
-//     orientation) and the central / wall flux times the weight;
-//  3. threads of a row: the lift of the point fluxes, M_ref^{-1} phi(point).
-// The face rule (order 8) integrates every face term exactly for p <= 4 on
-// both sides, so this is the reference's operator up to rounding. T = double:
-// FP64; T = float: the volume contractions and the lifts in FP32 on FP32
-// tables, compensated per-row sums, traces and fluxes in FP64.
+// Up to order 3 the LDS carve-up leaves room for five blocks per CU; the
+// register budget is set to match.
+template <int P>
+constexpr int simplex_min_waves() {
+    return P <= 3 ? 5 : 1;
+}
+
+// Tetrahedral elements of order P, one launch per order: a block takes EPB
+// consecutive device elements of that order, TPE threads per element, in
+// phases separated by barriers:
+//  1. the stage input (all four fields), v and g_r = sum_d (J^{-1})_{r,d} s_d
+//     per basis function, the faces' normals, area ratios, neighbours and
+//     orientation data, and the order's derivative tables into LDS;
+//  2. threads of a row: the volume terms;
+//  3. items (face, column): n.s and v at the face columns (the basis
+//     functions whose trace does not vanish there) of the element and, with
+//     the element's normal, of each neighbour;
+//  4. items (face, m): the chi_m-coefficients of both traces in the element's
+//     face coordinates (the neighbour's through the orientation that maps
+//     its face coordinates onto ours) and of the central / wall flux;
+//  5. threads of a row: the lift of the flux coefficients.
+// Every face term is a polynomial of degree <= 4 in the face coordinates and
+// chi spans P_4, so the coefficients represent it exactly; only chi_m of
+// degree <= p reach the lift (chi_m of higher degree are orthogonal to the
+// element's traces), so a face of an order-p element carries (p+1)(p+2)/2
+// coefficients. T = double: FP64; T = float: the volume contractions and the
+// lifts in FP32 on FP32 tables, compensated per-row sums, traces and flux
+// coefficients in FP64.
 template <int P, typename T>
-__device__ void simplex_acoustic_block(const SimplexParams& p, const int32_t* tix, int64_t first, int count,
-                                       double* lds) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(simplex_min_waves<P>())))
+void simplex_acoustic_rhs(const SimplexParams p) {
     using S = SimplexShape<P>;
     constexpr int NB = S::NB;
+    constexpr int NC = S::NCOL;
+    __shared__ __align__(16) double lds[S::TOTAL];  // every region starts at an even double: double2-aligned
+    __shared__ int32_t tix[kTixMax];  // table offsets, orientations, face column lists
+    for (int k = threadIdx.x; k < p.n_tix; k += kBlock) tix[k] = p.tix[k];
+
+    const int64_t first = p.group_start[P - 1] + xcd_block(blockIdx.x, gridDim.x) * S::EPB;
+    const int64_t left = p.group_start[P] - first;
+    const int count = left < S::EPB ? int(left) : S::EPB;
     const int tid = threadIdx.x;
     const int le = tid / S::TPE;
     const int i = tid % S::TPE;
@@ -188,53 +212,60 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, const int32_t* ti
     const bool row_ok = elem_ok && (i < NB);
     const int64_t E = p.n_elem;
     const int64_t base = p.group_dof[P - 1] + (e - p.group_start[P - 1]) * 4 * NB;
+    double* fm = lds + S::FM + le * 16;
+    int2* fi4 = reinterpret_cast<int2*>(lds + S::FI) + le * 4;
     double* su = lds + S::U + le * 4 * NB;
-    double* sn = lds + S::N + le * 16 * kMaxCols;
-    double* sf = lds + S::F + le * 8 * kFaceQ;
-    T* sDr = reinterpret_cast<T*>(lds + S::W);
-    const int32_t* tx = tix + (P - 1) * kTixStride;
+    T* sg = reinterpret_cast<T*>(lds + S::G) + le * 4 * NB;
+    T* sDr = reinterpret_cast<T*>(lds + S::DR);
     const T* tabT = std::is_same<T, float>::value ? reinterpret_cast<const T*>(p.tab32)
                                                   : reinterpret_cast<const T*>(p.tab);
 
-    // ---- phase 1: geometry, stage input, the order's derivative tables
-    double Ji[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // ---- phase 1: geometry, stage input, v and g, the derivative tables
     double cinv = 1.0;
-    int32_t nbf[4] = {-1, -1, -1, -1}, inf[4] = {0, 0, 0, 0};
     if (elem_ok) {
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            nbf[f] = p.nbr[f * E + e];
-            inf[f] = p.info[f * E + e];
-        }
-#pragma unroll
-        for (int q = 0; q < 9; ++q) Ji[q] = p.jinv[q * E + e];
         cinv = p.cinv[e];
+        if (i < 16) {
+            const int f = i / 4, c = i % 4;
+            fm[i] = c < 3 ? p.normal[(c * 4 + f) * E + e] : p.sigma[f * E + e];
+        }
+        if (i < 4) fi4[i] = make_int2(p.nbr[i * E + e], p.info[i * E + e]);
     }
     if (row_ok) {
+        double Ji[9];
 #pragma unroll
-        for (int f = 0; f < 4; ++f) su[f * NB + i] = combine(p.io, base + f * NB + i);
+        for (int q = 0; q < 9; ++q) Ji[q] = p.jinv[q * E + e];
+        double sv[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            sv[f] = combine(p.io, base + f * NB + i);
+            su[f * NB + i] = sv[f];
+        }
+        sg[i] = T(sv[0]);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) sg[(1 + r) * NB + i] = T(Ji[r * 3] * sv[1] + Ji[r * 3 + 1] * sv[2] + Ji[r * 3 + 2] * sv[3]);
     }
-    {
-        const T* src = tabT + tx[kTixDrT];
+    const T* gDr = tabT + p.tix[(P - 1) * kTixStride + kTixDrT];
+    if constexpr (S::DR_LDS) {
 #pragma unroll 4
-        for (int k = tid; k < 3 * NB * NB; k += kBlock) sDr[k] = src[k];
+        for (int k = tid; k < 3 * NB * NB; k += kBlock) sDr[k] = gDr[k];
     }
     __syncthreads();
 
-    // ---- phase 2a: volume, k_v = sum_r Dr_r g_r ; k_s_d = (1/c_inv) sum_r (J^{-1})_{r,d} Dr_r v,
-    // g_r = sum_d (J^{-1})_{r,d} s_d formed on the fly
+    // ---- phase 2: volume, k_v = sum_r Dr_r g_r ; k_s_d = (1/c_inv) sum_r (J^{-1})_{r,d} Dr_r v
     Acc<T> kv, ks[3];
     if (row_ok) {
+        T Jt[9];  // J^{-1} again (not kept in registers across the barrier)
+#pragma unroll
+        for (int q = 0; q < 9; ++q) Jt[q] = T(p.jinv[q * E + e]);
         T a[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
-#pragma unroll 2
+        const T* Dr = S::DR_LDS ? sDr : gDr;
+#pragma unroll 4
         for (int j = 0; j < NB; ++j) {
-            const double s0 = su[NB + j], s1 = su[2 * NB + j], s2 = su[3 * NB + j];
-            const T v = T(su[j]);
+            const T v = sg[j];
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
-                const T g = T(Ji[r * 3] * s0 + Ji[r * 3 + 1] * s1 + Ji[r * 3 + 2] * s2);
-                const T d = sDr[(r * NB + j) * NB + i];
-                a[r] += d * g;
+                const T d = Dr[(r * NB + j) * NB + i];
+                a[r] += d * sg[(1 + r) * NB + j];
                 b[r] += d * v;
             }
         }
@@ -242,100 +273,105 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, const int32_t* ti
         const T ci = T(1.0 / cinv);
 #pragma unroll
         for (int d = 0; d < 3; ++d)
-            ks[d] = Acc<T>((T(Ji[d]) * b[0] + T(Ji[3 + d]) * b[1] + T(Ji[6 + d]) * b[2]) * ci);
+            ks[d] = Acc<T>((Jt[d] * b[0] + Jt[3 + d] * b[1] + Jt[6 + d] * b[2]) * ci);
     }
-    __syncthreads();  // the derivative tables give way to the face data
+    __syncthreads();  // the volume data give way to the face data
 
-    // ---- the neighbours' face columns
+    // ---- phase 3: n.s and v at the face columns, own items (f, c < NC)
+    // first, then each neighbour's (f, c < ncol(p_n))
+    double2* so = reinterpret_cast<double2*>(lds + S::SO) + le * 4 * NC;
+    double2* sn = reinterpret_cast<double2*>(lds + S::SN) + le * 4 * kMaxCols;
     if (elem_ok) {
-        // items (face, field, column): 4 x 4 x kMaxCols
-#pragma unroll 4
-        for (int it = i; it < 16 * kMaxCols; it += S::TPE) {
-            const int f = it / (4 * kMaxCols), fld = (it / kMaxCols) % 4, c = it % kMaxCols;
-            const int32_t nb = f == 0 ? nbf[0] : f == 1 ? nbf[1] : f == 2 ? nbf[2] : nbf[3];
-            const int fi = f == 0 ? inf[0] : f == 1 ? inf[1] : f == 2 ? inf[2] : inf[3];
-            const int pn = info_pn(fi);
-            if (nb < 0 || c >= simplex_ncol(pn)) continue;
-            const int nbn = simplex_nb(pn);
-            const int col = tix[tix[(pn - 1) * kTixStride + kTixCols + info_lf(fi)] + c];
-            const int64_t nbase = p.group_dof[pn - 1] + (nb - p.group_start[pn - 1]) * 4 * nbn;
-            sn[(f * 4 + fld) * kMaxCols + c] = combine(p.io, nbase + fld * nbn + col);
-        }
-    }
-    __syncthreads();
-
-    // ---- phase 2b: fluxes at the face points, items (face, point)
-    if (elem_ok) {
-#pragma unroll 1
-        for (int it = i; it < 4 * kFaceQ; it += S::TPE) {
-            const int f = it / kFaceQ, q = it % kFaceQ;
-            const int32_t nb = f == 0 ? nbf[0] : f == 1 ? nbf[1] : f == 2 ? nbf[2] : nbf[3];
-            const int fi = f == 0 ? inf[0] : f == 1 ? inf[1] : f == 2 ? inf[2] : inf[3];
-            const double n0 = p.normal[(0 * 4 + f) * E + e], n1 = p.normal[(1 * 4 + f) * E + e],
-                         n2 = p.normal[(2 * 4 + f) * E + e];
-            const double* Vo = p.tab + tx[kTixVf + f] + (info_oown(fi) * S::NCOL) * kFaceQ + q;
-            const int32_t* colso = tix + tx[kTixCols + f];
-            double o4[4] = {0, 0, 0, 0};
-#pragma unroll 3
-            for (int c = 0; c < S::NCOL; ++c) {
-                const double phi = Vo[c * kFaceQ];
-                const int col = colso[c];
+        int nstart[5];
+        nstart[0] = 4 * NC;
 #pragma unroll
-                for (int fld = 0; fld < 4; ++fld) o4[fld] += phi * su[fld * NB + col];
-            }
-            double Jv, Js;
-            if (nb >= 0) {
-                double t4[4];
-                const double* cols4 = sn + f * 4 * kMaxCols;
-                const int lfn = info_lf(fi), on = info_onbr(fi);
-                switch (info_pn(fi)) {
-                    case 1: nbr_traces<1>(p, tix, lfn, on, q, cols4, t4); break;
-                    case 2: nbr_traces<2>(p, tix, lfn, on, q, cols4, t4); break;
-                    case 3: nbr_traces<3>(p, tix, lfn, on, q, cols4, t4); break;
-                    default: nbr_traces<4>(p, tix, lfn, on, q, cols4, t4); break;
-                }
-                Jv = -0.5 * (n0 * (o4[1] - t4[1]) + n1 * (o4[2] - t4[2]) + n2 * (o4[3] - t4[3]));
-                Js = -0.5 * (o4[0] - t4[0]);
+        for (int f = 0; f < 4; ++f) {
+            const int2 nf = fi4[f];
+            nstart[f + 1] = nstart[f] + (nf.x >= 0 ? simplex_ncol(info_pn(nf.y)) : 0);
+        }
+#pragma unroll 1
+        for (int it = i; it < nstart[4]; it += S::TPE) {
+            if (it < 4 * NC) {
+                const int f = it / NC, c = it % NC;
+                const int col = tix[tix[(P - 1) * kTixStride + kTixCols + f] + c];
+                so[f * NC + c] = make_double2(fm[f * 4] * su[NB + col] + fm[f * 4 + 1] * su[2 * NB + col] +
+                                                  fm[f * 4 + 2] * su[3 * NB + col],
+                                              su[col]);
             } else {
-                Jv = -(n0 * o4[1] + n1 * o4[2] + n2 * o4[3]);
-                Js = 0;
+                const int f = it < nstart[2] ? (it < nstart[1] ? 0 : 1) : (it < nstart[3] ? 2 : 3);
+                const int c = it - nstart[f];
+                const int2 nf = fi4[f];
+                const int pn = info_pn(nf.y);
+                const int nbn = simplex_nb(pn);
+                const int col = tix[tix[(pn - 1) * kTixStride + kTixCols + info_lf(nf.y)] + c];
+                const int64_t nbase = p.group_dof[pn - 1] + (nf.x - p.group_start[pn - 1]) * 4 * nbn + col;
+                double w[4];
+#pragma unroll
+                for (int fld = 0; fld < 4; ++fld) w[fld] = combine(p.io, nbase + fld * nbn);
+                sn[f * kMaxCols + c] =
+                    make_double2(fm[f * 4] * w[1] + fm[f * 4 + 1] * w[2] + fm[f * 4 + 2] * w[3], w[0]);
             }
-            const double w = p.face_w[q];
-            sf[(f * 2) * kFaceQ + q] = w * Jv;
-            sf[(f * 2 + 1) * kFaceQ + q] = w * Js;
         }
     }
     __syncthreads();
 
-    // ---- phase 3: the order's lift rows into LDS (over U and N, no longer
-    // needed), then k += sigma_f sum_q (M_ref^{-1} phi(point o(q)))_i flux(q)
-    T* sL = reinterpret_cast<T*>(lds + S::U);
-    {
-        const T* src = tabT + tx[kTixLf0];
-#pragma unroll 4
-        for (int k = tid; k < 4 * kFaceQ * NB; k += kBlock) sL[k] = src[k];
+    // ---- phase 4: flux coefficients, items (f, m < NC)
+    FluxPair<T>* sf = reinterpret_cast<FluxPair<T>*>(lds + S::SF) + le * 4 * NC;
+    if (elem_ok) {
+        const int ident = tix[kTixIdent];
+        const int32_t* tx = tix + (P - 1) * kTixStride;
+#pragma unroll 1
+        for (int it = i; it < 4 * NC; it += S::TPE) {
+            const int f = it / NC, m = it % NC;
+            const int2 nf = fi4[f];
+            double a_ns, a_v;
+            dot_pairs<NC>(p.tab + tx[kTixProj] + ((f * 6 + ident) * 15 + m) * NC, so + f * NC, a_ns, a_v);
+            double Fv, Fs;
+            if (nf.x >= 0) {
+                const int pn = info_pn(nf.y);
+                const int o = tix[kTixRel + info_oown(nf.y) * 6 + info_onbr(nf.y)];
+                const double2* xn = sn + f * kMaxCols;
+                const int lfn = info_lf(nf.y);
+                const double* proj = p.tab + tix[(pn - 1) * kTixStride + kTixProj];
+                double b_ns, b_v;
+                switch (pn) {
+                    case 1: dot_pairs<3>(proj + ((lfn * 6 + o) * 15 + m) * 3, xn, b_ns, b_v); break;
+                    case 2: dot_pairs<6>(proj + ((lfn * 6 + o) * 15 + m) * 6, xn, b_ns, b_v); break;
+                    case 3: dot_pairs<10>(proj + ((lfn * 6 + o) * 15 + m) * 10, xn, b_ns, b_v); break;
+                    default: dot_pairs<15>(proj + ((lfn * 6 + o) * 15 + m) * 15, xn, b_ns, b_v); break;
+                }
+                Fv = -0.5 * (a_ns - b_ns);
+                Fs = -0.5 * (a_v - b_v);
+            } else {
+                Fv = -a_ns;
+                Fs = 0;
+            }
+            sf[f * NC + m] = FluxPair<T>{T(Fv), T(Fs)};
+        }
     }
     __syncthreads();
+
+    // ---- phase 5: k += sigma_f sum_m (M_ref^{-1} int_f phi chi_m)_i F_m
     if (row_ok) {
         const T ci = T(1.0 / cinv);
-        const int32_t* perm = tix + kTixPerm;
+        const T* lift = tabT + tix[(P - 1) * kTixStride + kTixLift] + i;
 #pragma unroll 1
         for (int f = 0; f < 4; ++f) {
-            const int32_t* pf = perm + info_oown(inf[f]) * kFaceQ;
-            const T* Lf = sL + f * kFaceQ * NB + i;
+            const FluxPair<T>* fl = sf + f * NC;
             T lv = T(0), ls = T(0);
-#pragma unroll 8
-            for (int q = 0; q < kFaceQ; ++q) {
-                const T l = Lf[pf[q] * NB];
-                lv += l * T(sf[(f * 2) * kFaceQ + q]);
-                ls += l * T(sf[(f * 2 + 1) * kFaceQ + q]);
+#pragma unroll 5
+            for (int m = 0; m < NC; ++m) {
+                const T l = lift[(f * NC + m) * NB];
+                const FluxPair<T> x = fl[m];
+                lv += l * x.v;
+                ls += l * x.s;
             }
-            const T sig = T(p.sigma[f * E + e]);
+            const T sig = T(fm[f * 4 + 3]);
             kv.add(sig * lv);
             const T c = sig * ls * ci;
-            ks[0].add(c * T(p.normal[(0 * 4 + f) * E + e]));
-            ks[1].add(c * T(p.normal[(1 * 4 + f) * E + e]));
-            ks[2].add(c * T(p.normal[(2 * 4 + f) * E + e]));
+            ks[0].add(c * T(fm[f * 4]));
+            ks[1].add(c * T(fm[f * 4 + 1]));
+            ks[2].add(c * T(fm[f * 4 + 2]));
         }
         const double vals[4] = {double(kv.value()), double(ks[0].value()), double(ks[1].value()),
                                 double(ks[2].value())};
@@ -345,26 +381,6 @@ __device__ void simplex_acoustic_block(const SimplexParams& p, const int32_t* ti
             if (p.io.out_f32) static_cast<float*>(p.io.out)[o] = float(vals[f]);
             else static_cast<double*>(p.io.out)[o] = vals[f];
         }
-    }
-}
-
-// Blocks follow a schedule built on the host: the mesh is cut into slabs of
-// consecutive elements and, slab after slab, each order's elements of the
-// slab fill blocks of that order. With the XCD renumbering, an XCD works
-// through neighbouring slabs, so an element's neighbours of every order are
-// read while they are still in its L2.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void simplex_acoustic_rhs(const SimplexParams p) {
-    __shared__ double lds[lds_doubles()];
-    __shared__ int32_t s_tix[kTixMax];  // table offsets and face column lists, read in every phase
-    for (int k = threadIdx.x; k < p.n_tix; k += kBlock) s_tix[k] = p.tix[k];
-    const int4 bl = p.blocks[xcd_block(blockIdx.x, gridDim.x)];
-    __syncthreads();
-    switch (bl.x) {
-        case 1: simplex_acoustic_block<1, T>(p, s_tix, bl.y, bl.z, lds); break;
-        case 2: simplex_acoustic_block<2, T>(p, s_tix, bl.y, bl.z, lds); break;
-        case 3: simplex_acoustic_block<3, T>(p, s_tix, bl.y, bl.z, lds); break;
-        default: simplex_acoustic_block<4, T>(p, s_tix, bl.y, bl.z, lds); break;
     }
 }
 
@@ -594,9 +610,9 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     c->n_faces = n_total;
     c->n_internal = n_internal;
 
-    // reference tables of every order (see the kernel): DrT, M_ref, and per
-    // local face and orientation o the traces of the face columns at the
-    // permuted face rule points and the lift rows M_ref^{-1} phi(point)
+    // reference tables of every order (see the kernel): DrT, M_ref, the face
+    // columns, the chi-coefficients of the face columns' traces per local face
+    // and orientation, and the lift in the element's own face coordinates
     int perm[6][kFaceQ];  // face point q after orientation o of the reference triangle
     for (int o = 0; o < 6; ++o)
         for (int qq = 0; qq < kFaceQ; ++qq) {
@@ -609,16 +625,41 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
             require(hit >= 0, HGDG_ERR_ARG, "face rule is not symmetric under the orientation maps");
             perm[o][qq] = hit;
         }
+    // Both sides of a face reach a physical point through their orientation:
+    // own coordinates o_own(t), neighbour's o_nbr(t). Seen from the element,
+    // the neighbour's face coordinates are o_nbr(o_own^{-1}(s)), itself one
+    // of the six orientation maps (they form a group), identified here by
+    // its action on the (symmetric) face rule points.
+    std::vector<int32_t> tix(size_t(kTixIdent) + 1, 0);
+    int ident = -1;
+    for (int o = 0; o < 6 && ident < 0; ++o) {
+        bool id = true;
+        for (int qq = 0; qq < kFaceQ; ++qq) id = id && perm[o][qq] == qq;
+        if (id) ident = o;
+    }
+    require(ident >= 0, HGDG_ERR_ARG, "no identity among the face orientation maps");
+    tix[kTixIdent] = ident;
+    for (int oo = 0; oo < 6; ++oo) {
+        int inv[kFaceQ];
+        for (int qq = 0; qq < kFaceQ; ++qq) inv[perm[oo][qq]] = qq;
+        for (int on = 0; on < 6; ++on) {
+            int rel = -1;
+            for (int o = 0; o < 6 && rel < 0; ++o) {
+                bool same = true;
+                for (int r = 0; r < kFaceQ; ++r) same = same && perm[o][r] == perm[on][inv[r]];
+                if (same) rel = o;
+            }
+            require(rel >= 0, HGDG_ERR_ARG, "face orientation maps do not compose");
+            tix[size_t(kTixRel) + oo * 6 + on] = rel;
+        }
+    }
     sm.ops.clear();
     for (int q = 1; q <= kMaxSimplexOrder; ++q) sm.ops.push_back(simplex_ops(q));
     std::vector<double> tab;
-    std::vector<int32_t> tix(size_t(kTixPerm) + 6 * kFaceQ, 0);
-    for (int o = 0; o < 6; ++o)
-        for (int qq = 0; qq < kFaceQ; ++qq) tix[size_t(kTixPerm) + o * kFaceQ + qq] = perm[o][qq];
     std::vector<int32_t> cols;
     for (int q = 1; q <= kMaxSimplexOrder; ++q) {
         const SimplexOps& ops = sm.ops[q - 1];
-        const int nb = ops.nb;
+        const int nb = ops.nb, nc = simplex_ncol(q);
         int32_t* t = tix.data() + (q - 1) * kTixStride;
         t[kTixDrT] = int32_t(tab.size());
         for (int r = 0; r < 3; ++r)
@@ -626,83 +667,27 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
                 for (int i = 0; i < nb; ++i) tab.push_back(ops.Dr[(size_t(r) * nb + i) * nb + j]);
         t[kTixM] = int32_t(tab.size());
         tab.insert(tab.end(), ops.Mref.begin(), ops.Mref.end());
-        const std::vector<long double> Minv = inverse_ld(ops.Mref, nb);
         for (int f = 0; f < 4; ++f) {
-            std::vector<long double> phi(size_t(kFaceQ) * nb), grad(size_t(nb) * 3);
-            for (int qq = 0; qq < kFaceQ; ++qq) {
-                const long double tp[2] = {kFaceRulePts[qq][0], kFaceRulePts[qq][1]};
-                long double x[3];
-                tet_face_map(f, tp, x);
-                tet_basis(q, x, &phi[size_t(qq) * nb], grad.data());
-            }
-            require(int(ops.cols[f].size()) == simplex_ncol(q), HGDG_ERR_ARG, "unexpected face column count");
-            t[kTixVf + f] = int32_t(tab.size());
-            for (int o = 0; o < 6; ++o)
-                for (int col : ops.cols[f])
-                    for (int qq = 0; qq < kFaceQ; ++qq) tab.push_back(double(phi[size_t(perm[o][qq]) * nb + col]));
-            t[kTixLf + f] = int32_t(tab.size());
-            for (int o = 0; o < 6; ++o)
-                for (int qq = 0; qq < kFaceQ; ++qq)
-                    for (int i = 0; i < nb; ++i) {
-                        long double sum = 0;
-                        for (int col : ops.cols[f])
-                            sum += Minv[size_t(i) * nb + col] * phi[size_t(perm[o][qq]) * nb + col];
-                        tab.push_back(double(sum));
-                    }
+            require(int(ops.cols[f].size()) == nc, HGDG_ERR_ARG, "unexpected face column count");
             t[kTixCols + f] = int32_t(cols.size());  // relative, fixed below
             cols.insert(cols.end(), ops.cols[f].begin(), ops.cols[f].end());
         }
-        // the lift rows at the unpermuted points, all four faces: [f][q][i]
-        t[kTixLf0] = int32_t(tab.size());
-        for (int f = 0; f < 4; ++f) {
-            std::vector<long double> phi(size_t(kFaceQ) * nb), grad(size_t(nb) * 3);
-            for (int qq = 0; qq < kFaceQ; ++qq) {
-                const long double tp[2] = {kFaceRulePts[qq][0], kFaceRulePts[qq][1]};
-                long double x[3];
-                tet_face_map(f, tp, x);
-                tet_basis(q, x, &phi[size_t(qq) * nb], grad.data());
+        t[kTixProj] = int32_t(tab.size());  // [f][o][m < kFaceBasis][c < nc]
+        for (int f = 0; f < 4; ++f)
+            for (int o = 0; o < 6; ++o) {
+                require(ops.proj[f][o].size() == size_t(kFaceBasis) * nc, HGDG_ERR_ARG, "face trace table size");
+                tab.insert(tab.end(), ops.proj[f][o].begin(), ops.proj[f][o].end());
             }
-            for (int qq = 0; qq < kFaceQ; ++qq)
-                for (int i = 0; i < nb; ++i) {
-                    long double sum = 0;
-                    for (int col : ops.cols[f]) sum += Minv[size_t(i) * nb + col] * phi[size_t(qq) * nb + col];
-                    tab.push_back(double(sum));
-                }
-        }
+        t[kTixLift] = int32_t(tab.size());  // [f][m < nc][i]: only chi_m of degree <= q meet the element's traces
+        for (int f = 0; f < 4; ++f)
+            for (int m = 0; m < nc; ++m)
+                for (int i = 0; i < nb; ++i) tab.push_back(ops.lift[f][ident][size_t(i) * kFaceBasis + m]);
     }
     const int32_t col_base = int32_t(tix.size());
     for (int q = 0; q < kMaxSimplexOrder; ++q)
         for (int f = 0; f < 4; ++f) tix[q * kTixStride + kTixCols + f] += col_base;
     tix.insert(tix.end(), cols.begin(), cols.end());
     require(tix.size() <= size_t(kTixMax), HGDG_ERR_ARG, "table index larger than its LDS copy");
-
-    // block schedule: slabs of consecutive mesh elements, each order's
-    // elements of a slab in blocks of that order's size
-    {
-        const int64_t slab = std::max<int64_t>(1024, (ne + 255) / 256);
-        static const int epb[4] = {SimplexShape<1>::EPB, SimplexShape<2>::EPB, SimplexShape<3>::EPB,
-                                   SimplexShape<4>::EPB};
-        std::vector<int32_t> blocks;
-        int64_t cursor[4] = {sm.group_start[0], sm.group_start[1], sm.group_start[2], sm.group_start[3]};
-        for (int64_t lo = 0; lo < ne; lo += slab) {
-            const int64_t hi = std::min(ne, lo + slab);
-            for (int g = 0; g < 4; ++g) {
-                // device elements of order g+1 with mesh index below hi (mesh order inside a group)
-                int64_t end = cursor[g];
-                while (end < sm.group_start[g + 1] && sm.mesh_of[end] < hi) ++end;
-                for (int64_t b = cursor[g]; b < end; b += epb[g]) {
-                    blocks.push_back(g + 1);
-                    blocks.push_back(int32_t(b));
-                    blocks.push_back(int32_t(std::min<int64_t>(epb[g], end - b)));
-                    blocks.push_back(0);
-                }
-                cursor[g] = end;
-            }
-        }
-        sm.n_blocks = int64_t(blocks.size() / 4);
-        sm.d_blocks.upload(blocks, c->stream);
-        HGDG_HIP(hipStreamSynchronize(c->stream));  // `blocks` goes out of scope
-    }
 
     sm.d_tables.upload(tab, c->stream);
     const std::vector<float> tab32(tab.begin(), tab.end());  // lives until the synchronize below
@@ -753,16 +738,31 @@ void simplex_rhs(hgdg_ctx* c, const StageIO& io) {
     p.tab32 = sm.d_tables32.ptr;
     p.tix = sm.d_tab_index.ptr;
     p.n_tix = int32_t(sm.d_tab_index.n);
-    p.blocks = reinterpret_cast<const int4*>(sm.d_blocks.ptr);
-    for (int q = 0; q < kFaceQ; ++q) p.face_w[q] = kFaceRuleW[q];
-    const int grid = int(sm.n_blocks);
-    if (grid > 0) {
-        if (c->op.precision == HGDG_PREC_MIXED_FP32)
-            hipLaunchKernelGGL(simplex_acoustic_rhs<float>, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
-        else
-            hipLaunchKernelGGL(simplex_acoustic_rhs<double>, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
+    const bool mixed = c->op.precision == HGDG_PREC_MIXED_FP32;
+    // one launch per order, each sized to its own LDS carve-up
+    for (int g = 0; g < 4; ++g) {
+        const int grid = p.group_block[g + 1] - p.group_block[g];
+        if (grid == 0) continue;
+        switch (g + 1) {
+            case 1:
+                if (mixed) hipLaunchKernelGGL((simplex_acoustic_rhs<1, float>), dim3(grid), dim3(kBlock), 0, c->stream, p);
+                else hipLaunchKernelGGL((simplex_acoustic_rhs<1, double>), dim3(grid), dim3(kBlock), 0, c->stream, p);
+                break;
+            case 2:
+                if (mixed) hipLaunchKernelGGL((simplex_acoustic_rhs<2, float>), dim3(grid), dim3(kBlock), 0, c->stream, p);
+                else hipLaunchKernelGGL((simplex_acoustic_rhs<2, double>), dim3(grid), dim3(kBlock), 0, c->stream, p);
+                break;
+            case 3:
+                if (mixed) hipLaunchKernelGGL((simplex_acoustic_rhs<3, float>), dim3(grid), dim3(kBlock), 0, c->stream, p);
+                else hipLaunchKernelGGL((simplex_acoustic_rhs<3, double>), dim3(grid), dim3(kBlock), 0, c->stream, p);
+                break;
+            default:
+                if (mixed) hipLaunchKernelGGL((simplex_acoustic_rhs<4, float>), dim3(grid), dim3(kBlock), 0, c->stream, p);
+                else hipLaunchKernelGGL((simplex_acoustic_rhs<4, double>), dim3(grid), dim3(kBlock), 0, c->stream, p);
+                break;
+        }
+        HGDG_HIP(hipGetLastError());
     }
-    HGDG_HIP(hipGetLastError());
 }
 
 void simplex_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, double* max_abs) {
