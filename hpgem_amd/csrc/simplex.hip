@@ -115,8 +115,9 @@ struct SimplexParams {
 
 __host__ __device__ constexpr int simplex_nb(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
 __host__ __device__ constexpr int simplex_ncol(int p) { return (p + 1) * (p + 2) / 2; }
-// threads per element: 32 up to order 3 (a wavefront holds two elements), nb at order 4
-__host__ __device__ constexpr int simplex_tpe(int p) { return simplex_nb(p) > 32 ? simplex_nb(p) : 32; }
+// threads per element: 16 at order 1 (four elements per wavefront), 32 at
+// orders 2 and 3 (two), nb at order 4
+__host__ __device__ constexpr int simplex_tpe(int p) { return simplex_nb(p) > 32 ? simplex_nb(p) : p == 1 ? 16 : 32; }
 
 template <int P>
 struct SimplexShape {

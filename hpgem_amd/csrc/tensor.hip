@@ -382,9 +382,197 @@ struct AcousticParams {
     const double* Mi;
 };
 
+// Four lanes per pencil, one per field (v, s_1 .. s_3), so a lane carries
+// one field's N1 values. Both equations then
+// run the same instructions:
+//   volume: acc = sum_r scale_r (Dr along r) src_r, with src_r = g_r =
+//     sum_d (J^{-1})_{r,d} s_d and scale_r = 1 on the v lane, src_r = v and
+//     scale_r = (J^{-1})_{r,d} on the s_d lane;
+//   faces: each lane forms its own field's jump; the v lane gets
+//     sum_k w_k [s_k] by a reduction over the pencil's four lanes, the s
+//     lanes get [v] from the v lane, both by lane shuffles within the quad.
+template <int DIM, int N1>
+struct AcousticShape {
+    static constexpr int NF = DIM + 1;
+    static constexpr int NP = Pencil<DIM, N1>::NP;
+    static constexpr int NB = Pencil<DIM, N1>::NB;
+    static constexpr int TPE = DIM == 3 ? 4 * NP : NP;  // threads per element
+    static constexpr int EPB = kBlock / TPE;
+};
+
 template <int DIM, int N1, typename T>
-__device__ __forceinline__ void acoustic_block(const AcousticParams& p, int64_t blk) {
+__device__ __forceinline__ void acoustic_block_quad(const AcousticParams& p, int64_t blk) {
+    using S = AcousticShape<DIM, N1>;
+    constexpr int NF = S::NF;
+    constexpr int NB = S::NB;
+    __shared__ double s_w[S::EPB * NF * NB];      // FP64 stage input
+    __shared__ T s_g[S::EPB * (DIM + 1) * NB];    // v, then g_r, in T
+    __shared__ double s_J[S::EPB * DIM * DIM];    // J^{-1}
+    __shared__ T s_D[N1 * N1];
+    __shared__ T s_Mi[2 * N1];
+    const int tid = threadIdx.x;
+    if (tid < N1 * N1) s_D[tid] = T(p.Dr[tid]);
+    if (tid < 2 * N1) s_Mi[tid] = T(p.Mi[(tid % N1) * N1 + tid / N1]);
+    const int le = tid / S::TPE;
+    const int pc = (tid % S::TPE) / 4;
+    const int fld = tid % 4;
+    const int64_t e = blk * S::EPB + le;
+    const bool active = (le < S::EPB) && (e < p.n_elem);
+    const bool lane = active && fld < NF;
+    const int ib = pc % N1;
+    const int ic = DIM == 3 ? pc / N1 : 0;
+    double* sw = s_w + le * NF * NB;
+    T* sg = s_g + le * (DIM + 1) * NB;
+    double* sJ = s_J + le * DIM * DIM;
+    const int64_t base = e * NF * NB;
+    const int d = fld - 1;  // s lanes: the component
+
+    // ---- stage input, v and g_r into LDS
+    double w[N1];
+#pragma unroll
+    for (int a = 0; a < N1; ++a) w[a] = 0.0;
+    if (lane) {
+        combine_pencil<N1>(p.io, base + fld * NB + pc * N1, w);
+#pragma unroll
+        for (int a = 0; a < N1; ++a) sw[fld * NB + pc * N1 + a] = w[a];
+        if (pc == 0 && fld == 0)
+#pragma unroll
+            for (int k = 0; k < DIM * DIM; ++k) sJ[k] = p.jinv[k * p.n_elem + e];
+    }
+    {
+        // lane 1 + r of the quad forms g_r at the pencil's nodes, lane 0 writes v
+        double Jr[DIM];
+        const int r = fld - 1;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) Jr[k] = (active && r >= 0 && r < DIM) ? p.jinv[(r * DIM + k) * p.n_elem + e] : 0.0;
+#pragma unroll
+        for (int a = 0; a < N1; ++a) {
+            double g = 0;
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) g += Jr[k] * __shfl(w[a], 1 + k, 4);
+            if (lane) sg[(r + 1) * NB + pc * N1 + a] = fld == 0 ? T(w[a]) : T(g);
+        }
+    }
+    const double cinv = active ? p.cinv[e] : 1.0;
+    __syncthreads();
+
+    // ---- volume
+    Acc<T> acc[N1];
+    if (lane) {
+#pragma unroll
+        for (int a = 0; a < N1; ++a) acc[a] = Acc<T>(T(0));
+        T vol[N1];
+#pragma unroll
+        for (int a = 0; a < N1; ++a) vol[a] = T(0);
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) {
+            const T* src = fld == 0 ? sg + (1 + r) * NB : sg;
+            const T scale = fld == 0 ? T(1) : T(sJ[r * DIM + d]);
+            T t[N1];
+            if (r == 0) {  // along this pencil
+#pragma unroll
+                for (int a = 0; a < N1; ++a) {
+                    T sv = T(0);
+#pragma unroll
+                    for (int j = 0; j < N1; ++j) sv += s_D[a * N1 + j] * src[pc * N1 + j];
+                    t[a] = sv;
+                }
+            } else {  // across the element's other pencils
+                const int my = r == 1 ? ib : ic;
+#pragma unroll
+                for (int a = 0; a < N1; ++a) t[a] = T(0);
+#pragma unroll
+                for (int j = 0; j < N1; ++j) {
+                    const int opc = r == 1 ? j + N1 * ic : ib + N1 * j;
+                    const T dr = s_D[my * N1 + j];
+#pragma unroll
+                    for (int a = 0; a < N1; ++a) t[a] += dr * src[opc * N1 + a];
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < N1; ++a) vol[a] += scale * t[a];
+        }
+#pragma unroll
+        for (int a = 0; a < N1; ++a) acc[a] = Acc<T>(vol[a]);
+    }
+
+    // ---- faces: x faces one value per pencil, y/z faces a whole pencil
+#pragma unroll 1
+    for (int f = 0; f < 2 * DIM; ++f) {
+        const int df = f / 2, s = f % 2;
+        const int64_t nb = active ? p.nbr[f * p.n_elem + e] : -1;
+        const bool inner = nb >= 0;
+        const int own_pc = df == 0 ? pc : df == 1 ? s + N1 * ic : ib + N1 * s;
+        const int nbr_pc = df == 0 ? pc : df == 1 ? (1 - s) + N1 * ic : ib + N1 * (1 - s);
+        const int na = df == 0 ? 1 : N1;       // trace values per pencil
+        const int off = df == 0 ? s : 0;       // own node of an x face
+        const int noff = df == 0 ? 1 - s : 0;  // neighbour's node
+        double jmp[N1];
+#pragma unroll
+        for (int a = 0; a < N1; ++a) jmp[a] = 0.0;
+        if (lane) {
+            double oth[N1];
+#pragma unroll
+            for (int a = 0; a < N1; ++a) oth[a] = 0.0;
+            if (inner) {
+                const int64_t lnb = nb - blk * S::EPB;
+                if (lnb >= 0 && lnb < S::EPB) {
+#pragma unroll
+                    for (int a = 0; a < N1; ++a)
+                        if (a < na) oth[a] = s_w[(lnb * NF + fld) * NB + nbr_pc * N1 + noff + a];
+                } else if (df == 0) {
+                    oth[0] = combine(p.io, nb * NF * NB + fld * NB + nbr_pc * N1 + noff);
+                } else {
+                    combine_pencil<N1>(p.io, nb * NF * NB + fld * NB + nbr_pc * N1, oth);
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < N1; ++a)
+                if (a < na) jmp[a] = sw[fld * NB + own_pc * N1 + off + a] - (inner ? oth[a] : 0.0);
+        }
+        const double wd = (2 * s - 1) * (lane && fld > 0 ? sJ[df * DIM + d] : 0.0);  // w_d on the s_d lane
+        const int my = df == 1 ? ib : ic;
+#pragma unroll
+        for (int a = 0; a < N1; ++a) {
+            if (a >= na) break;
+            // v lane: sum_k w_k [s_k] over the quad; s lanes: [v] from the v lane
+            double x = wd * jmp[a];
+            x += __shfl_xor(x, 1, 4);
+            x += __shfl_xor(x, 2, 4);
+            const double jv = __shfl(jmp[a], 0, 4);
+            double c;
+            if (fld == 0) c = inner ? -0.5 * x : -x;
+            else c = inner ? wd * (-0.5 * jv) : 0.0;
+            if (lane) {
+                if (df == 0) {
+#pragma unroll
+                    for (int b = 0; b < N1; ++b) acc[b].add(s_Mi[s * N1 + b] * T(c));
+                } else {
+                    acc[a].add(s_Mi[s * N1 + my] * T(c));
+                }
+            }
+        }
+    }
+    if (!lane) return;
+    const T ci = T(1.0 / cinv);
+    const int64_t o = base + fld * NB + pc * N1;
+    if (p.io.out_f32) {
+        float* out = static_cast<float*>(p.io.out);
+#pragma unroll
+        for (int a = 0; a < N1; ++a) out[o + a] = float(fld == 0 ? acc[a].value() : ci * acc[a].value());
+    } else {
+        double* out = static_cast<double*>(p.io.out);
+#pragma unroll
+        for (int a = 0; a < N1; ++a) out[o + a] = double(fld == 0 ? acc[a].value() : ci * acc[a].value());
+    }
+}
+
+// In 2D (three fields, little work per pencil) one thread per pencil does
+// all fields; the quad layout pays off from 3D on.
+template <int DIM, int N1, typename T>
+__device__ __forceinline__ void acoustic_block_pencil(const AcousticParams& p, int64_t blk) {
     using S = Pencil<DIM, N1>;
+    static_assert(AcousticShape<DIM, N1>::EPB == S::EPB, "launch geometry");
     constexpr int NF = DIM + 1;
     constexpr int NB = S::NB;
     __shared__ double s_w[S::EPB * NF * NB];
@@ -578,6 +766,12 @@ __device__ __forceinline__ void acoustic_block(const AcousticParams& p, int64_t 
             for (int d = 0; d < DIM; ++d) out[o + (1 + d) * NB + a] = double(ci * as[d][a].value());
         }
     }
+}
+
+template <int DIM, int N1, typename T>
+__device__ __forceinline__ void acoustic_block(const AcousticParams& p, int64_t blk) {
+    if constexpr (DIM == 3) acoustic_block_quad<DIM, N1, T>(p, blk);
+    else acoustic_block_pencil<DIM, N1, T>(p, blk);
 }
 
 template <int DIM, int N1, typename T>
@@ -1274,7 +1468,7 @@ void tensor_rhs(hgdg_ctx* c, const StageIO& io) {
         a.Mi = Mi_ptr(c);
         dispatch_tensor(c->dim, c->tm.n1, [&](auto D, auto N) {
             constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
-            using S = Pencil<DIMc, N1c>;
+            using S = AcousticShape<DIMc, N1c>;
             const int64_t grid = (c->n_elem + S::EPB - 1) / S::EPB;
             if (mixed)
                 hipLaunchKernelGGL((tensor_acoustic_rhs<DIMc, N1c, float>), dim3(unsigned(grid)), dim3(kBlock), 0,
@@ -1421,8 +1615,10 @@ size_t tensor_batch_param_bytes(const hgdg_ctx* c) {
 int64_t tensor_batch_blocks(const hgdg_ctx* c) {
     int64_t blocks = 0;
     dispatch_tensor(c->dim, c->tm.n1, [&](auto D, auto N) {
-        using S = Pencil<decltype(D)::value, decltype(N)::value>;
-        blocks = (c->n_elem + S::EPB - 1) / S::EPB;
+        constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
+        const int64_t epb = c->op.kind == HGDG_OP_ACOUSTIC_CENTRAL ? AcousticShape<DIMc, N1c>::EPB
+                                                                   : Pencil<DIMc, N1c>::EPB;
+        blocks = (c->n_elem + epb - 1) / epb;
     });
     return blocks;
 }
