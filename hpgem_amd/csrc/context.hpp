@@ -40,6 +40,8 @@ struct SimplexMesh {
     std::vector<SimplexOps> ops;                 // per order (index p-1), built on demand
     DevBuf<double> d_tables;                     // Dr, proj, lift of every order
     DevBuf<float> d_tables32;                    // the same in FP32 (mixed path)
+    DevBuf<int32_t> d_blocks;                    // RHS block schedule: [n_blocks][4] order, first, count, 0
+    int64_t n_blocks = 0;
     DevBuf<int32_t> d_tab_index;                 // offsets and column lists into d_tables
     DevBuf<int64_t> d_elem_off;                  // [E + 1] first DOF of each device element
     DevBuf<double> d_geo;                        // J^{-1} [9][E], then c_inv [E], sigma [4][E], normal [3][4][E]

@@ -111,6 +111,7 @@ struct SimplexParams {
     const float* tab32;    // the same tables in FP32 (mixed path), same offsets
     const int32_t* tix;
     int32_t n_tix;
+    const int4* blocks;    // per block: order, first device element, element count, unused
 };
 
 __host__ __device__ constexpr int simplex_nb(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
@@ -164,13 +165,6 @@ __device__ __forceinline__ void dot_pairs(const double* row, const double2* x, d
     }
 }
 
-// Up to order 3 the LDS carve-up leaves room for five blocks per CU; the
-// register budget is set to match.
-template <int P>
-constexpr int simplex_min_waves() {
-    return P <= 3 ? 5 : 1;
-}
-
 // Tetrahedral elements of order P, one launch per order: a block takes EPB
 // consecutive device elements of that order, TPE threads per element, in
 // phases separated by barriers:
@@ -193,18 +187,11 @@ constexpr int simplex_min_waves() {
 // lifts in FP32 on FP32 tables, compensated per-row sums, traces and flux
 // coefficients in FP64.
 template <int P, typename T>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(simplex_min_waves<P>())))
-void simplex_acoustic_rhs(const SimplexParams p) {
+__device__ __forceinline__ void simplex_block(const SimplexParams& p, const int32_t* tix, int64_t first, int count,
+                                              double* lds) {
     using S = SimplexShape<P>;
     constexpr int NB = S::NB;
     constexpr int NC = S::NCOL;
-    __shared__ __align__(16) double lds[S::TOTAL];  // every region starts at an even double: double2-aligned
-    __shared__ int32_t tix[kTixMax];  // table offsets, orientations, face column lists
-    for (int k = threadIdx.x; k < p.n_tix; k += kBlock) tix[k] = p.tix[k];
-
-    const int64_t first = p.group_start[P - 1] + xcd_block(blockIdx.x, gridDim.x) * S::EPB;
-    const int64_t left = p.group_start[P] - first;
-    const int count = left < S::EPB ? int(left) : S::EPB;
     const int tid = threadIdx.x;
     const int le = tid / S::TPE;
     const int i = tid % S::TPE;
@@ -382,6 +369,35 @@ void simplex_acoustic_rhs(const SimplexParams p) {
             if (p.io.out_f32) static_cast<float*>(p.io.out)[o] = float(vals[f]);
             else static_cast<double*>(p.io.out)[o] = vals[f];
         }
+    }
+}
+
+constexpr int simplex_lds_doubles() {
+    int m = SimplexShape<1>::TOTAL;
+    m = SimplexShape<2>::TOTAL > m ? SimplexShape<2>::TOTAL : m;
+    m = SimplexShape<3>::TOTAL > m ? SimplexShape<3>::TOTAL : m;
+    m = SimplexShape<4>::TOTAL > m ? SimplexShape<4>::TOTAL : m;
+    return m;
+}
+
+// All orders in one launch. Blocks follow a schedule built on the host: the
+// mesh is cut into slabs of consecutive elements and, slab after slab, each
+// order's elements of the slab fill blocks of that order. With the XCD
+// renumbering an XCD works through neighbouring slabs, so the face columns an
+// element gathers from neighbours of every order are read while they are
+// still in that XCD's L2. The LDS carve-up of the largest order leaves room
+// for five blocks per CU; the register budget is set to match.
+template <typename T>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void simplex_acoustic_rhs(const SimplexParams p) {
+    __shared__ __align__(16) double lds[simplex_lds_doubles()];  // regions start at even doubles: double2-aligned
+    __shared__ int32_t tix[kTixMax];                              // table offsets, orientations, face column lists
+    for (int k = threadIdx.x; k < p.n_tix; k += kBlock) tix[k] = p.tix[k];
+    const int4 bl = p.blocks[xcd_block(blockIdx.x, gridDim.x)];
+    switch (bl.x) {  // the same order for the whole block
+        case 1: simplex_block<1, T>(p, tix, bl.y, bl.z, lds); break;
+        case 2: simplex_block<2, T>(p, tix, bl.y, bl.z, lds); break;
+        case 3: simplex_block<3, T>(p, tix, bl.y, bl.z, lds); break;
+        default: simplex_block<4, T>(p, tix, bl.y, bl.z, lds); break;
     }
 }
 
@@ -690,6 +706,34 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     tix.insert(tix.end(), cols.begin(), cols.end());
     require(tix.size() <= size_t(kTixMax), HGDG_ERR_ARG, "table index larger than its LDS copy");
 
+    // block schedule: slabs of consecutive mesh elements, each order's
+    // elements of a slab in blocks of that order's size
+    {
+        const int64_t slab = std::max<int64_t>(1024, (ne + 255) / 256);
+        static const int epb[4] = {SimplexShape<1>::EPB, SimplexShape<2>::EPB, SimplexShape<3>::EPB,
+                                   SimplexShape<4>::EPB};
+        std::vector<int32_t> blocks;
+        int64_t cursor[4] = {sm.group_start[0], sm.group_start[1], sm.group_start[2], sm.group_start[3]};
+        for (int64_t lo = 0; lo < ne; lo += slab) {
+            const int64_t hi = std::min(ne, lo + slab);
+            for (int g = 0; g < 4; ++g) {
+                // device elements of order g+1 with mesh index below hi (mesh order inside a group)
+                int64_t end = cursor[g];
+                while (end < sm.group_start[g + 1] && sm.mesh_of[end] < hi) ++end;
+                for (int64_t b = cursor[g]; b < end; b += epb[g]) {
+                    blocks.push_back(g + 1);
+                    blocks.push_back(int32_t(b));
+                    blocks.push_back(int32_t(std::min<int64_t>(epb[g], end - b)));
+                    blocks.push_back(0);
+                }
+                cursor[g] = end;
+            }
+        }
+        sm.n_blocks = int64_t(blocks.size() / 4);
+        sm.d_blocks.upload(blocks, c->stream);
+        HGDG_HIP(hipStreamSynchronize(c->stream));  // `blocks` goes out of scope
+    }
+
     sm.d_tables.upload(tab, c->stream);
     const std::vector<float> tab32(tab.begin(), tab.end());  // lives until the synchronize below
     sm.d_tables32.upload(tab32, c->stream);
@@ -739,31 +783,15 @@ void simplex_rhs(hgdg_ctx* c, const StageIO& io) {
     p.tab32 = sm.d_tables32.ptr;
     p.tix = sm.d_tab_index.ptr;
     p.n_tix = int32_t(sm.d_tab_index.n);
-    const bool mixed = c->op.precision == HGDG_PREC_MIXED_FP32;
-    // one launch per order, each sized to its own LDS carve-up
-    for (int g = 0; g < 4; ++g) {
-        const int grid = p.group_block[g + 1] - p.group_block[g];
-        if (grid == 0) continue;
-        switch (g + 1) {
-            case 1:
-                if (mixed) hipLaunchKernelGGL((simplex_acoustic_rhs<1, float>), dim3(grid), dim3(kBlock), 0, c->stream, p);
-                else hipLaunchKernelGGL((simplex_acoustic_rhs<1, double>), dim3(grid), dim3(kBlock), 0, c->stream, p);
-                break;
-            case 2:
-                if (mixed) hipLaunchKernelGGL((simplex_acoustic_rhs<2, float>), dim3(grid), dim3(kBlock), 0, c->stream, p);
-                else hipLaunchKernelGGL((simplex_acoustic_rhs<2, double>), dim3(grid), dim3(kBlock), 0, c->stream, p);
-                break;
-            case 3:
-                if (mixed) hipLaunchKernelGGL((simplex_acoustic_rhs<3, float>), dim3(grid), dim3(kBlock), 0, c->stream, p);
-                else hipLaunchKernelGGL((simplex_acoustic_rhs<3, double>), dim3(grid), dim3(kBlock), 0, c->stream, p);
-                break;
-            default:
-                if (mixed) hipLaunchKernelGGL((simplex_acoustic_rhs<4, float>), dim3(grid), dim3(kBlock), 0, c->stream, p);
-                else hipLaunchKernelGGL((simplex_acoustic_rhs<4, double>), dim3(grid), dim3(kBlock), 0, c->stream, p);
-                break;
-        }
-        HGDG_HIP(hipGetLastError());
+    p.blocks = reinterpret_cast<const int4*>(sm.d_blocks.ptr);
+    const int grid = int(sm.n_blocks);
+    if (grid > 0) {
+        if (c->op.precision == HGDG_PREC_MIXED_FP32)
+            hipLaunchKernelGGL(simplex_acoustic_rhs<float>, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
+        else
+            hipLaunchKernelGGL(simplex_acoustic_rhs<double>, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
     }
+    HGDG_HIP(hipGetLastError());
 }
 
 void simplex_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, double* max_abs) {
