@@ -167,9 +167,10 @@ def run_gpu(steps, warmup):
             "copy_gbs": copy_gbs if prec == "fp64" else None,
             "host_wall_s": wall,
         }
-        tiles, tile_ms = api.getAdvectionTiles()
-        results[prec]["tiles_per_workgroup"] = tiles
-        results[prec]["tile_candidates_ms"] = dict(zip(("1", "2", "4", "8"), (float(x) for x in tile_ms)))
+        shape, shape_ms = api.getAdvectionTiles()
+        results[prec]["advection_launch"] = f"{shape} tile(s) per workgroup"
+        results[prec]["launch_candidates_ms"] = dict(zip(("tiles1", "tiles2", "tiles4", "tiles8"),
+                                                         (float(x) for x in shape_ms)))
         n_dof, n_elem, n_faces = api.n_dof, api.n_elements, api.n_faces
         if prec == "fp64":
             ref_api = api

@@ -505,10 +505,16 @@ int hgdg_set_advection_tiles(hgdg_ctx* ctx, int tiles) {
         require(tiles == 0 || tiles == 1 || tiles == 2 || tiles == 4 || tiles == 8, HGDG_ERR_ARG,
                 "tiles must be 0 (timed choice), 1, 2, 4 or 8");
         HGDG_HIP(hipSetDevice(ctx->device));
+        const int before = ctx->adv_tiles_fixed;
         ctx->adv_tiles_fixed = tiles;
         if (ctx->has_op && !ctx->simplex()) {
             ctx->drop_graph();
-            hgdg::tensor_tune(ctx);
+            try {
+                hgdg::tensor_tune(ctx);
+            } catch (...) {
+                ctx->adv_tiles_fixed = before;  // keep the previous choice in force
+                throw;
+            }
         }
     });
 }

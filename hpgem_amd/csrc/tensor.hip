@@ -150,26 +150,23 @@ __device__ __forceinline__ bool advection_load(const RhsParams& p, int64_t blk, 
     return active;
 }
 
-// Everything after the block's stage input is in LDS: this thread's pencil w
-// (also in registers), the block's elements in s_w.
-template <int DIM, int N1, typename T>
-__device__ __forceinline__ void advection_finish(const RhsParams& p, int64_t blk, const double (&w)[N1],
-                                                 const ElemMeta<DIM>& meta, const double* s_w, const T* s_D,
-                                                 const T* s_Mi) {
+// The arithmetic of one pencil once the block's stage input is in LDS (sw:
+// this element's coefficients; w: this thread's pencil, also in registers):
+// volume, upwind face jumps in FP64, lift, store. The neighbours' traces come
+// from xo(s) (x face s, this pencil's value) and yz(d, s, f, nbr_pc, other)
+// (y/z face: the neighbour's pencil nbr_pc), called only for faces whose mode
+// needs the neighbour; every kernel variant shares this code, so they agree
+// bit for bit.
+template <int DIM, int N1, typename T, typename XO, typename YZ>
+__device__ __forceinline__ void advection_compute(const RhsParams& p, int64_t e, int pc, const double (&w)[N1],
+                                                  uint32_t md, const double* ahd, const double* sw, const T* s_D,
+                                                  const T* s_Mi, XO xo, YZ yz) {
     using S = Pencil<DIM, N1>;
-    const int tid = threadIdx.x;
-    const int le = tid / S::NP;
-    const int pc = tid % S::NP;
-    const int64_t e = blk * S::EPB + le;
-    const bool active = (le < S::EPB) && (e < p.n_elem);
     const int ib = pc % N1;                    // y index of this pencil
     const int ic = DIM == 3 ? pc / N1 : 0;     // z index of this pencil
-    const double* sw = s_w + le * S::NB;
-    if (!active) return;
-
     T ah[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) ah[d] = T(meta.ah[d]);
+    for (int d = 0; d < DIM; ++d) ah[d] = T(ahd[d]);
 
     // volume (strong form): -sum_d ahat_d (Dr along d) w
     T vol[N1];
@@ -201,26 +198,11 @@ __device__ __forceinline__ void advection_finish(const RhsParams& p, int64_t blk
     for (int a = 0; a < N1; ++a) acc[a] = Acc<T>(vol[a]);
 
     // faces: -(2s-1) ahat_d Mi(idx_d, s) (T_{d,s} - own_{d,s}); jumps in FP64
-    const uint32_t md = meta.md;
 #pragma unroll
     for (int s = 0; s < 2; ++s) {  // x faces: one trace value per pencil
         const uint32_t m = (md >> (3 * s)) & 7u;
         if (m == MODE_OWN) continue;
-        double other = 0.0;
-        if (m == MODE_NBR || m == MODE_AVG) {
-            const int64_t nb = meta.nb[s];
-            const int64_t lnb = nb - blk * S::EPB;
-            if (lnb >= 0 && lnb < S::EPB) {
-                other = s_w[lnb * S::NB + pc * N1 + (1 - s)];  // neighbour in this block
-            } else {
-                const int64_t idx = nb * S::NB + pc * N1 + (1 - s);
-                other = p.io.coef[0] * load_in(p.io, 0, idx);
-                for (int i = 1; i < kMaxIn; ++i) {
-                    if (i >= p.io.n_in) break;
-                    other += p.io.coef[i] * load_in(p.io, i, idx);
-                }
-            }
-        }
+        const double other = (m == MODE_NBR || m == MODE_AVG) ? xo(s) : 0.0;
         const T jump = T(face_jump(m, w[s], other));
         const T sc = (s ? T(-1) : T(1)) * ah[0] * jump;
 #pragma unroll
@@ -237,14 +219,7 @@ __device__ __forceinline__ void advection_finish(const RhsParams& p, int64_t blk
             const int nbr_pc = d == 1 ? (1 - s) + N1 * ic : ib + N1 * (1 - s);
             double other[N1];
             if (m == MODE_NBR || m == MODE_AVG) {
-                const int64_t nb = meta.nb[f];
-                const int64_t lnb = nb - blk * S::EPB;
-                if (lnb >= 0 && lnb < S::EPB) {
-#pragma unroll
-                    for (int a = 0; a < N1; ++a) other[a] = s_w[lnb * S::NB + nbr_pc * N1 + a];
-                } else {
-                    combine_pencil<N1>(p.io, nb * S::NB + nbr_pc * N1, other);
-                }
+                yz(d, s, f, nbr_pc, other);
             } else {
 #pragma unroll
                 for (int a = 0; a < N1; ++a) other[a] = 0.0;
@@ -265,6 +240,38 @@ __device__ __forceinline__ void advection_finish(const RhsParams& p, int64_t blk
 #pragma unroll
         for (int a = 0; a < N1; ++a) out[a] = double(acc[a].value());
     }
+}
+
+// General meshes: neighbour ids from the per-element face data; a neighbour
+// inside the block is read from LDS, any other from HBM.
+template <int DIM, int N1, typename T>
+__device__ __forceinline__ void advection_finish(const RhsParams& p, int64_t blk, const double (&w)[N1],
+                                                 const ElemMeta<DIM>& meta, const double* s_w, const T* s_D,
+                                                 const T* s_Mi) {
+    using S = Pencil<DIM, N1>;
+    const int tid = threadIdx.x;
+    const int le = tid / S::NP;
+    const int pc = tid % S::NP;
+    const int64_t e = blk * S::EPB + le;
+    const bool active = (le < S::EPB) && (e < p.n_elem);
+    if (!active) return;
+    auto xo = [&](int s) -> double {
+        const int64_t nb = meta.nb[s];
+        const int64_t lnb = nb - blk * S::EPB;
+        if (lnb >= 0 && lnb < S::EPB) return s_w[lnb * S::NB + pc * N1 + (1 - s)];  // neighbour in this block
+        return combine(p.io, nb * S::NB + pc * N1 + (1 - s));
+    };
+    auto yz = [&](int, int, int f, int nbr_pc, double (&other)[N1]) {
+        const int64_t nb = meta.nb[f];
+        const int64_t lnb = nb - blk * S::EPB;
+        if (lnb >= 0 && lnb < S::EPB) {
+#pragma unroll
+            for (int a = 0; a < N1; ++a) other[a] = s_w[lnb * S::NB + nbr_pc * N1 + a];
+        } else {
+            combine_pencil<N1>(p.io, nb * S::NB + nbr_pc * N1, other);
+        }
+    };
+    advection_compute<DIM, N1, T>(p, e, pc, w, meta.md, meta.ah, s_w + le * S::NB, s_D, s_Mi, xo, yz);
 }
 
 template <int DIM, int N1, typename T>
@@ -1430,10 +1437,11 @@ void tensor_tune(hgdg_ctx* c) {
     HGDG_HIP(hipEventCreate(&ev[1]));
     constexpr int kCand[4] = {1, 2, 4, 8};
     constexpr int kReps = 20;
+    constexpr int n_cand = 4;
     double best = 0;
     int best_tiles = 1;
     for (int pass = 0; pass < 2; ++pass)
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < n_cand; ++i) {
             for (int r = 0; r < 3; ++r) launch_advection(c, p, kCand[i]);
             HGDG_HIP(hipEventRecord(ev[0], c->stream));
             for (int r = 0; r < kReps; ++r) launch_advection(c, p, kCand[i]);
@@ -1444,7 +1452,7 @@ void tensor_tune(hgdg_ctx* c) {
             const double t = double(ms) / kReps;
             if (pass == 0 || t < c->adv_tile_ms[i]) c->adv_tile_ms[i] = t;
         }
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < n_cand; ++i)
         if (i == 0 || c->adv_tile_ms[i] < best) {
             best = c->adv_tile_ms[i];
             best_tiles = kCand[i];

@@ -254,6 +254,9 @@ def test_mixed_precision_within_1e6_of_fp64(n, p, a):
     ([16, 16], 2, "fp64", True),
     ([5, 4, 3], 2, "fp64", False),       # 60 elements: a ragged last tile, boundary faces
     ([7, 5], 4, "mixed", False),
+    ([5, 4, 3], 2, "fp64", True),        # periodic, ragged last tile
+    ([3, 3, 3], 3, "mixed", True),       # every neighbour inside one or two blocks
+    ([24, 16, 8], 3, "fp64", True),      # non-cubic box
 ])
 def test_tile_counts_give_bitwise_identical_rhs(n, p, precision, periodic):
     """Every launch shape of the advection kernel (one tile per workgroup, or
@@ -272,17 +275,18 @@ def test_tile_counts_give_bitwise_identical_rhs(n, p, precision, periodic):
     dev.setTimeIntegrationVector(0, rng.standard_normal(dev.n_dof))
     dev.setTimeIntegrationVector(1, rng.standard_normal(dev.n_dof))
     outs = {}
-    for t in (1, 2, 4, 8):
+    shapes = (1, 2, 4, 8)
+    for t in shapes:
         dev.setAdvectionTiles(t)
         assert dev.getAdvectionTiles()[0] == t
         dev.setTimeIntegrationVector(2, np.full(dev.n_dof, np.nan))
         dev.computeTimeDerivative([0, 1], [1.0, 0.25], 2)
         outs[t] = dev.getTimeIntegrationVector(2)
     assert np.all(np.isfinite(outs[1]))
-    for t in (2, 4, 8):
+    for t in shapes[1:]:
         np.testing.assert_array_equal(outs[t], outs[1])
     dev.setAdvectionTiles(0)
-    assert dev.getAdvectionTiles()[0] in (1, 2, 4, 8)
+    assert dev.getAdvectionTiles()[0] in shapes
     dev.close()
 
 
