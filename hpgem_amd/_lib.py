@@ -143,6 +143,8 @@ class MeshFileInfo(ctypes.Structure):
 SIGNATURES += [
     ("hgdg_mesh_read", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(MeshFileInfo), _D, _I64, _I32, _I64, _I32,
                                       ctypes.c_char_p]),
+    ("hgdg_mesh_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int64, _D, _I64, _I32,
+                                       ctypes.c_char_p]),
 ]
 
 

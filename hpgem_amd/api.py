@@ -54,6 +54,18 @@ class Mesh:
         mesh.version, mesh.n_nodes, mesh.n_edges = info.version, info.n_nodes, info.n_edges
         return mesh
 
+    def write(self, path):
+        """Write a .hpgem file, version 2, one partition (the Preprocessor's
+        outputMesh, applications/Preprocessor/lib/output_impl.h:91-208)."""
+        lib = _lib.load()
+        coords = np.ascontiguousarray(self.coords, dtype=np.float64)
+        nodes = np.ascontiguousarray(self.node_ids, dtype=np.int64)
+        zones = None if self.zones is None else np.ascontiguousarray(self.zones, dtype=np.int32)
+        names = "\n".join(self.zone_names).encode() if self.zone_names else None
+        check(lib.hgdg_mesh_write(str(path).encode(), int(self.shape), int(self.n_elements), dptr(coords),
+                                  nodes.ctypes.data_as(_lib._I64),
+                                  None if zones is None else zones.ctypes.data_as(_lib._I32), names))
+
     @classmethod
     def structured(cls, n, lower=None, upper=None, periodic=True, simplices=False):
         """hpGEM Preprocessor structured mesh (applications/Preprocessor/lib/hpgem.cpp:100-235)."""

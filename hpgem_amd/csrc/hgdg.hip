@@ -6,6 +6,8 @@
 // entry points of include/hgdg.h. The right-hand-side kernels live in
 // tensor.hip (squares, cubes) and simplex.hip (tetrahedra).
 #include <algorithm>
+#include <fstream>
+#include <sstream>
 #include <cmath>
 #include <functional>
 #include <limits>
@@ -827,6 +829,42 @@ int hgdg_mesh_read(const char* path, hgdg_mesh_file_info* info, double* coords, 
         if (face_elements) std::copy(m.face_elements.begin(), m.face_elements.end(), face_elements);
         if (face_local) std::copy(m.face_local.begin(), m.face_local.end(), face_local);
         if (zone_names) std::copy(names.c_str(), names.c_str() + names.size() + 1, zone_names);
+    });
+}
+
+int hgdg_mesh_write(const char* path, int shape, int64_t n_elements, const double* coords, const int64_t* node_ids,
+                    const int32_t* zones, const char* zone_names) {
+    return run([&] {
+        require(path && coords && node_ids && n_elements >= 0, HGDG_ERR_ARG, "NULL argument or negative count");
+        hgdg::MeshFile m;
+        static const int dims[8] = {0, 1, 2, 2, 3, 3, 3, 3}, verts[8] = {0, 2, 4, 3, 8, 4, 6, 5};
+        require(shape >= 1 && shape <= 7, HGDG_ERR_ARG, "unknown shape");
+        m.shape = shape;
+        m.dim = dims[shape];
+        m.n_vertices = verts[shape];
+        m.n_elements = n_elements;
+        const size_t nv = size_t(n_elements) * m.n_vertices;
+        m.coords.assign(coords, coords + nv * m.dim);
+        m.node_ids.assign(node_ids, node_ids + nv);
+        if (zones) m.zones.assign(zones, zones + n_elements);
+        if (zone_names && *zone_names) {
+            std::string all(zone_names), name;
+            std::istringstream in(all);
+            while (std::getline(in, name)) m.zone_names.push_back(name);
+        }
+        int32_t max_zone = -1;
+        for (int32_t z : m.zones) {
+            require(z >= 0, HGDG_ERR_ARG, "negative zone index");
+            max_zone = std::max(max_zone, z);
+        }
+        require(max_zone < int32_t(std::max<size_t>(m.zone_names.size(), 1)), HGDG_ERR_ARG,
+                "zone index without a zone name");
+        const std::string text = hgdg::format_mesh_file(m);
+        std::ofstream out(path, std::ios::binary);
+        require(bool(out), HGDG_ERR_ARG, std::string("cannot open ") + path);
+        out << text;
+        out.close();
+        require(bool(out), HGDG_ERR_ARG, std::string("cannot write ") + path);
     });
 }
 

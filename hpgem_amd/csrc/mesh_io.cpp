@@ -1,4 +1,5 @@
-// Reader of hpGEM's .hpgem mesh files (format versions 1 and 2), host only.
+// Reader and writer of hpGEM's .hpgem mesh files (format versions 1 and 2;
+// the writer produces version 2), host only.
 //
 // The format, as MeshFileInformation::readInformation
 // (kernel/Base/MeshFileInformation.cpp:48-103) and MeshManipulator::readMesh
@@ -22,8 +23,13 @@
 // it uses. Only single-partition files are accepted (one device, one rank),
 // which is what the reference requires of a run on one process
 // (MeshManipulator_Impl.h:1538-1542).
+#include <algorithm>
+#include <array>
 #include <cctype>
+#include <cstdio>
 #include <cstdlib>
+#include <map>
+#include <set>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -117,7 +123,133 @@ int faces_of(int shape) {
     }
 }
 
+// Local codim-1 and codim-2 entities of each geometry as vertex lists, in the
+// order the Preprocessor numbers them (applications/Preprocessor/lib/mesh/
+// ElementShapes.cpp:74-169): the file lists entities in order of first
+// appearance under this numbering.
+struct LocalEntities {
+    std::vector<std::vector<int>> faces, edges;  // edges: only for 3D shapes
+};
+
+LocalEntities local_entities(int shape) {
+    switch (shape) {
+        case MESH_SHAPE_LINE: return {{{0}, {1}}, {}};
+        case MESH_SHAPE_TRIANGLE: return {{{0, 1}, {0, 2}, {1, 2}}, {}};
+        case MESH_SHAPE_SQUARE: return {{{0, 1}, {0, 2}, {1, 3}, {2, 3}}, {}};
+        case MESH_SHAPE_TETRAHEDRON:
+            return {{{0, 3, 2}, {0, 1, 3}, {0, 2, 1}, {1, 2, 3}}, {{0, 1}, {0, 2}, {0, 3}, {2, 3}, {1, 3}, {1, 2}}};
+        case MESH_SHAPE_CUBE:
+            return {{{0, 1, 2, 3}, {0, 1, 4, 5}, {0, 2, 4, 6}, {1, 3, 5, 7}, {2, 3, 6, 7}, {4, 5, 6, 7}},
+                    {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}}};
+        case MESH_SHAPE_PRISM:
+            return {{{0, 2, 1}, {3, 4, 5}, {2, 0, 5, 3}, {0, 1, 3, 4}, {1, 2, 4, 5}},
+                    {{0, 1}, {0, 2}, {1, 2}, {3, 4}, {3, 5}, {4, 5}, {0, 3}, {1, 4}, {2, 5}}};
+        case MESH_SHAPE_PYRAMID:
+            return {{{3, 4, 1, 2}, {3, 1, 0}, {2, 4, 0}, {1, 2, 0}, {4, 3, 0}},
+                    {{0, 1}, {0, 2}, {0, 3}, {0, 4}, {1, 2}, {2, 4}, {4, 3}, {3, 1}}};
+        default: return {};
+    }
+}
+
+// Entities of one codimension: key = sorted node ids, value = the list of
+// (element, local index) in the order elements meet it.
+struct EntityTable {
+    std::map<std::vector<int64_t>, int64_t> id;
+    std::vector<std::vector<std::pair<int64_t, int>>> uses;
+    void add(std::vector<int64_t> key, int64_t element, int local) {
+        std::sort(key.begin(), key.end());
+        auto it = id.find(key);
+        if (it == id.end()) {
+            id.emplace(std::move(key), int64_t(uses.size()));
+            uses.push_back({{element, local}});
+        } else {
+            uses[size_t(it->second)].push_back({element, local});
+        }
+    }
+};
+
+std::string hexfloat(double v) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%a", v);
+    return buf;
+}
+
 }  // namespace
+
+std::string format_mesh_file(const MeshFile& m) {
+    const int dim = m.dim, nv = m.n_vertices;
+    const int64_t ne = m.n_elements;
+    if (shape_of(dim, nv) != m.shape || m.shape == 0) throw MeshReadError("unknown element geometry");
+    if (m.coords.size() != size_t(ne) * nv * dim || m.node_ids.size() != size_t(ne) * nv ||
+        (!m.zones.empty() && m.zones.size() != size_t(ne)))
+        throw MeshReadError("array sizes do not match the element count");
+    const std::vector<std::string> zone_names = m.zone_names.empty() ? std::vector<std::string>{"Main"} : m.zone_names;
+    int64_t n_nodes = 0;
+    for (int64_t id : m.node_ids) {
+        if (id < 0) throw MeshReadError("negative node id");
+        n_nodes = std::max(n_nodes, id + 1);
+    }
+    // distinct coordinate copies of each node, ascending (lexicographic <)
+    using Coord = std::array<double, 3>;
+    std::vector<std::set<Coord>> copies(static_cast<size_t>(n_nodes));
+    auto coord_of = [&](int64_t e, int v) {
+        Coord c{0.0, 0.0, 0.0};
+        for (int d = 0; d < dim; ++d) c[d] = m.coords[(size_t(e) * nv + v) * dim + d];
+        return c;
+    };
+    for (int64_t e = 0; e < ne; ++e)
+        for (int v = 0; v < nv; ++v) copies[size_t(m.node_ids[size_t(e) * nv + v])].insert(coord_of(e, v));
+    for (const auto& c : copies)
+        if (c.empty()) throw MeshReadError("node ids must number the nodes 0 .. n-1 without gaps");
+    // entities by first appearance
+    const LocalEntities loc = local_entities(m.shape);
+    EntityTable faces, edges;
+    for (int64_t e = 0; e < ne; ++e) {
+        const int64_t* nid = m.node_ids.data() + size_t(e) * nv;
+        for (int codim = 1; codim <= 2 && codim < dim; ++codim) {
+            const auto& list = codim == 1 ? loc.faces : loc.edges;
+            EntityTable& table = codim == 1 ? faces : edges;
+            for (size_t i = 0; i < list.size(); ++i) {
+                std::vector<int64_t> key;
+                for (int v : list[i]) key.push_back(nid[v]);
+                table.add(std::move(key), e, int(i));
+            }
+        }
+    }
+    std::ostringstream out;
+    out << "mesh 2\n" << n_nodes << " " << ne << " " << dim;
+    if (dim > 1) out << " " << faces.uses.size();
+    if (dim > 2) out << " " << edges.uses.size();
+    out << "\n1 " << n_nodes << " \n";
+    out << "zones\n" << zone_names.size() << "\n";
+    for (const auto& z : zone_names) out << z << "\n";
+    out << "nodes\n";
+    for (const auto& c : copies) {
+        out << "1 0 \n" << c.size() << " ";
+        for (const Coord& x : c)
+            for (int d = 0; d < dim; ++d) out << hexfloat(x[d]) << " ";
+        out << "\n";
+    }
+    out << "elements\n";
+    for (int64_t e = 0; e < ne; ++e) {
+        out << nv << " ";
+        for (int v = 0; v < nv; ++v) {
+            const int64_t node = m.node_ids[size_t(e) * nv + v];
+            const auto& c = copies[size_t(node)];
+            out << node << " " << std::distance(c.begin(), c.find(coord_of(e, v))) << " ";
+        }
+        out << "0 0 \n" << (m.zones.empty() ? 0 : m.zones[size_t(e)]) << "\n";
+    }
+    for (int codim = 1; codim <= 2 && codim < dim; ++codim) {
+        out << "codim" << codim << "\n";
+        for (const auto& u : (codim == 1 ? faces : edges).uses) {
+            out << u.size() << " ";
+            for (const auto& el : u) out << el.first << " " << el.second << " ";
+            out << "1 0 \n";
+        }
+    }
+    return out.str();
+}
 
 MeshFile read_mesh_file(const std::string& path) {
     std::ifstream in(path, std::ios::binary);

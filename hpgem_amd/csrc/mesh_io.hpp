@@ -38,4 +38,13 @@ struct MeshFile {
 // MeshReadError with file:line on malformed input.
 MeshFile read_mesh_file(const std::string& path);
 
+// The text of a version-2, single-partition file for a mesh given as the
+// reader returns it (coords, node_ids, zones, zone_names; shape and
+// n_vertices set; faces are not needed): nodes 0 .. n-1 with their distinct
+// coordinate copies in ascending order, elements, then the codim-1 and
+// codim-2 entities numbered by first appearance, element by element, with
+// the Preprocessor's local entity order. Throws MeshReadError on
+// inconsistent input.
+std::string format_mesh_file(const MeshFile& m);
+
 }  // namespace hgdg

@@ -341,6 +341,18 @@ typedef struct hgdg_mesh_file_info {
 int hgdg_mesh_read(const char* path, hgdg_mesh_file_info* info, double* coords, int64_t* node_ids,
                    int32_t* zones, int64_t* face_elements, int32_t* face_local, char* zone_names);
 
+/* Write a mesh as a .hpgem file (format version 2, one partition), as the
+ * Preprocessor's outputMesh does (applications/Preprocessor/lib/output_impl.h:
+ * 91-208): nodes with their distinct coordinate copies in ascending order,
+ * elements, then the codim-1 and codim-2 entities numbered by first
+ * appearance with the Preprocessor's local entity order
+ * (applications/Preprocessor/lib/mesh/ElementShapes.cpp:74-169). Arrays as
+ * hgdg_mesh_read returns them: coords [n_elements][vertices][dim], node_ids
+ * [n_elements][vertices] numbering the nodes 0 .. n-1; zones may be NULL (all
+ * in zone 0), zone_names '\n'-separated or NULL ("Main"). */
+int hgdg_mesh_write(const char* path, int shape, int64_t n_elements, const double* coords, const int64_t* node_ids,
+                    const int32_t* zones, const char* zone_names);
+
 #ifdef __cplusplus
 }
 #endif
