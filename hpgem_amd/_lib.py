@@ -145,6 +145,8 @@ SIGNATURES += [
                                       ctypes.c_char_p]),
     ("hgdg_mesh_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int64, _D, _I64, _I32,
                                        ctypes.c_char_p]),
+    ("hgdg_vtk_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int64, _D, _I32, ctypes.c_int,
+                                      ctypes.c_int, _D, ctypes.c_int64, ctypes.c_char_p]),
 ]
 
 

@@ -86,6 +86,20 @@ class Mesh:
         return cls(shape.value, coords, nodes)
 
 
+def write_vtk(base_name, mesh, orders, state, names):
+    """hpGEM's VTK output at order 1 (VTKSpecificTimeWriter,
+    kernel/Output/VTKSpecificTimeWriter_Impl.h): <base_name>.pvtu and
+    <base_name>.0.vtu with one point array per field, each element's values
+    at its own vertices. state in hpGEM's layout (getTimeIntegrationVector)."""
+    lib = _lib.load()
+    ordv = np.ascontiguousarray(np.broadcast_to(np.asarray(orders, dtype=np.int32), (mesh.n_elements,)))
+    st = np.ascontiguousarray(state, dtype=np.float64)
+    names = list(names)
+    check(lib.hgdg_vtk_write(str(base_name).encode(), int(mesh.shape), int(mesh.n_elements), dptr(mesh.coords),
+                             ordv.ctypes.data_as(_lib._I32), int(ordv[0]) if len(ordv) else 1, len(names),
+                             dptr(st), int(st.size), "\n".join(names).encode()))
+
+
 def basis_size(shape, p):
     return _lib.load().hgdg_basis_size(int(shape), int(p))
 
@@ -209,6 +223,18 @@ class HpgemAPIDevice:
         if v.size != self.n_dof:
             raise ValueError(f"expected {self.n_dof} values, got {v.size}")
         check(self.lib.hgdg_state_set(self.h, vec, dptr(v)))
+
+    def writeVTK(self, base_name, vec=None, names=None):
+        """Write time-integration vector `vec` (default: the solution) as VTK
+        (VTKSpecificTimeWriter at order 1). Fields are named u (advection) or
+        v, s_x, s_y, s_z (acoustics) unless names are given."""
+        vec = self.solutionVectorId_ if vec is None else vec
+        orders, counts = np.unique(self._order, return_counts=True)
+        nb_total = sum(basis_size(self.mesh.shape, int(p)) * int(c) for p, c in zip(orders, counts))
+        n_fields = self.n_dof // nb_total
+        if names is None:
+            names = ["u"] if n_fields == 1 else ["v", "s_x", "s_y", "s_z"][:n_fields]
+        write_vtk(base_name, self.mesh, self._order, self.getTimeIntegrationVector(vec), names)
 
     def getTimeIntegrationVector(self, vec):
         out = np.zeros(self.n_dof)

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "context.hpp"
+#include "vtk_io.hpp"
 #include "mesh_io.hpp"
 
 namespace hgdg {
@@ -829,6 +830,41 @@ int hgdg_mesh_read(const char* path, hgdg_mesh_file_info* info, double* coords, 
         if (face_elements) std::copy(m.face_elements.begin(), m.face_elements.end(), face_elements);
         if (face_local) std::copy(m.face_local.begin(), m.face_local.end(), face_local);
         if (zone_names) std::copy(names.c_str(), names.c_str() + names.size() + 1, zone_names);
+    });
+}
+
+int hgdg_vtk_write(const char* base_name, int shape, int64_t n_elements, const double* coords, const int32_t* orders,
+                   int uniform_order, int n_fields, const double* state, int64_t state_len, const char* field_names) {
+    return run([&] {
+        require(base_name && coords && state && n_elements >= 0 && n_fields >= 1, HGDG_ERR_ARG,
+                "NULL argument or bad count");
+        require(shape == hgdg::SHAPE_SQUARE || shape == hgdg::SHAPE_CUBE || shape == hgdg::SHAPE_TETRAHEDRON,
+                HGDG_ERR_UNSUPPORTED, "VTK output is available for squares, cubes and tetrahedra");
+        std::vector<int32_t> ord(orders ? orders : &uniform_order, orders ? orders + n_elements : &uniform_order + 1);
+        if (!orders) ord.assign(size_t(n_elements), uniform_order);
+        std::vector<std::string> names;
+        if (field_names && *field_names) {
+            std::istringstream in{std::string(field_names)};
+            std::string name;
+            while (std::getline(in, name)) names.push_back(name);
+        } else {
+            for (int f = 0; f < n_fields; ++f) names.push_back("field" + std::to_string(f));
+        }
+        require(int(names.size()) == n_fields, HGDG_ERR_ARG, "one field name per field");
+        int64_t need = 0;
+        for (int32_t p : ord) {
+            const int q = std::max(1, std::min(int(p), 5));
+            need += int64_t(n_fields) * (shape == hgdg::SHAPE_TETRAHEDRON ? hgdg::tet_nb(std::min(q, 4))
+                                         : shape == hgdg::SHAPE_CUBE ? (q + 1) * (q + 1) * (q + 1) : (q + 1) * (q + 1));
+        }
+        require(state_len == need, HGDG_ERR_ARG,
+                "state length " + std::to_string(state_len) + " does not match the fields and orders (" +
+                    std::to_string(need) + ")");
+        try {
+            hgdg::write_vtk(base_name, shape, n_elements, coords, ord.data(), n_fields, state, names);
+        } catch (const std::runtime_error& ex) {
+            require(false, HGDG_ERR_ARG, ex.what());
+        }
     });
 }
 

@@ -353,6 +353,19 @@ int hgdg_mesh_read(const char* path, hgdg_mesh_file_info* info, double* coords, 
 int hgdg_mesh_write(const char* path, int shape, int64_t n_elements, const double* coords, const int64_t* node_ids,
                     const int32_t* zones, const char* zone_names);
 
+/* Write a state as VTK XML (order 1), as hpGEM's VTKSpecificTimeWriter does
+ * (kernel/Output/VTKSpecificTimeWriter_Impl.h, VTKStandardElements.cpp):
+ * <base_name>.pvtu naming one piece <base_name>.0.vtu, one cell per element
+ * (VTK quad / hexahedron / tetra), the points being each element's own
+ * vertices and each field's value there, arrays base64 "binary" with a uint32
+ * byte count. coords as hgdg_mesh_desc takes them; orders per element or NULL
+ * (uniform_order); state in hpGEM's layout (as hgdg_state_get returns it),
+ * state_len doubles;
+ * field_names '\n'-separated, one per field, or NULL ("field0", ...). Host
+ * only: needs no device. */
+int hgdg_vtk_write(const char* base_name, int shape, int64_t n_elements, const double* coords, const int32_t* orders,
+                   int uniform_order, int n_fields, const double* state, int64_t state_len, const char* field_names);
+
 #ifdef __cplusplus
 }
 #endif
