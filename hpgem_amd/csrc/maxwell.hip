@@ -68,15 +68,35 @@ __device__ __forceinline__ int mx_lf(int v) { return v & 3; }
 __device__ __forceinline__ int mx_oown(int v) { return (v >> 2) & 7; }
 __device__ __forceinline__ int mx_onbr(int v) { return (v >> 5) & 7; }
 
+typedef double mx_double4 __attribute__((ext_vector_type(4)));
+
+// Face values of one side in LDS: Xt[k][i], k = 6 q + d, d < 3 the component
+// d of T = n x phi_i at face point q, d >= 3 the component d - 3 of curl
+// phi_i; i padded to 32 with zeros.
+constexpr int kMxK = 6 * kFaceQ;  // 96
+constexpr int kMxPad = 32;
+
+// B operand of the face GEMM block(i, j) = sum_k Xt[k][i] Y[j][k]: Y[j][k] =
+// ws_q (factor C_j + ls T_j)_d for k = 6q + d, d < 3, and ws_q factor T_j,d-3
+// for d >= 3 (ws_q = face weight x area scale), from the side's face values.
+__device__ __forceinline__ double mx_b(const double* xt, int k, int j, const double* wq, double factor, double ls) {
+    const int q = k / 6, d = k % 6;
+    if (d < 3) return wq[q] * (factor * xt[(6 * q + 3 + d) * kMxPad + j] + ls * xt[(6 * q + d) * kMxPad + j]);
+    return wq[q] * factor * xt[(6 * q + d - 3) * kMxPad + j];
+}
+
 __global__ __launch_bounds__(kBlock) void maxwell_assemble_kernel(const MaxwellParams p) {
     __shared__ double s_geo[2][20];
-    __shared__ double s_tc[2][kFaceQ][kMaxHcurl][6];  // side (own, neighbour): T [3], C [3]
+    __shared__ double s_x[2][kMxK * kMxPad];  // side (own, neighbour): face values, Xt layout
     __shared__ double s_face[8];
+    __shared__ double s_wq[kFaceQ];           // face weight x area scale
     const int tid = threadIdx.x;
+    const int lane = tid % 64, wave = tid / 64;
     const int64_t e = blockIdx.x;
     const int n = p.n;
     const int nn = n * n;
     if (tid < 20) s_geo[0][tid] = p.geo[e * 20 + tid];
+    for (int k = tid; k < 2 * kMxK * kMxPad; k += kBlock) (&s_x[0][0])[k] = 0.0;  // zero padding columns once
     __syncthreads();
     const double* J = s_geo[0];
     const double* Ji = s_geo[0] + 9;
@@ -95,25 +115,33 @@ __global__ __launch_bounds__(kBlock) void maxwell_assemble_kernel(const MaxwellP
             G[a * 3 + b] = g / fabs(det);
             H[a * 3 + b] = h * eps * fabs(det);
         }
-    // element matrices: diagonal block accumulator in registers (entries tid, tid + 256, ...)
-    constexpr int kPer = (kMaxHcurl * kMaxHcurl + kBlock - 1) / kBlock;
-    double diag[kPer];
+    // the permittivity mass, all threads over its entries
     const int64_t me = p.mesh_of[e];
+    for (int k = tid; k < nn; k += kBlock) {
+        double m = 0;
 #pragma unroll
-    for (int r = 0; r < kPer; ++r) {
-        const int k = tid + r * kBlock;
-        diag[r] = 0;
-        if (k < nn) {
-            double s = 0, m = 0;
-#pragma unroll
-            for (int ab = 0; ab < 9; ++ab) {
-                s += G[ab] * p.Kref[ab * nn + k];
-                m += H[ab] * p.Nref[ab * nn + k];
-            }
-            diag[r] = s;
-            p.M[me * nn + k] = m;
-        }
+        for (int ab = 0; ab < 9; ++ab) m += H[ab] * p.Nref[ab * nn + k];
+        p.M[me * nn + k] = m;
     }
+    // this wave's 16 x 16 tile of the 32 x 32 (padded) block: entries
+    // (row0 + 4 r + lane / 16, col0 + lane % 16), r = 0..3, as the MFMA holds them
+    const int tiles = (n + 15) / 16;
+    const int ti = wave / 2, tj = wave % 2;
+    const bool tile_on = ti < tiles && tj < tiles;
+    const int row0 = 16 * ti, col0 = 16 * tj;
+    const int col = col0 + lane % 16;
+    mx_double4 acc_d = {0.0, 0.0, 0.0, 0.0};  // the diagonal block: S_e, then the self face terms
+    if (tile_on)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = row0 + 4 * r + lane / 16;
+            if (row < n && col < n) {
+                double s = 0;
+#pragma unroll
+                for (int ab = 0; ab < 9; ++ab) s += G[ab] * p.Kref[ab * nn + row * n + col];
+                acc_d[r] = s;
+            }
+        }
     for (int f = 0; f < 4; ++f) {
         const int64_t nb = p.nbr[f * p.n_elem + e];
         const int inf = p.info[f * p.n_elem + e];
@@ -122,7 +150,8 @@ __global__ __launch_bounds__(kBlock) void maxwell_assemble_kernel(const MaxwellP
         if (nb >= 0 && tid < 20) s_geo[1][tid] = p.geo[nb * 20 + tid];
         __syncthreads();
         const double* nrm = s_face;
-        // T and C at the face points, own side (items: point q, function i)
+        if (tid < kFaceQ) s_wq[tid] = p.face_w[tid] * s_face[3];
+        // T and C at the face points, both sides (items: side, point q, function i)
         for (int it = tid; it < 2 * kFaceQ * n; it += kBlock) {
             const int side = it / (kFaceQ * n);
             if (side == 1 && nb < 0) continue;
@@ -139,55 +168,53 @@ __global__ __launch_bounds__(kBlock) void maxwell_assemble_kernel(const MaxwellP
                 cu[d] = (g[d * 3 + 0] * c[0] + g[d * 3 + 1] * c[1] + g[d * 3 + 2] * c[2]) / g[18];
             }
             const double sg = side == 0 ? 1.0 : -1.0;
-            double* out = s_tc[side][q][i];
-            out[0] = sg * (nrm[1] * ph[2] - nrm[2] * ph[1]);
-            out[1] = sg * (nrm[2] * ph[0] - nrm[0] * ph[2]);
-            out[2] = sg * (nrm[0] * ph[1] - nrm[1] * ph[0]);
-            out[3] = cu[0];
-            out[4] = cu[1];
-            out[5] = cu[2];
+            double* xt = s_x[side] + (6 * q) * kMxPad + i;
+            xt[0 * kMxPad] = sg * (nrm[1] * ph[2] - nrm[2] * ph[1]);
+            xt[1 * kMxPad] = sg * (nrm[2] * ph[0] - nrm[0] * ph[2]);
+            xt[2 * kMxPad] = sg * (nrm[0] * ph[1] - nrm[1] * ph[0]);
+            xt[3 * kMxPad] = cu[0];
+            xt[4 * kMxPad] = cu[1];
+            xt[5 * kMxPad] = cu[2];
         }
         __syncthreads();
-        const double wsc = s_face[3], ls = s_face[4], factor = s_face[5];
-        // self term into the diagonal block, neighbour term into its block
-#pragma unroll
-        for (int r = 0; r < kPer; ++r) {
-            const int k = tid + r * kBlock;
-            if (k >= nn) continue;
-            const int i = k / n, j = k % n;
-            double s = 0;
-            for (int q = 0; q < kFaceQ; ++q) {
-                const double* ti = s_tc[0][q][i];
-                const double* tj = s_tc[0][q][j];
-                const double ct = tj[3] * ti[0] + tj[4] * ti[1] + tj[5] * ti[2];
-                const double tc = tj[0] * ti[3] + tj[1] * ti[4] + tj[2] * ti[5];
-                const double tt = tj[0] * ti[0] + tj[1] * ti[1] + tj[2] * ti[2];
-                s += p.face_w[q] * (factor * (ct + tc) + ls * tt);
+        if (!tile_on) continue;
+        const double ls = s_face[4], factor = s_face[5];
+        // self term: acc_d += X_own^T Y_own over the 96 face values
+        {
+            const double* xa = s_x[0];
+#pragma unroll 4
+            for (int s4 = 0; s4 < kMxK / 4; ++s4) {
+                const int k = 4 * s4 + lane / 16;
+                const double a = xa[k * kMxPad + row0 + lane % 16];
+                const double b = mx_b(xa, k, col0 + lane % 16, s_wq, factor, ls);
+                acc_d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc_d, 0, 0, 0);
             }
-            diag[r] += wsc * s;
         }
-        if (nb >= 0) {
+        if (nb >= 0) {  // neighbour term: its block X_own^T Y_nbr
+            const double* xa = s_x[0];
+            const double* xb = s_x[1];
+            mx_double4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+            for (int s4 = 0; s4 < kMxK / 4; ++s4) {
+                const int k = 4 * s4 + lane / 16;
+                const double a = xa[k * kMxPad + row0 + lane % 16];
+                const double b = mx_b(xb, k, col0 + lane % 16, s_wq, factor, ls);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            }
             double* blk = p.K + p.slot[e * 5 + 1 + f] * nn;
-            for (int k = tid; k < nn; k += kBlock) {
-                const int i = k / n, j = k % n;
-                double s = 0;
-                for (int q = 0; q < kFaceQ; ++q) {
-                    const double* ti = s_tc[0][q][i];
-                    const double* tj = s_tc[1][q][j];
-                    const double ct = tj[3] * ti[0] + tj[4] * ti[1] + tj[5] * ti[2];
-                    const double tc = tj[0] * ti[3] + tj[1] * ti[4] + tj[2] * ti[5];
-                    const double tt = tj[0] * ti[0] + tj[1] * ti[1] + tj[2] * ti[2];
-                    s += p.face_w[q] * (factor * (ct + tc) + ls * tt);
-                }
-                blk[k] = wsc * s;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = row0 + 4 * r + lane / 16;
+                if (row < n && col < n) blk[row * n + col] = acc[r];
             }
         }
     }
+    if (!tile_on) return;
     double* dblk = p.K + p.slot[e * 5] * nn;
 #pragma unroll
-    for (int r = 0; r < kPer; ++r) {
-        const int k = tid + r * kBlock;
-        if (k < nn) dblk[k] = diag[r];
+    for (int r = 0; r < 4; ++r) {
+        const int row = row0 + 4 * r + lane / 16;
+        if (row < n && col < n) dblk[row * n + col] = acc_d[r];
     }
 }
 
