@@ -111,14 +111,27 @@ struct hgdg_server {
     double t_first_submit = -1, t_last_done = 0;
     double dof_rhs = 0;  // sum over completed solves of DOF x RHS evaluations
     std::thread preparer, worker;
-    // device staging for batched launch parameters
+    // device staging for batched launch parameters; they stay valid while the
+    // set of resident solves is the same (the launches of one RK step do not
+    // change from step to step)
     hgdg::DevBuf<char> d_blob;
     char* h_blob = nullptr;
     size_t h_blob_bytes = 0;
+    struct Launch {
+        int kind, dim, n1;
+        size_t params, off;
+        int n;
+        int64_t blocks;
+    };
+    std::vector<int64_t> staged;  // tickets of the solves the device blob describes (never reused, unlike addresses)
+    std::vector<Launch> launches;
+    size_t ax_items = 0, ax_off_pos = 0;
+    int64_t ax_blocks = 0;
 
     void prepare_loop();
     void worker_loop();
-    void tick(std::vector<hgdg::Solve*>& solves);
+    void stage(std::vector<hgdg::Solve*>& solves);
+    void tick(std::vector<hgdg::Solve*>& solves, int n_steps);
 };
 
 namespace hgdg {
@@ -205,14 +218,13 @@ void hgdg_server::prepare_loop() {
     }
 }
 
-void hgdg_server::tick(std::vector<hgdg::Solve*>& solves) {
+void hgdg_server::stage(std::vector<hgdg::Solve*>& solves) {
     using namespace hgdg;
     // group by kernel family
     std::map<std::array<int, 3>, std::vector<Solve*>> groups;
     for (Solve* s : solves) groups[{s->ctx->op.kind, s->ctx->dim, s->ctx->tm.n1}].push_back(s);
     // lay out the parameter blob: per stage and family [params][offsets], then the updates
-    struct Launch { int kind, dim, n1; size_t params, off; int n; int64_t blocks; };
-    std::vector<Launch> launches;
+    launches.clear();
     std::vector<char> blob;
     auto align = [&](size_t a) { blob.resize((blob.size() + a - 1) / a * a); };
     for (int stage = 0; stage < 4; ++stage) {
@@ -249,7 +261,7 @@ void hgdg_server::tick(std::vector<hgdg::Solve*>& solves) {
         }
     }
     align(64);
-    const size_t ax_items = blob.size();
+    ax_items = blob.size();
     blob.resize(blob.size() + sizeof(AxpyItem) * solves.size());
     std::vector<int64_t> ax_off(solves.size() + 1, 0);
     for (size_t i = 0; i < solves.size(); ++i) {
@@ -267,10 +279,12 @@ void hgdg_server::tick(std::vector<hgdg::Solve*>& solves) {
         ax_off[i + 1] = ax_off[i] + (it.n + kBlock - 1) / kBlock;
     }
     align(64);
-    const size_t ax_off_pos = blob.size();
+    ax_off_pos = blob.size();
+    ax_blocks = ax_off.back();
     blob.resize(blob.size() + sizeof(int64_t) * ax_off.size());
     std::memcpy(blob.data() + ax_off_pos, ax_off.data(), sizeof(int64_t) * ax_off.size());
 
+    staged.clear();  // nothing is staged while the blob is replaced
     if (d_blob.n < blob.size()) d_blob.alloc(blob.size() * 2);
     if (h_blob_bytes < blob.size()) {
         if (h_blob) (void)hipHostFree(h_blob);
@@ -279,15 +293,23 @@ void hgdg_server::tick(std::vector<hgdg::Solve*>& solves) {
     }
     std::memcpy(h_blob, blob.data(), blob.size());
     HGDG_HIP(hipMemcpyAsync(d_blob.ptr, h_blob, blob.size(), hipMemcpyHostToDevice, stream));
-    size_t li = 0;
-    for (int stage = 0; stage < 4; ++stage)
-        for (size_t g = 0; g < groups.size(); ++g, ++li) {
-            const Launch& L = launches[li];
+    for (Solve* sv : solves) staged.push_back(sv->req.ticket);
+}
+
+// n_steps RK4 steps of every resident solve: the launches of one step,
+// enqueued n_steps times, then one synchronisation.
+void hgdg_server::tick(std::vector<hgdg::Solve*>& solves, int n_steps) {
+    using namespace hgdg;
+    bool same = solves.size() == staged.size();
+    for (size_t i = 0; same && i < solves.size(); ++i) same = solves[i]->req.ticket == staged[i];
+    if (!same) stage(solves);
+    for (int step = 0; step < n_steps; ++step) {
+        for (const Launch& L : launches)
             tensor_batch_launch(stream, L.kind, L.dim, L.n1, mixed, d_blob.ptr + L.params,
                                 reinterpret_cast<const int64_t*>(d_blob.ptr + L.off), L.n, L.blocks);
-        }
-    axpy_batch_launch(stream, reinterpret_cast<const AxpyItem*>(d_blob.ptr + ax_items),
-                      reinterpret_cast<const int64_t*>(d_blob.ptr + ax_off_pos), int(solves.size()), ax_off.back());
+        axpy_batch_launch(stream, reinterpret_cast<const AxpyItem*>(d_blob.ptr + ax_items),
+                          reinterpret_cast<const int64_t*>(d_blob.ptr + ax_off_pos), int(solves.size()), ax_blocks);
+    }
     HGDG_HIP(hipStreamSynchronize(stream));
 }
 
@@ -318,15 +340,23 @@ void hgdg_server::worker_loop() {
         }
         std::string err;
         if (!running.empty()) {
+            // steps between synchronisations: more than one saves host round
+            // trips but delays retiring and joining; measured on the serving
+            // stream, the GPU is the bottleneck and one step per sync gives
+            // the lowest latencies
+            constexpr int kStepsPerSync = 1;
+            int n_steps = kStepsPerSync;
+            for (hgdg::Solve* s : running) n_steps = std::min(n_steps, int(s->req.d.n_steps - s->steps_done));
             try {
-                tick(running);
+                tick(running, n_steps);
                 for (hgdg::Solve* s : running) {
-                    ++s->steps_done;
+                    s->steps_done += n_steps;
                     if (s->steps_done >= s->req.d.n_steps) finished.push_back(s);
                 }
             } catch (const std::exception& ex) {
                 err = ex.what();
                 finished = mine;  // a failed tick ends every resident solve
+                staged.clear();
             }
         }
         for (hgdg::Solve* s : finished) {
