@@ -78,14 +78,14 @@ int64_t footprint(const hgdg_solve_desc& d, bool mixed) {
     return dof * (8 + 4 * (mixed ? 4 : 8)) + cells * (8 * (d.dim * d.dim + 2) + 4 * 2 * d.dim + 8);
 }
 
+// A solve's context runs on a stream of the server's (the preparer's while it
+// is built, the worker's once resident), which outlives it.
 void destroy_ctx(hgdg_ctx* c) {
     if (!c) return;
     (void)hipStreamSynchronize(c->stream);
     c->drop_graph();
     c->vecs.clear();
-    hipStream_t s = c->stream;
     delete c;
-    if (s) (void)hipStreamDestroy(s);
 }
 
 }  // namespace
@@ -97,7 +97,8 @@ struct hgdg_server {
     bool mixed = false;
     int64_t budget = 0;
     int max_active = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // the worker's: every resident solve's steps
+    hipStream_t prep_stream = nullptr;  // the preparer's: building solves
     std::mutex mu;
     std::condition_variable cv;
     bool stop = false;
@@ -124,6 +125,14 @@ struct hgdg_server {
         int64_t blocks;
     };
     std::vector<int64_t> staged;  // tickets of the solves the device blob describes (never reused, unlike addresses)
+    hipGraph_t step_graph = nullptr;  // one RK4 step of the staged solves, captured once per resident set
+    hipGraphExec_t step_exec = nullptr;
+    void drop_step_graph() {
+        if (step_exec) (void)hipGraphExecDestroy(step_exec);
+        if (step_graph) (void)hipGraphDestroy(step_graph);
+        step_exec = nullptr;
+        step_graph = nullptr;
+    }
     std::vector<Launch> launches;
     size_t ax_items = 0, ax_off_pos = 0;
     int64_t ax_blocks = 0;
@@ -137,11 +146,11 @@ struct hgdg_server {
 namespace hgdg {
 namespace {
 
-hgdg_ctx* build_solve(int device, bool mixed, const Request& r) {
+hgdg_ctx* build_solve(int device, bool mixed, const Request& r, hipStream_t stream) {
     const hgdg_solve_desc& d = r.d;
     std::unique_ptr<hgdg_ctx> c(new hgdg_ctx);
     c->device = device;
-    HGDG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->stream = stream;
     const int32_t n[3] = {d.cells[0], d.cells[1], d.cells[2]};
     c->shape = d.dim == 3 ? SHAPE_CUBE : SHAPE_SQUARE;
     tensor_structured_upload(c.get(), d.dim, n, d.order);
@@ -198,7 +207,7 @@ void hgdg_server::prepare_loop() {
         s->req = std::move(r);
         s->bytes = need;
         try {
-            s->ctx = hgdg::build_solve(device, mixed, s->req);
+            s->ctx = hgdg::build_solve(device, mixed, s->req, prep_stream);
         } catch (const std::exception& ex) {
             std::lock_guard<std::mutex> lk(mu);
             auto& res = results[s->req.ticket];
@@ -293,23 +302,34 @@ void hgdg_server::stage(std::vector<hgdg::Solve*>& solves) {
     }
     std::memcpy(h_blob, blob.data(), blob.size());
     HGDG_HIP(hipMemcpyAsync(d_blob.ptr, h_blob, blob.size(), hipMemcpyHostToDevice, stream));
-    for (Solve* sv : solves) staged.push_back(sv->req.ticket);
-}
-
-// n_steps RK4 steps of every resident solve: the launches of one step,
-// enqueued n_steps times, then one synchronisation.
-void hgdg_server::tick(std::vector<hgdg::Solve*>& solves, int n_steps) {
-    using namespace hgdg;
-    bool same = solves.size() == staged.size();
-    for (size_t i = 0; same && i < solves.size(); ++i) same = solves[i]->req.ticket == staged[i];
-    if (!same) stage(solves);
-    for (int step = 0; step < n_steps; ++step) {
+    // one step's launches as a graph: a tick is then one graph launch
+    drop_step_graph();
+    HGDG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
         for (const Launch& L : launches)
             tensor_batch_launch(stream, L.kind, L.dim, L.n1, mixed, d_blob.ptr + L.params,
                                 reinterpret_cast<const int64_t*>(d_blob.ptr + L.off), L.n, L.blocks);
         axpy_batch_launch(stream, reinterpret_cast<const AxpyItem*>(d_blob.ptr + ax_items),
                           reinterpret_cast<const int64_t*>(d_blob.ptr + ax_off_pos), int(solves.size()), ax_blocks);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
     }
+    HGDG_HIP(hipStreamEndCapture(stream, &step_graph));
+    HGDG_HIP(hipGraphInstantiate(&step_exec, step_graph, nullptr, nullptr, 0));
+    for (Solve* sv : solves) staged.push_back(sv->req.ticket);
+}
+
+// n_steps RK4 steps of every resident solve (the captured step replayed),
+// then one synchronisation.
+void hgdg_server::tick(std::vector<hgdg::Solve*>& solves, int n_steps) {
+    using namespace hgdg;
+    bool same = solves.size() == staged.size();
+    for (size_t i = 0; same && i < solves.size(); ++i) same = solves[i]->req.ticket == staged[i];
+    if (!same) stage(solves);
+    for (int step = 0; step < n_steps; ++step) HGDG_HIP(hipGraphLaunch(step_exec, stream));
     HGDG_HIP(hipStreamSynchronize(stream));
 }
 
@@ -326,6 +346,7 @@ void hgdg_server::worker_loop() {
                 hgdg::Solve* s = ready.front();
                 ready.pop_front();
                 s->t_start = t;
+                s->ctx->stream = stream;  // resident: the worker's stream from now on
                 results[s->req.ticket].status = 1;
                 results[s->req.ticket].t_start = t;
                 mine.push_back(s);
@@ -357,6 +378,7 @@ void hgdg_server::worker_loop() {
                 err = ex.what();
                 finished = mine;  // a failed tick ends every resident solve
                 staged.clear();
+                drop_step_graph();
             }
         }
         for (hgdg::Solve* s : finished) {
@@ -436,6 +458,7 @@ int hgdg_server_create(hgdg_server** out, int device, int precision, int64_t hbm
         s->budget = hbm_budget_bytes;
         s->max_active = max_active;
         HGDG_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        HGDG_HIP(hipStreamCreateWithFlags(&s->prep_stream, hipStreamNonBlocking));
         hgdg_server* raw = s.get();
         s->preparer = std::thread([raw] { raw->prepare_loop(); });
         s->worker = std::thread([raw] { raw->worker_loop(); });
@@ -561,9 +584,11 @@ int hgdg_server_destroy(hgdg_server* s) {
             delete r;
         }
         (void)hipSetDevice(s->device);
+        s->drop_step_graph();
         if (s->h_blob) (void)hipHostFree(s->h_blob);
         s->d_blob.reset();
         if (s->stream) (void)hipStreamDestroy(s->stream);
+        if (s->prep_stream) (void)hipStreamDestroy(s->prep_stream);
         delete s;
     });
 }
