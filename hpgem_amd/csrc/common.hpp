@@ -68,6 +68,33 @@ __device__ __forceinline__ double combine(const StageIO& p, int64_t idx) {
     return s;
 }
 
+// combine() at K indices with the same rounding, input vector by input
+// vector: the K loads of one input are independent and issue together, so a
+// thread waits once per input instead of once per value.
+template <int K>
+__device__ __forceinline__ void combine_n(const StageIO& p, const int64_t (&idx)[K], double (&s)[K]) {
+#pragma unroll
+    for (int i = 0; i < kMaxIn; ++i) {
+        if (i >= p.n_in) break;
+        double x[K];
+        if ((p.in_f32 >> i) & 1u) {
+            const float* v = static_cast<const float*>(p.in[i]);
+#pragma unroll
+            for (int k = 0; k < K; ++k) x[k] = double(v[idx[k]]);
+        } else {
+            const double* v = static_cast<const double*>(p.in[i]);
+#pragma unroll
+            for (int k = 0; k < K; ++k) x[k] = v[idx[k]];
+        }
+        const double c = p.coef[i];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (i == 0) s[k] = c * x[k];
+            else s[k] += c * x[k];
+        }
+    }
+}
+
 // Residual accumulation: FP64 plainly; FP32 with a compensated (TwoSum)
 // running sum for the face contributions.
 template <typename T>

@@ -40,7 +40,7 @@ namespace {
 constexpr int kTixDrT = 0;           // [3][nb][nb]: DrT[r][j][i] = Dr_r(i, j)
 constexpr int kTixM = 1;             // [nb][nb]: M_ref
 constexpr int kTixCols = 2;          // + f
-constexpr int kTixProj = 6;          // [4 f][6 o][15 m][ncol]: chi_m-coefficients of the face columns' traces
+constexpr int kTixProj = 6;          // [4 f][6 o][ncol][15 m]: chi_m-coefficients of the face columns' traces
 constexpr int kTixLift = 7;          // [4 f][ncol m][nb i]: (M_ref^{-1} int_f phi chi_m)_i, own face coordinates
 constexpr int kTixStride = 8;
 constexpr int kTixRel = 4 * kTixStride;  // [6 own][6 neighbour]: orientation of the neighbour's face seen from ours
@@ -151,14 +151,15 @@ struct FluxPair {
     T v, s;  // flux coefficient of the v equation, of the s equations
 };
 
-// Sum over c < NC of row[c] * x[c] for both components of x.
+// Sum over c < NC of col[15 c] * x[c] for both components of x (a column
+// of a [c][15] table: the threads of consecutive m read consecutive words).
 template <int NC>
-__device__ __forceinline__ void dot_pairs(const double* row, const double2* x, double& a, double& b) {
+__device__ __forceinline__ void dot_pairs(const double* col, const double2* x, double& a, double& b) {
     a = 0;
     b = 0;
 #pragma unroll 5
     for (int c = 0; c < NC; ++c) {
-        const double r = row[c];
+        const double r = col[c * 15];
         const double2 v = x[c];
         a += r * v.x;
         b += r * v.y;
@@ -222,12 +223,11 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
         double Ji[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) Ji[q] = p.jinv[q * E + e];
+        const int64_t at[4] = {base + i, base + NB + i, base + 2 * NB + i, base + 3 * NB + i};
         double sv[4];
+        combine_n(p.io, at, sv);
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            sv[f] = combine(p.io, base + f * NB + i);
-            su[f * NB + i] = sv[f];
-        }
+        for (int f = 0; f < 4; ++f) su[f * NB + i] = sv[f];
         sg[i] = T(sv[0]);
 #pragma unroll
         for (int r = 0; r < 3; ++r) sg[(1 + r) * NB + i] = T(Ji[r * 3] * sv[1] + Ji[r * 3 + 1] * sv[2] + Ji[r * 3 + 2] * sv[3]);
@@ -270,34 +270,58 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
     double2* so = reinterpret_cast<double2*>(lds + S::SO) + le * 4 * NC;
     double2* sn = reinterpret_cast<double2*>(lds + S::SN) + le * 4 * kMaxCols;
     if (elem_ok) {
+#pragma unroll 1
+        for (int it = i; it < 4 * NC; it += S::TPE) {
+            const int f = it / NC, c = it % NC;
+            const int col = tix[tix[(P - 1) * kTixStride + kTixCols + f] + c];
+            so[f * NC + c] = make_double2(fm[f * 4] * su[NB + col] + fm[f * 4 + 1] * su[2 * NB + col] +
+                                              fm[f * 4 + 2] * su[3 * NB + col],
+                                          su[col]);
+        }
         int nstart[5];
-        nstart[0] = 4 * NC;
+        nstart[0] = 0;
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
             const int2 nf = fi4[f];
             nstart[f + 1] = nstart[f] + (nf.x >= 0 ? simplex_ncol(info_pn(nf.y)) : 0);
         }
+        // the neighbours' items, at most 4 * kMaxCols, two per thread at a
+        // time: both items take their addresses first and load together (an
+        // item past the end reads the element's own first value and is not
+        // stored)
+        constexpr int NIT = (4 * kMaxCols + S::TPE - 1) / S::TPE;
 #pragma unroll 1
-        for (int it = i; it < nstart[4]; it += S::TPE) {
-            if (it < 4 * NC) {
-                const int f = it / NC, c = it % NC;
-                const int col = tix[tix[(P - 1) * kTixStride + kTixCols + f] + c];
-                so[f * NC + c] = make_double2(fm[f * 4] * su[NB + col] + fm[f * 4 + 1] * su[2 * NB + col] +
-                                                  fm[f * 4 + 2] * su[3 * NB + col],
-                                              su[col]);
-            } else {
-                const int f = it < nstart[2] ? (it < nstart[1] ? 0 : 1) : (it < nstart[3] ? 2 : 3);
-                const int c = it - nstart[f];
-                const int2 nf = fi4[f];
-                const int pn = info_pn(nf.y);
-                const int nbn = simplex_nb(pn);
-                const int col = tix[tix[(pn - 1) * kTixStride + kTixCols + info_lf(nf.y)] + c];
-                const int64_t nbase = p.group_dof[pn - 1] + (nf.x - p.group_start[pn - 1]) * 4 * nbn + col;
-                double w[4];
+        for (int k0 = 0; k0 < NIT && i + k0 * S::TPE < nstart[4]; k0 += 2) {
+            int64_t at[8];
+            int slot[2];
 #pragma unroll
-                for (int fld = 0; fld < 4; ++fld) w[fld] = combine(p.io, nbase + fld * nbn);
-                sn[f * kMaxCols + c] =
-                    make_double2(fm[f * 4] * w[1] + fm[f * 4 + 1] * w[2] + fm[f * 4 + 2] * w[3], w[0]);
+            for (int k = 0; k < 2; ++k) {
+                const int it = i + (k0 + k) * S::TPE;
+                int64_t nbase = base;
+                int nbn = 0;
+                slot[k] = -1;
+                if (it < nstart[4]) {
+                    const int f = it < nstart[2] ? (it < nstart[1] ? 0 : 1) : (it < nstart[3] ? 2 : 3);
+                    const int c = it - nstart[f];
+                    const int2 nf = fi4[f];
+                    const int pn = info_pn(nf.y);
+                    nbn = simplex_nb(pn);
+                    const int col = tix[tix[(pn - 1) * kTixStride + kTixCols + info_lf(nf.y)] + c];
+                    nbase = p.group_dof[pn - 1] + (nf.x - p.group_start[pn - 1]) * 4 * nbn + col;
+                    slot[k] = f * kMaxCols + c;
+                }
+#pragma unroll
+                for (int fld = 0; fld < 4; ++fld) at[4 * k + fld] = nbase + fld * nbn;
+            }
+            double w[8];
+            combine_n(p.io, at, w);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                if (slot[k] < 0) continue;
+                const int f = slot[k] / kMaxCols;
+                sn[slot[k]] = make_double2(fm[f * 4] * w[4 * k + 1] + fm[f * 4 + 1] * w[4 * k + 2] +
+                                               fm[f * 4 + 2] * w[4 * k + 3],
+                                           w[4 * k]);
             }
         }
     }
@@ -313,7 +337,7 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
             const int f = it / NC, m = it % NC;
             const int2 nf = fi4[f];
             double a_ns, a_v;
-            dot_pairs<NC>(p.tab + tx[kTixProj] + ((f * 6 + ident) * 15 + m) * NC, so + f * NC, a_ns, a_v);
+            dot_pairs<NC>(p.tab + tx[kTixProj] + (f * 6 + ident) * 15 * NC + m, so + f * NC, a_ns, a_v);
             double Fv, Fs;
             if (nf.x >= 0) {
                 const int pn = info_pn(nf.y);
@@ -323,10 +347,10 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
                 const double* proj = p.tab + tix[(pn - 1) * kTixStride + kTixProj];
                 double b_ns, b_v;
                 switch (pn) {
-                    case 1: dot_pairs<3>(proj + ((lfn * 6 + o) * 15 + m) * 3, xn, b_ns, b_v); break;
-                    case 2: dot_pairs<6>(proj + ((lfn * 6 + o) * 15 + m) * 6, xn, b_ns, b_v); break;
-                    case 3: dot_pairs<10>(proj + ((lfn * 6 + o) * 15 + m) * 10, xn, b_ns, b_v); break;
-                    default: dot_pairs<15>(proj + ((lfn * 6 + o) * 15 + m) * 15, xn, b_ns, b_v); break;
+                    case 1: dot_pairs<3>(proj + (lfn * 6 + o) * 15 * 3 + m, xn, b_ns, b_v); break;
+                    case 2: dot_pairs<6>(proj + (lfn * 6 + o) * 15 * 6 + m, xn, b_ns, b_v); break;
+                    case 3: dot_pairs<10>(proj + (lfn * 6 + o) * 15 * 10 + m, xn, b_ns, b_v); break;
+                    default: dot_pairs<15>(proj + (lfn * 6 + o) * 15 * 15 + m, xn, b_ns, b_v); break;
                 }
                 Fv = -0.5 * (a_ns - b_ns);
                 Fs = -0.5 * (a_v - b_v);
@@ -689,11 +713,12 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
             t[kTixCols + f] = int32_t(cols.size());  // relative, fixed below
             cols.insert(cols.end(), ops.cols[f].begin(), ops.cols[f].end());
         }
-        t[kTixProj] = int32_t(tab.size());  // [f][o][m < kFaceBasis][c < nc]
+        t[kTixProj] = int32_t(tab.size());  // [f][o][c < nc][m < kFaceBasis]
         for (int f = 0; f < 4; ++f)
             for (int o = 0; o < 6; ++o) {
                 require(ops.proj[f][o].size() == size_t(kFaceBasis) * nc, HGDG_ERR_ARG, "face trace table size");
-                tab.insert(tab.end(), ops.proj[f][o].begin(), ops.proj[f][o].end());
+                for (int c = 0; c < nc; ++c)
+                    for (int m = 0; m < kFaceBasis; ++m) tab.push_back(ops.proj[f][o][size_t(m) * nc + c]);
             }
         t[kTixLift] = int32_t(tab.size());  // [f][m < nc][i]: only chi_m of degree <= q meet the element's traces
         for (int f = 0; f < 4; ++f)
