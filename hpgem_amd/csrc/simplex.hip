@@ -142,7 +142,9 @@ struct SimplexShape {
     static constexpr int SN = SO + EPB * 8 * NCOL;          // [EPB][4 faces][kMaxCols] (double2): the same for the neighbours
     static constexpr int SF = SN + EPB * 8 * kMaxCols;      // [EPB][4 faces][NCOL] (T pairs): flux coefficients
     static constexpr int FACE_END = SF + EPB * 8 * NCOL;
-    static constexpr int TOTAL = FACE_END > VOL_END ? FACE_END : VOL_END;
+    static constexpr bool LIFT_LDS = P < 4;                 // lift table staged in LDS (order 4: read through the caches)
+    static constexpr int LIFT = FACE_END > VOL_END ? FACE_END : VOL_END;  // [4 f][NCOL m][NB i] (T), the whole kernel
+    static constexpr int TOTAL = LIFT + (LIFT_LDS ? 4 * NCOL * NB : 0);
     static_assert(FI % 2 == 0 && U % 2 == 0 && W % 2 == 0 && SN % 2 == 0 && SF % 2 == 0, "double2 alignment");
 };
 
@@ -236,6 +238,12 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
     if constexpr (S::DR_LDS) {
 #pragma unroll 4
         for (int k = tid; k < 3 * NB * NB; k += kBlock) sDr[k] = gDr[k];
+    }
+    const T* gLift = tabT + p.tix[(P - 1) * kTixStride + kTixLift];
+    T* sLift = reinterpret_cast<T*>(lds + S::LIFT);
+    if constexpr (S::LIFT_LDS) {
+#pragma unroll 4
+        for (int k = tid; k < 4 * NC * NB; k += kBlock) sLift[k] = gLift[k];
     }
     __syncthreads();
 
@@ -345,12 +353,17 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
                 const double2* xn = sn + f * kMaxCols;
                 const int lfn = info_lf(nf.y);
                 const double* proj = p.tab + tix[(pn - 1) * kTixStride + kTixProj];
-                double b_ns, b_v;
-                switch (pn) {
-                    case 1: dot_pairs<3>(proj + (lfn * 6 + o) * 15 * 3 + m, xn, b_ns, b_v); break;
-                    case 2: dot_pairs<6>(proj + (lfn * 6 + o) * 15 * 6 + m, xn, b_ns, b_v); break;
-                    case 3: dot_pairs<10>(proj + (lfn * 6 + o) * 15 * 10 + m, xn, b_ns, b_v); break;
-                    default: dot_pairs<15>(proj + (lfn * 6 + o) * 15 * 15 + m, xn, b_ns, b_v); break;
+                // one loop for every neighbour order: the items of a wave
+                // meet neighbours of different orders, and a loop bounded
+                // per thread costs the longest of them, not their sum
+                const int ncn = simplex_ncol(pn);
+                const double* col = proj + (lfn * 6 + o) * 15 * ncn + m;
+                double b_ns = 0, b_v = 0;
+                for (int c = 0; c < ncn; ++c) {
+                    const double r = col[c * 15];
+                    const double2 v = xn[c];
+                    b_ns += r * v.x;
+                    b_v += r * v.y;
                 }
                 Fv = -0.5 * (a_ns - b_ns);
                 Fs = -0.5 * (a_v - b_v);
@@ -366,7 +379,7 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
     // ---- phase 5: k += sigma_f sum_m (M_ref^{-1} int_f phi chi_m)_i F_m
     if (row_ok) {
         const T ci = T(1.0 / cinv);
-        const T* lift = tabT + tix[(P - 1) * kTixStride + kTixLift] + i;
+        const T* lift = (S::LIFT_LDS ? sLift : gLift) + i;
 #pragma unroll 1
         for (int f = 0; f < 4; ++f) {
             const FluxPair<T>* fl = sf + f * NC;
