@@ -43,6 +43,20 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t n) {
 }
 
 
+// Moves inside groups of four lanes through DPP quad permutations (a VALU
+// operand modifier) instead of ds_bpermute (an LDS round trip). QP is the
+// quad_perm control: bits [2l+1:2l] name the source lane of lane l; every lane
+// of the quad must be active.
+template <int QP>
+__device__ __forceinline__ double quad_perm(double x) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), QP, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), QP, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+constexpr int kQuadXor1 = 0xB1;  // lanes (1, 0, 3, 2)
+constexpr int kQuadXor2 = 0x4E;  // lanes (2, 3, 0, 1)
+__host__ __device__ constexpr int quad_bcast(int l) { return l * 0x55; }  // lanes (l, l, l, l)
+
 // Inputs and output of one stage evaluation: out = f(sum_i coef_i in_i).
 struct StageIO {
     const void* in[kMaxIn];

@@ -117,6 +117,71 @@ __device__ __forceinline__ void combine_pencil(const StageIO& p, int64_t base, d
     }
 }
 
+// Stage input at the traces of all 2 DIM faces of a pencil, input vector by
+// input vector with one precision branch each, so that every load of an
+// input issues before any is used: x faces (f < 2) one value at at[f], y/z
+// faces the pencil [at[f], at[f] + N1). Same rounding as combine().
+template <int DIM, int N1>
+__device__ __forceinline__ void combine_traces(const StageIO& p, const int64_t (&at)[2 * DIM],
+                                               double (&w)[2 * DIM][N1]) {
+#pragma unroll
+    for (int i = 0; i < kMaxIn; ++i) {
+        if (i >= p.n_in) break;
+        double x[2 * DIM][N1];
+        if ((p.in_f32 >> i) & 1u) {
+            const float* v = static_cast<const float*>(p.in[i]);
+#pragma unroll
+            for (int f = 0; f < 2; ++f) x[f][0] = v[at[f]];
+#pragma unroll
+            for (int f = 2; f < 2 * DIM; ++f) {
+                const float* q = v + at[f];
+                if constexpr (N1 % 4 == 0) {
+#pragma unroll
+                    for (int a = 0; a < N1; a += 4) {
+                        const float4 t = *reinterpret_cast<const float4*>(q + a);
+                        x[f][a] = t.x; x[f][a + 1] = t.y; x[f][a + 2] = t.z; x[f][a + 3] = t.w;
+                    }
+                } else if constexpr (N1 % 2 == 0) {
+#pragma unroll
+                    for (int a = 0; a < N1; a += 2) {
+                        const float2 t = *reinterpret_cast<const float2*>(q + a);
+                        x[f][a] = t.x; x[f][a + 1] = t.y;
+                    }
+                } else {
+#pragma unroll
+                    for (int a = 0; a < N1; ++a) x[f][a] = q[a];
+                }
+            }
+        } else {
+            const double* v = static_cast<const double*>(p.in[i]);
+#pragma unroll
+            for (int f = 0; f < 2; ++f) x[f][0] = v[at[f]];
+#pragma unroll
+            for (int f = 2; f < 2 * DIM; ++f) {
+                const double* q = v + at[f];
+                if constexpr (N1 % 2 == 0) {
+#pragma unroll
+                    for (int a = 0; a < N1; a += 2) {
+                        const double2 t = *reinterpret_cast<const double2*>(q + a);
+                        x[f][a] = t.x; x[f][a + 1] = t.y;
+                    }
+                } else {
+#pragma unroll
+                    for (int a = 0; a < N1; ++a) x[f][a] = q[a];
+                }
+            }
+        }
+        const double c = p.coef[i];
+#pragma unroll
+        for (int f = 0; f < 2 * DIM; ++f)
+#pragma unroll
+            for (int a = 0; a < (f < 2 ? 1 : N1); ++a) {
+                if (i == 0) w[f][a] = c * x[f][a];
+                else w[f][a] += c * x[f][a];
+            }
+    }
+}
+
 __device__ __forceinline__ double face_jump(uint32_t m, double own, double other) {
     if (m == MODE_NONE) return -own;
     if (m == MODE_HALF_OWN) return -0.5 * own;
@@ -456,11 +521,18 @@ __device__ __forceinline__ void acoustic_block_quad(const AcousticParams& p, int
         for (int a = 0; a < N1; ++a) {
             double g = 0;
 #pragma unroll
-            for (int k = 0; k < DIM; ++k) g += Jr[k] * __shfl(w[a], 1 + k, 4);
+            for (int k = 0; k < DIM; ++k) {
+                const double wk = k == 0 ? quad_perm<quad_bcast(1)>(w[a])
+                                : k == 1 ? quad_perm<quad_bcast(2)>(w[a]) : quad_perm<quad_bcast(3)>(w[a]);
+                g += Jr[k] * wk;
+            }
             if (lane) sg[(r + 1) * NB + pc * N1 + a] = fld == 0 ? T(w[a]) : T(g);
         }
     }
     const double cinv = active ? p.cinv[e] : 1.0;
+    int32_t nbi[2 * DIM];  // the face neighbours, loaded with the stage input
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f) nbi[f] = active ? p.nbr[f * p.n_elem + e] : -1;
     __syncthreads();
 
     // ---- volume
@@ -503,39 +575,50 @@ __device__ __forceinline__ void acoustic_block_quad(const AcousticParams& p, int
         for (int a = 0; a < N1; ++a) acc[a] = Acc<T>(vol[a]);
     }
 
-    // ---- faces: x faces one value per pencil, y/z faces a whole pencil
-#pragma unroll 1
+    // ---- faces: x faces one value per pencil, y/z faces a whole pencil.
+    // The neighbours' traces of all faces first, every global load issued
+    // before any is used (a neighbour in the block is read from LDS; a face
+    // that loads nothing points at the thread's own pencil).
+    double oth[2 * DIM][N1];
+    {
+        int64_t at[2 * DIM];
+#pragma unroll
+        for (int f = 0; f < 2 * DIM; ++f) {
+            const int df = f / 2, s = f % 2;
+            const int nbr_pc = df == 0 ? pc : df == 1 ? (1 - s) + N1 * ic : ib + N1 * (1 - s);
+            const int noff = df == 0 ? 1 - s : 0;
+            const int64_t lnb = int64_t(nbi[f]) - blk * S::EPB;
+            const bool glob = lane && nbi[f] >= 0 && !(lnb >= 0 && lnb < S::EPB);
+            at[f] = glob ? int64_t(nbi[f]) * NF * NB + fld * NB + nbr_pc * N1 + noff : base + fld * NB + pc * N1;
+        }
+        if (lane) combine_traces<DIM, N1>(p.io, at, oth);  // (an inactive thread's own pencil may lie past the end)
+#pragma unroll
+        for (int f = 0; f < 2 * DIM; ++f) {
+            const int df = f / 2, s = f % 2;
+            const int nbr_pc = df == 0 ? pc : df == 1 ? (1 - s) + N1 * ic : ib + N1 * (1 - s);
+            const int noff = df == 0 ? 1 - s : 0;
+            const int64_t lnb = int64_t(nbi[f]) - blk * S::EPB;
+            if (lane && nbi[f] >= 0 && lnb >= 0 && lnb < S::EPB) {
+#pragma unroll
+                for (int a = 0; a < (df == 0 ? 1 : N1); ++a)
+                    oth[f][a] = s_w[(lnb * NF + fld) * NB + nbr_pc * N1 + noff + a];
+            }
+        }
+    }
+#pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) {
         const int df = f / 2, s = f % 2;
-        const int64_t nb = active ? p.nbr[f * p.n_elem + e] : -1;
-        const bool inner = nb >= 0;
+        const bool inner = nbi[f] >= 0;
         const int own_pc = df == 0 ? pc : df == 1 ? s + N1 * ic : ib + N1 * s;
-        const int nbr_pc = df == 0 ? pc : df == 1 ? (1 - s) + N1 * ic : ib + N1 * (1 - s);
-        const int na = df == 0 ? 1 : N1;       // trace values per pencil
-        const int off = df == 0 ? s : 0;       // own node of an x face
-        const int noff = df == 0 ? 1 - s : 0;  // neighbour's node
+        const int na = df == 0 ? 1 : N1;  // trace values per pencil
+        const int off = df == 0 ? s : 0;  // own node of an x face
         double jmp[N1];
 #pragma unroll
         for (int a = 0; a < N1; ++a) jmp[a] = 0.0;
         if (lane) {
-            double oth[N1];
-#pragma unroll
-            for (int a = 0; a < N1; ++a) oth[a] = 0.0;
-            if (inner) {
-                const int64_t lnb = nb - blk * S::EPB;
-                if (lnb >= 0 && lnb < S::EPB) {
-#pragma unroll
-                    for (int a = 0; a < N1; ++a)
-                        if (a < na) oth[a] = s_w[(lnb * NF + fld) * NB + nbr_pc * N1 + noff + a];
-                } else if (df == 0) {
-                    oth[0] = combine(p.io, nb * NF * NB + fld * NB + nbr_pc * N1 + noff);
-                } else {
-                    combine_pencil<N1>(p.io, nb * NF * NB + fld * NB + nbr_pc * N1, oth);
-                }
-            }
 #pragma unroll
             for (int a = 0; a < N1; ++a)
-                if (a < na) jmp[a] = sw[fld * NB + own_pc * N1 + off + a] - (inner ? oth[a] : 0.0);
+                if (a < na) jmp[a] = sw[fld * NB + own_pc * N1 + off + a] - (inner ? oth[f][a] : 0.0);
         }
         const double wd = (2 * s - 1) * (lane && fld > 0 ? sJ[df * DIM + d] : 0.0);  // w_d on the s_d lane
         const int my = df == 1 ? ib : ic;
@@ -544,9 +627,9 @@ __device__ __forceinline__ void acoustic_block_quad(const AcousticParams& p, int
             if (a >= na) break;
             // v lane: sum_k w_k [s_k] over the quad; s lanes: [v] from the v lane
             double x = wd * jmp[a];
-            x += __shfl_xor(x, 1, 4);
-            x += __shfl_xor(x, 2, 4);
-            const double jv = __shfl(jmp[a], 0, 4);
+            x += quad_perm<kQuadXor1>(x);
+            x += quad_perm<kQuadXor2>(x);
+            const double jv = quad_perm<quad_bcast(0)>(jmp[a]);
             double c;
             if (fld == 0) c = inner ? -0.5 * x : -x;
             else c = inner ? wd * (-0.5 * jv) : 0.0;
