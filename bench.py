@@ -41,7 +41,7 @@ SEED = 1
 # commands and per-kernel figures are in
 # profiles/rhs_kernels_32cube_p3_and_tet24.md. Counters cannot be read from
 # inside this process, so the figure is reported as measured there.
-MEASURED_TRAFFIC = {"double": 58.18e6, "float": 37.65e6}
+MEASURED_TRAFFIC = {"double": 57.82e6, "float": 37.45e6}
 # dt = CFL h / |a|. At CFL 0.1 the RK4 step of this p = 3 operator has
 # spectral radius ~1.47 (power iteration on the oracle, 4^3 periodic mesh), so
 # round-off grows ~1e8-fold over 50 steps; 0.05 lies inside the stability
@@ -230,6 +230,20 @@ def run_profile_workload(steps):
             api.computeTimeDerivative([0, 1], [1.0, 1e-4], 2)
         api.synchronize()
         api.close()
+    # the hex acoustic workload (32^3, p=3, four fields)
+    hexes = Mesh.structured([32, 32, 32])
+    for prec in ("fp64", "mixed"):
+        api = HpgemAPIDevice(hexes, 3, precision=prec, kind=OP_ACOUSTIC_CENTRAL, c_inv=np.ones(hexes.n_elements),
+                             tableau=RK4, n_extra_vectors=2)
+        api.setInitialSolutionModes([1.0], [[1.0, 2.0, 1.0]], field=0)
+        if prec == "mixed":
+            for v in (1, 2):
+                api.setStorage(v, "mixed")
+        api.computeTimeDerivative([0], [1.0], 1)
+        for _ in range(steps):
+            api.computeTimeDerivative([0, 1], [1.0, 1e-4], 2)
+        api.synchronize()
+        api.close()
     print(json.dumps({"profile_workload": "done", "steps": steps}))
 
 
@@ -243,7 +257,10 @@ def run_other_config(name, steps, warmup):
     tet24_hp_acoustic: acoustic system on 24^3 cubes x 5 tets (69120, walls),
       per-element p in {1..4} and c in U(0.5, 2) from the seed, Gaussian
       pressure pulse (width 0.05) at a seeded centre, dt = 0.04 h / c_max,
-      100 steps."""
+      100 steps.
+    hex32_p3_acoustic: acoustic system on a periodic 32^3 hex box, p=3 (four
+      fields, 8,388,608 DOF), c in U(0.5, 2) per element from the seed, one
+      pressure sine mode, dt = 0.05 h / c_max, 100 steps."""
     from hpgem_amd import OP_ACOUSTIC_CENTRAL, HpgemAPIDevice, Mesh, RK4
 
     events = HipEvents()
@@ -257,6 +274,15 @@ def run_other_config(name, steps, warmup):
         make = lambda prec: HpgemAPIDevice(mesh, 2, velocity=vel, precision=prec, tableau=RK4, n_extra_vectors=2)
         init = lambda api: api.setInitialSolutionModes(amp, kk)
         kernel = "tensor_advection_rhs<2,3,{}>"
+    elif name == "hex32_p3_acoustic":
+        mesh = Mesh.structured([32, 32, 32])
+        c = rng.uniform(0.5, 2.0, size=mesh.n_elements)
+        cinv = 1.0 / c ** 2
+        dt = 0.05 * (1 / 32) / float(c.max())
+        make = lambda prec: HpgemAPIDevice(mesh, 3, precision=prec, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv,
+                                           tableau=RK4, n_extra_vectors=2)
+        init = lambda api: api.setInitialSolutionModes([1.0], [[1.0, 2.0, 1.0]], field=0)
+        kernel = "tensor_acoustic_rhs<3,4,{}>"
     else:
         mesh = Mesh.structured([24, 24, 24], periodic=[0, 0, 0], simplices=True)
         orders = rng.integers(1, 5, size=mesh.n_elements).astype(np.int32)
@@ -411,7 +437,7 @@ def main():
     others = None
     if not args.headline_only:
         others = {name: run_other_config(name, args.steps, args.warmup)
-                  for name in ("quad16_p2_advection", "tet24_hp_acoustic")}
+                  for name in ("quad16_p2_advection", "tet24_hp_acoustic", "hex32_p3_acoustic")}
         others["maxwell16_p2_assembly"] = run_maxwell_assembly()
     serving = None
     if args.serving_requests > 0:
