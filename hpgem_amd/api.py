@@ -217,6 +217,25 @@ class HpgemAPIDevice:
         check(self.lib.hgdg_get_advection_tiles(self.h, ctypes.byref(t), dptr(ms)))
         return t.value, ms
 
+    # ---- threads per element of the tetrahedral kernel ---------------------
+    def setSimplexThreads(self, threads):
+        """Threads per element for orders 1..4 (four values): each 0 to time
+        both candidates of that order and keep the faster (done by default
+        when the operator is set), or one of the candidates (16/32, 32/64,
+        32/64, 35/64)."""
+        t = np.ascontiguousarray(threads, dtype=np.int32)
+        if t.shape != (4,):
+            raise ValueError("four values, orders 1..4")
+        check(self.lib.hgdg_set_simplex_threads(self.h, t.ctypes.data_as(_lib._I32)))
+
+    def getSimplexThreads(self):
+        """(threads per element in use for orders 1..4, timed ms per launch
+        [order][candidate], 0 where not timed)."""
+        t = np.zeros(4, dtype=np.int32)
+        ms = np.zeros(8)
+        check(self.lib.hgdg_get_simplex_threads(self.h, t.ctypes.data_as(_lib._I32), dptr(ms)))
+        return t, ms.reshape(4, 2)
+
     # ---- time-integration vectors (reference layout) ----------------------
     def setTimeIntegrationVector(self, vec, values):
         v = np.ascontiguousarray(values, dtype=np.float64)

@@ -40,8 +40,11 @@ struct SimplexMesh {
     std::vector<SimplexOps> ops;                 // per order (index p-1), built on demand
     DevBuf<double> d_tables;                     // Dr, proj, lift of every order
     DevBuf<float> d_tables32;                    // the same in FP32 (mixed path)
-    DevBuf<int32_t> d_blocks;                    // RHS block schedule: [n_blocks][4] order, first, count, 0
+    DevBuf<int32_t> d_blocks;                    // RHS block schedule: [n_blocks][4] order, first, count, variant
     int64_t n_blocks = 0;
+    int32_t tpe_variant[4] = {0, 0, 0, 0};       // per order: threads-per-element candidate in use (0 or 1)
+    int32_t tpe_fixed[4] = {0, 0, 0, 0};         // per order: 0 = timed choice, else the fixed threads per element
+    double tpe_ms[4][2] = {};                    // per order and candidate: timed RHS launch (ms), 0 if not timed
     DevBuf<int32_t> d_tab_index;                 // offsets and column lists into d_tables
     DevBuf<int64_t> d_elem_off;                  // [E + 1] first DOF of each device element
     DevBuf<double> d_geo;                        // J^{-1} [9][E], then c_inv [E], sigma [4][E], normal [3][4][E]
@@ -148,6 +151,9 @@ void maxwell_assemble(hgdg_ctx* c, int order, const double* eps, double stab, in
 // simplex.hip
 void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m);
 void simplex_operator_set(hgdg_ctx* c, const hgdg_op_desc* op);
+void simplex_schedule(hgdg_ctx* c);
+void simplex_tune(hgdg_ctx* c);
+int simplex_threads(int order, int variant);
 void simplex_rhs(hgdg_ctx* c, const StageIO& io);
 void simplex_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* l2, double* max_abs);
 void simplex_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, const double* amp, const double* k,

@@ -182,7 +182,8 @@ void operator_set(hgdg_ctx* c, const hgdg_op_desc* op) {
     c->op.c_inv = nullptr;
     c->has_op = true;
     c->drop_graph();
-    if (!c->simplex()) tensor_tune(c);
+    if (c->simplex()) simplex_tune(c);
+    else tensor_tune(c);
     if (old_fields != c->n_fields) c->vecs.clear();  // the field count changed the vector size
 }
 
@@ -528,6 +529,45 @@ int hgdg_get_advection_tiles(hgdg_ctx* ctx, int* tiles, double* ms) {
         *tiles = ctx->adv_tiles;
         if (ms)
             for (int i = 0; i < 4; ++i) ms[i] = ctx->adv_tile_ms[i];
+    });
+}
+
+int hgdg_set_simplex_threads(hgdg_ctx* ctx, const int32_t* threads) {
+    return run([&] {
+        require(ctx != nullptr && threads != nullptr, HGDG_ERR_ARG, "ctx and threads must be non-NULL");
+        for (int g = 0; g < 4; ++g)
+            require(threads[g] == 0 || threads[g] == hgdg::simplex_threads(g + 1, 0) ||
+                        threads[g] == hgdg::simplex_threads(g + 1, 1),
+                    HGDG_ERR_ARG,
+                    "threads per element of order " + std::to_string(g + 1) + " must be 0 (timed choice), " +
+                        std::to_string(hgdg::simplex_threads(g + 1, 0)) + " or " +
+                        std::to_string(hgdg::simplex_threads(g + 1, 1)));
+        HGDG_HIP(hipSetDevice(ctx->device));
+        int32_t before[4];
+        std::copy(ctx->sm.tpe_fixed, ctx->sm.tpe_fixed + 4, before);
+        std::copy(threads, threads + 4, ctx->sm.tpe_fixed);
+        if (ctx->has_op && ctx->simplex()) {
+            ctx->drop_graph();
+            try {
+                hgdg::simplex_tune(ctx);
+            } catch (...) {
+                std::copy(before, before + 4, ctx->sm.tpe_fixed);  // keep the previous choice in force
+                throw;
+            }
+        }
+    });
+}
+
+int hgdg_get_simplex_threads(hgdg_ctx* ctx, int32_t* threads, double* ms) {
+    return run([&] {
+        require(ctx != nullptr && threads != nullptr, HGDG_ERR_ARG, "ctx and threads must be non-NULL");
+        for (int g = 0; g < 4; ++g) {
+            threads[g] = hgdg::simplex_threads(g + 1, ctx->sm.tpe_variant[g]);
+            if (ms) {
+                ms[2 * g] = ctx->sm.tpe_ms[g][0];
+                ms[2 * g + 1] = ctx->sm.tpe_ms[g][1];
+            }
+        }
     });
 }
 

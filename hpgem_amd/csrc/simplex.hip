@@ -23,6 +23,7 @@
 #include <string>
 #include <type_traits>
 #include <unordered_map>
+#include <utility>
 
 #include "context.hpp"
 
@@ -111,20 +112,27 @@ struct SimplexParams {
     const float* tab32;    // the same tables in FP32 (mixed path), same offsets
     const int32_t* tix;
     int32_t n_tix;
-    const int4* blocks;    // per block: order, first device element, element count, unused
+    const int4* blocks;    // per block: order, first device element, element count, threads-per-element variant (informative)
 };
 
 __host__ __device__ constexpr int simplex_nb(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
 __host__ __device__ constexpr int simplex_ncol(int p) { return (p + 1) * (p + 2) / 2; }
-// threads per element: 16 at order 1 (four elements per wavefront), 32 at
-// orders 2 and 3 (two), nb at order 4
+// Threads per element of the RHS kernel, two candidates per order (a row
+// phase needs at least nb threads): variant 0 = 16 at order 1 (four elements
+// per wavefront), 32 at orders 2 and 3 (two), nb at order 4; variant 1 = 32
+// at order 1, 64 (one element per wavefront) above. The library times both per order when the operator is set and
+// keeps the faster; results do not depend on the choice (every row and item is
+// computed by the same instructions whichever thread takes it).
 __host__ __device__ constexpr int simplex_tpe(int p) { return simplex_nb(p) > 32 ? simplex_nb(p) : p == 1 ? 16 : 32; }
+__host__ __device__ constexpr int simplex_tpe_alt(int p) { return p == 1 ? 32 : 64; }
+__host__ __device__ constexpr int simplex_tpe_of(int p, int variant) { return variant ? simplex_tpe_alt(p) : simplex_tpe(p); }
 
-template <int P>
+template <int P, int TPE_ = simplex_tpe(P)>
 struct SimplexShape {
+    static_assert(TPE_ >= simplex_nb(P) && TPE_ <= 256, "a row phase needs one thread per basis function");
     static constexpr int NB = simplex_nb(P);
     static constexpr int NCOL = simplex_ncol(P);
-    static constexpr int TPE = simplex_tpe(P);  // threads per element
+    static constexpr int TPE = TPE_;  // threads per element
     static constexpr int EPB = kBlock / TPE;
     // LDS carve-up of the RHS kernel (in doubles; T-typed regions use the
     // first half of theirs in FP32). FM, FI and U live through the whole
@@ -189,10 +197,10 @@ __device__ __forceinline__ void dot_pairs(const double* col, const double2* x, d
 // coefficients. T = double: FP64; T = float: the volume contractions and the
 // lifts in FP32 on FP32 tables, compensated per-row sums, traces and flux
 // coefficients in FP64.
-template <int P, typename T>
+template <int P, int TPE, typename T>
 __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int32_t* tix, int64_t first, int count,
                                               double* lds) {
-    using S = SimplexShape<P>;
+    using S = SimplexShape<P, TPE>;
     constexpr int NB = S::NB;
     constexpr int NC = S::NCOL;
     const int tid = threadIdx.x;
@@ -410,10 +418,12 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
 }
 
 constexpr int simplex_lds_doubles() {
-    int m = SimplexShape<1>::TOTAL;
-    m = SimplexShape<2>::TOTAL > m ? SimplexShape<2>::TOTAL : m;
-    m = SimplexShape<3>::TOTAL > m ? SimplexShape<3>::TOTAL : m;
-    m = SimplexShape<4>::TOTAL > m ? SimplexShape<4>::TOTAL : m;
+    constexpr int t[8] = {SimplexShape<1>::TOTAL, SimplexShape<2>::TOTAL, SimplexShape<3>::TOTAL,
+                          SimplexShape<4>::TOTAL, SimplexShape<1, simplex_tpe_alt(1)>::TOTAL,
+                          SimplexShape<2, simplex_tpe_alt(2)>::TOTAL, SimplexShape<3, simplex_tpe_alt(3)>::TOTAL,
+                          SimplexShape<4, simplex_tpe_alt(4)>::TOTAL};
+    int m = 0;
+    for (int k = 0; k < 8; ++k) m = t[k] > m ? t[k] : m;
     return m;
 }
 
@@ -424,17 +434,17 @@ constexpr int simplex_lds_doubles() {
 // element gathers from neighbours of every order are read while they are
 // still in that XCD's L2. The LDS carve-up of the largest order leaves room
 // for five blocks per CU; the register budget is set to match.
-template <typename T>
+template <typename T, int V>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void simplex_acoustic_rhs(const SimplexParams p) {
     __shared__ __align__(16) double lds[simplex_lds_doubles()];  // regions start at even doubles: double2-aligned
     __shared__ int32_t tix[kTixMax];                              // table offsets, orientations, face column lists
     for (int k = threadIdx.x; k < p.n_tix; k += kBlock) tix[k] = p.tix[k];
     const int4 bl = p.blocks[xcd_block(blockIdx.x, gridDim.x)];
-    switch (bl.x) {  // the same order for the whole block
-        case 1: simplex_block<1, T>(p, tix, bl.y, bl.z, lds); break;
-        case 2: simplex_block<2, T>(p, tix, bl.y, bl.z, lds); break;
-        case 3: simplex_block<3, T>(p, tix, bl.y, bl.z, lds); break;
-        default: simplex_block<4, T>(p, tix, bl.y, bl.z, lds); break;
+    switch (bl.x) {  // the same order for the whole block; bit p-1 of V: the order's threads-per-element variant
+        case 1: simplex_block<1, simplex_tpe_of(1, V & 1), T>(p, tix, bl.y, bl.z, lds); break;
+        case 2: simplex_block<2, simplex_tpe_of(2, (V >> 1) & 1), T>(p, tix, bl.y, bl.z, lds); break;
+        case 3: simplex_block<3, simplex_tpe_of(3, (V >> 2) & 1), T>(p, tix, bl.y, bl.z, lds); break;
+        default: simplex_block<4, simplex_tpe_of(4, (V >> 3) & 1), T>(p, tix, bl.y, bl.z, lds); break;
     }
 }
 
@@ -744,33 +754,11 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     tix.insert(tix.end(), cols.begin(), cols.end());
     require(tix.size() <= size_t(kTixMax), HGDG_ERR_ARG, "table index larger than its LDS copy");
 
-    // block schedule: slabs of consecutive mesh elements, each order's
-    // elements of a slab in blocks of that order's size
-    {
-        const int64_t slab = std::max<int64_t>(1024, (ne + 255) / 256);
-        static const int epb[4] = {SimplexShape<1>::EPB, SimplexShape<2>::EPB, SimplexShape<3>::EPB,
-                                   SimplexShape<4>::EPB};
-        std::vector<int32_t> blocks;
-        int64_t cursor[4] = {sm.group_start[0], sm.group_start[1], sm.group_start[2], sm.group_start[3]};
-        for (int64_t lo = 0; lo < ne; lo += slab) {
-            const int64_t hi = std::min(ne, lo + slab);
-            for (int g = 0; g < 4; ++g) {
-                // device elements of order g+1 with mesh index below hi (mesh order inside a group)
-                int64_t end = cursor[g];
-                while (end < sm.group_start[g + 1] && sm.mesh_of[end] < hi) ++end;
-                for (int64_t b = cursor[g]; b < end; b += epb[g]) {
-                    blocks.push_back(g + 1);
-                    blocks.push_back(int32_t(b));
-                    blocks.push_back(int32_t(std::min<int64_t>(epb[g], end - b)));
-                    blocks.push_back(0);
-                }
-                cursor[g] = end;
-            }
-        }
-        sm.n_blocks = int64_t(blocks.size() / 4);
-        sm.d_blocks.upload(blocks, c->stream);
-        HGDG_HIP(hipStreamSynchronize(c->stream));  // `blocks` goes out of scope
+    for (int g = 0; g < 4; ++g) {
+        sm.tpe_variant[g] = sm.tpe_fixed[g] == simplex_tpe_alt(g + 1) ? 1 : 0;
+        sm.tpe_ms[g][0] = sm.tpe_ms[g][1] = 0;
     }
+    simplex_schedule(c);
 
     sm.d_tables.upload(tab, c->stream);
     const std::vector<float> tab32(tab.begin(), tab.end());  // lives until the synchronize below
@@ -800,7 +788,42 @@ void simplex_operator_set(hgdg_ctx* c, const hgdg_op_desc* op) {
     HGDG_HIP(hipStreamSynchronize(c->stream));
 }
 
-void simplex_rhs(hgdg_ctx* c, const StageIO& io) {
+int simplex_threads(int order, int variant) { return simplex_tpe_of(order, variant); }
+
+// Block schedule: the mesh is cut into slabs of consecutive elements and, slab
+// after slab, each order's elements of the slab fill blocks of that order's
+// size (256 / threads per element of the order's variant in use).
+void simplex_schedule(hgdg_ctx* c) {
+    SimplexMesh& sm = c->sm;
+    const int64_t ne = c->n_elem;
+    const int64_t slab = std::max<int64_t>(1024, (ne + 255) / 256);
+    int epb[4];
+    for (int g = 0; g < 4; ++g) epb[g] = kBlock / simplex_tpe_of(g + 1, sm.tpe_variant[g]);
+    std::vector<int32_t> blocks;
+    int64_t cursor[4] = {sm.group_start[0], sm.group_start[1], sm.group_start[2], sm.group_start[3]};
+    for (int64_t lo = 0; lo < ne; lo += slab) {
+        const int64_t hi = std::min(ne, lo + slab);
+        for (int g = 0; g < 4; ++g) {
+            // device elements of order g+1 with mesh index below hi (mesh order inside a group)
+            int64_t end = cursor[g];
+            while (end < sm.group_start[g + 1] && sm.mesh_of[end] < hi) ++end;
+            for (int64_t b = cursor[g]; b < end; b += epb[g]) {
+                blocks.push_back(g + 1);
+                blocks.push_back(int32_t(b));
+                blocks.push_back(int32_t(std::min<int64_t>(epb[g], end - b)));
+                blocks.push_back(sm.tpe_variant[g]);
+            }
+            cursor[g] = end;
+        }
+    }
+    sm.n_blocks = int64_t(blocks.size() / 4);
+    sm.d_blocks.upload(blocks, c->stream);
+    HGDG_HIP(hipStreamSynchronize(c->stream));  // `blocks` goes out of scope
+}
+
+namespace {
+
+SimplexParams simplex_params(hgdg_ctx* c, const StageIO& io) {
     SimplexMesh& sm = c->sm;
     const int64_t ne = c->n_elem;
     SimplexParams p{};
@@ -822,14 +845,99 @@ void simplex_rhs(hgdg_ctx* c, const StageIO& io) {
     p.tix = sm.d_tab_index.ptr;
     p.n_tix = int32_t(sm.d_tab_index.n);
     p.blocks = reinterpret_cast<const int4*>(sm.d_blocks.ptr);
+    return p;
+}
+
+using SimplexKernel = void (*)(SimplexParams);
+
+template <typename T, int... V>
+constexpr std::array<SimplexKernel, sizeof...(V)> simplex_kernels(std::integer_sequence<int, V...>) {
+    return {&simplex_acoustic_rhs<T, V>...};
+}
+
+void launch_simplex(hgdg_ctx* c, const SimplexParams& p) {
+    // one kernel per combination of the orders' threads-per-element variants
+    static const auto k64 = simplex_kernels<double>(std::make_integer_sequence<int, 16>{});
+    static const auto k32 = simplex_kernels<float>(std::make_integer_sequence<int, 16>{});
+    const SimplexMesh& sm = c->sm;
+    int v = 0;
+    for (int g = 0; g < 4; ++g) v |= (sm.tpe_variant[g] & 1) << g;
     const int grid = int(sm.n_blocks);
     if (grid > 0) {
-        if (c->op.precision == HGDG_PREC_MIXED_FP32)
-            hipLaunchKernelGGL(simplex_acoustic_rhs<float>, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
-        else
-            hipLaunchKernelGGL(simplex_acoustic_rhs<double>, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
+        const SimplexKernel k = c->op.precision == HGDG_PREC_MIXED_FP32 ? k32[size_t(v)] : k64[size_t(v)];
+        hipLaunchKernelGGL(k, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
     }
     HGDG_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void simplex_rhs(hgdg_ctx* c, const StageIO& io) { launch_simplex(c, simplex_params(c, io)); }
+
+// Threads per element, order by order: both candidates of an order are timed
+// on the whole mesh (the other orders at their current choice) as one RK
+// stage (u + a k -> k' on zero vectors of the operator's storage types; the
+// kernel has no data-dependent branches), and the faster is kept. Orders with
+// a fixed choice or without elements are not timed.
+void simplex_tune(hgdg_ctx* c) {
+    SimplexMesh& sm = c->sm;
+    for (int g = 0; g < 4; ++g) {
+        sm.tpe_variant[g] = sm.tpe_fixed[g] == simplex_tpe_alt(g + 1) ? 1 : 0;
+        sm.tpe_ms[g][0] = sm.tpe_ms[g][1] = 0;
+    }
+    bool any = false;
+    for (int g = 0; g < 4; ++g) any = any || (sm.tpe_fixed[g] == 0 && sm.group_start[g + 1] > sm.group_start[g]);
+    if (!any) {
+        simplex_schedule(c);
+        return;
+    }
+    const bool mixed = c->op.precision == HGDG_PREC_MIXED_FP32;
+    const size_t n = size_t(c->n_dof());
+    DevBuf<double> u, k, out;
+    u.alloc(n);
+    k.alloc(n);
+    out.alloc(n);
+    HGDG_HIP(hipMemsetAsync(u.ptr, 0, n * sizeof(double), c->stream));
+    HGDG_HIP(hipMemsetAsync(k.ptr, 0, n * sizeof(double), c->stream));
+    StageIO io{};
+    io.n_in = 2;
+    io.in[0] = u.ptr;
+    io.coef[0] = 1.0;
+    io.in[1] = k.ptr;
+    io.coef[1] = 0.5;
+    io.in_f32 = mixed ? 2u : 0u;
+    io.out = out.ptr;
+    io.out_f32 = mixed ? 1 : 0;
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events() {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    HGDG_HIP(hipEventCreate(&ev.e[0]));
+    HGDG_HIP(hipEventCreate(&ev.e[1]));
+    constexpr int kReps = 10;
+    for (int g = 0; g < 4; ++g) {
+        if (sm.tpe_fixed[g] != 0 || sm.group_start[g + 1] == sm.group_start[g]) continue;
+        for (int pass = 0; pass < 2; ++pass)
+            for (int v = 0; v < 2; ++v) {
+                sm.tpe_variant[g] = v;
+                simplex_schedule(c);
+                const SimplexParams p = simplex_params(c, io);
+                for (int r = 0; r < 2; ++r) launch_simplex(c, p);
+                HGDG_HIP(hipEventRecord(ev.e[0], c->stream));
+                for (int r = 0; r < kReps; ++r) launch_simplex(c, p);
+                HGDG_HIP(hipEventRecord(ev.e[1], c->stream));
+                HGDG_HIP(hipEventSynchronize(ev.e[1]));
+                float ms = 0;
+                HGDG_HIP(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
+                const double t = double(ms) / kReps;
+                if (pass == 0 || t < sm.tpe_ms[g][v]) sm.tpe_ms[g][v] = t;
+            }
+        sm.tpe_variant[g] = sm.tpe_ms[g][1] < sm.tpe_ms[g][0] ? 1 : 0;
+    }
+    simplex_schedule(c);
 }
 
 void simplex_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, double* max_abs) {

@@ -127,6 +127,20 @@ int hgdg_set_advection_tiles(hgdg_ctx* ctx, int tiles);
  * 1, 2, 4, 8 (0 where not timed). ms may be NULL. */
 int hgdg_get_advection_tiles(hgdg_ctx* ctx, int* tiles, double* ms);
 
+/* Threads per element of the tetrahedral RHS kernel, per order 1..4
+ * (threads[p-1]): 0 (the default) lets hgdg_operator_set time both candidates
+ * of that order on the mesh and keep the faster; otherwise one of the two
+ * candidates, 16 or 32 (order 1), 32 or 64 (orders 2 and 3), 35 or 64
+ * (order 4). Results are bitwise the same for every choice. No
+ * counterpart in the reference (a launch parameter of this implementation);
+ * applies to the next hgdg_operator_set, or at once if an operator is set. */
+int hgdg_set_simplex_threads(hgdg_ctx* ctx, const int32_t* threads);
+
+/* The threads per element in use for orders 1..4 and the timed average launch
+ * (ms) of both candidates of each order, ms[2(p-1) + k] (0 where not timed).
+ * ms may be NULL. */
+int hgdg_get_simplex_threads(hgdg_ctx* ctx, int32_t* threads, double* ms);
+
 /* Allocate n time-integration vectors on the device (all zero).
  * Replaces setNumberOfTimeIntegrationVectorsGlobally (HpgemAPIBase_Impl.h:141-156). */
 int hgdg_state_alloc(hgdg_ctx* ctx, int n_vectors);

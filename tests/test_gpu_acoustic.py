@@ -211,6 +211,65 @@ def test_tet_mixed_rhs_single_eval():
         d.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tet_threads_per_element_give_bitwise_identical_rhs(precision):
+    """Every threads-per-element candidate of every order (the timed choice
+    and both fixed ones) computes each row and item with the same
+    instructions, so the RHS is bitwise the same; the timed choice is one of
+    the candidates and every order present was timed."""
+    mesh = Mesh.structured([3, 3, 2], periodic=[0, 0, 0], simplices=True)
+    rng = np.random.default_rng(31)
+    p = rng.integers(1, 5, size=mesh.n_elements).astype(np.int32)
+    cinv = 1.0 / rng.uniform(0.5, 2.0, size=mesh.n_elements) ** 2
+    dev = HpgemAPIDevice(mesh, p, precision=precision, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=RK4,
+                         n_extra_vectors=2)
+    cand = np.array([[16, 32], [32, 64], [32, 64], [35, 64]], dtype=np.int32)
+    threads, ms = dev.getSimplexThreads()
+    print(f"tuned threads {threads}, ms {ms.tolist()}")
+    for g in range(4):
+        assert threads[g] in cand[g]
+        if np.any(p == g + 1):
+            assert np.all(ms[g] > 0)
+    if precision == "mixed":
+        dev.setStorage(1, "mixed")
+        dev.setStorage(2, "mixed")
+    dev.setTimeIntegrationVector(0, rng.standard_normal(dev.n_dof))
+    dev.setTimeIntegrationVector(1, rng.standard_normal(dev.n_dof))
+    outs = []
+    for combo in range(16):
+        t = np.array([cand[g][(combo >> g) & 1] for g in range(4)], dtype=np.int32)
+        dev.setSimplexThreads(t)
+        np.testing.assert_array_equal(dev.getSimplexThreads()[0], t)
+        dev.setTimeIntegrationVector(2, np.full(dev.n_dof, np.nan))
+        dev.computeTimeDerivative([0, 1], [1.0, 0.25], 2)
+        outs.append(dev.getTimeIntegrationVector(2))
+    assert np.all(np.isfinite(outs[0]))
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
+    dev.setSimplexThreads([0, 0, 0, 0])
+    assert all(dev.getSimplexThreads()[0][g] in cand[g] for g in range(4))
+    with pytest.raises(Exception):
+        dev.setSimplexThreads([24, 0, 0, 0])
+    dev.close()
+
+
+def test_tet_alternative_threads_match_oracle():
+    """The larger threads-per-element candidate of every order against the
+    oracle (the default candidates are covered by the other parity tests)."""
+    print("[case] tet acoustic n=[3, 2, 2] p=random threads=[32, 64, 64, 64]", flush=True)
+    dev, (ref, ref_ld) = make([3, 2, 2], "random", 41)
+    dev.setSimplexThreads([32, 64, 64, 64])
+    rng = np.random.default_rng(42)
+    u = rng.standard_normal(dev.n_dof)
+    k = rng.standard_normal(dev.n_dof)
+    dev.setTimeIntegrationVector(0, u)
+    dev.setTimeIntegrationVector(1, k)
+    dev.computeTimeDerivative([0, 1], [1.0, -0.2], 2)
+    assert_parity(dev.getTimeIntegrationVector(2), ref.time_derivative([u, k], [1.0, -0.2]),
+                  ref_ld.time_derivative([u, k], [1.0, -0.2]))
+    dev.close()
+
+
 def test_tet_mixed_within_1e6_after_rk4():
     """hp tets (p in 1..4), RK4 with FP32 stage derivatives: relative L2
     distance to the FP64 run stays below 1e-6."""
