@@ -192,7 +192,8 @@ def run_profile_workload(steps):
     """Kernels only, for counter runs: `steps` FP64 RHS evaluations, `steps`
     mixed RHS evaluations and `steps` plain copies of one FP64 state vector
     (2,097,152 doubles = 16 MiB read + 16 MiB written), the copies being the
-    reference for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    reference for scaling the FETCH_SIZE / WRITE_SIZE counters; then the same
+    for the hp-tet and hex acoustic workloads, and two Maxwell assemblies."""
     from hpgem_amd import HpgemAPIDevice, Mesh, RK4
 
     mesh = Mesh.structured([N_CELLS] * 3)
@@ -230,6 +231,12 @@ def run_profile_workload(steps):
             api.computeTimeDerivative([0, 1], [1.0, 1e-4], 2)
         api.synchronize()
         api.close()
+    # the Maxwell assembly (16^3 x 5 tets, p=2): two launches
+    from hpgem_amd.api import maxwell_assemble
+    mx = Mesh.structured([16, 16, 16], periodic=[0, 0, 0], simplices=True)
+    eps = np.random.default_rng(SEED).choice([1.0, 2.25, 12.0], size=mx.n_elements)
+    for _ in range(2):
+        maxwell_assemble(mx, 2, eps, 10.0 * 4)
     # the hex acoustic workload (32^3, p=3, four fields)
     hexes = Mesh.structured([32, 32, 32])
     for prec in ("fp64", "mixed"):
