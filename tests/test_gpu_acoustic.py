@@ -106,7 +106,9 @@ def test_acoustic_energy_is_conserved_on_gpu():
 
 
 TENSOR_CASES = [([4, 4], 2, True), ([4, 3], 3, False), ([3, 3, 3], 1, True), ([3, 3, 3], 2, False),
-                ([3, 4, 3], 3, True), ([3, 3, 3], 4, False)]
+                ([3, 4, 3], 3, True), ([3, 3, 3], 4, False),
+                # grids whose rows are whole blocks and whose 8 rows split over the XCDs: banded block order
+                ([4, 8, 3], 3, True), ([2, 8, 2], 4, False)]
 
 
 @pytest.mark.parametrize("n,p,periodic", TENSOR_CASES)
