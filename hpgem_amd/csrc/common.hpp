@@ -28,6 +28,26 @@ struct Failure : std::runtime_error {
             throw ::hgdg::Failure(HGDG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(err_)); \
     } while (0)
 
+// Two HIP events for timing, destroyed on every exit (also when a HIP call
+// between creation and the end of the scope throws).
+struct EventPair {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    EventPair() {
+        HGDG_HIP(hipEventCreate(&e[0]));
+        const hipError_t err = hipEventCreate(&e[1]);
+        if (err != hipSuccess) {
+            (void)hipEventDestroy(e[0]);
+            throw Failure(HGDG_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(err));
+        }
+    }
+    ~EventPair() {
+        (void)hipEventDestroy(e[0]);
+        (void)hipEventDestroy(e[1]);
+    }
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+};
+
 constexpr int kMaxIn = 8;
 constexpr int kBlock = 256;
 

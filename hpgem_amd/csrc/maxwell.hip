@@ -372,9 +372,8 @@ void maxwell_assemble(hgdg_ctx* c, int order, const double* eps, double stab, in
     p.K = d_K.ptr;
     p.M = d_M.ptr;
     p.mesh_of = d_mesh_of.ptr;
-    hipEvent_t t0, t1;
-    HGDG_HIP(hipEventCreate(&t0));
-    HGDG_HIP(hipEventCreate(&t1));
+    EventPair ev;
+    hipEvent_t t0 = ev.e[0], t1 = ev.e[1];
     HGDG_HIP(hipEventRecord(t0, c->stream));
     hipLaunchKernelGGL(maxwell_assemble_kernel, dim3(unsigned(ne)), dim3(kBlock), 0, c->stream, p);
     HGDG_HIP(hipGetLastError());
@@ -385,8 +384,6 @@ void maxwell_assemble(hgdg_ctx* c, int order, const double* eps, double stab, in
     float ms = 0;
     HGDG_HIP(hipEventElapsedTime(&ms, t0, t1));
     c->last_kernel_ms = double(ms);
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
 }
 
 }  // namespace hgdg

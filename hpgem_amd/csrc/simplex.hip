@@ -908,15 +908,7 @@ void simplex_tune(hgdg_ctx* c) {
     io.in_f32 = mixed ? 2u : 0u;
     io.out = out.ptr;
     io.out_f32 = mixed ? 1 : 0;
-    struct Events {
-        hipEvent_t e[2] = {nullptr, nullptr};
-        ~Events() {
-            for (hipEvent_t x : e)
-                if (x) (void)hipEventDestroy(x);
-        }
-    } ev;
-    HGDG_HIP(hipEventCreate(&ev.e[0]));
-    HGDG_HIP(hipEventCreate(&ev.e[1]));
+    EventPair ev;
     constexpr int kReps = 10;
     for (int g = 0; g < 4; ++g) {
         if (sm.tpe_fixed[g] != 0 || sm.group_start[g + 1] == sm.group_start[g]) continue;

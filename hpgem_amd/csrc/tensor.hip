@@ -1515,9 +1515,8 @@ void tensor_tune(hgdg_ctx* c) {
     io.out = out.ptr;
     io.out_f32 = mixed ? 1 : 0;
     const RhsParams p = advection_params(c, io);
-    hipEvent_t ev[2];
-    HGDG_HIP(hipEventCreate(&ev[0]));
-    HGDG_HIP(hipEventCreate(&ev[1]));
+    EventPair events;
+    hipEvent_t* ev = events.e;
     constexpr int kCand[4] = {1, 2, 4, 8};
     constexpr int kReps = 20;
     constexpr int n_cand = 4;
@@ -1540,8 +1539,6 @@ void tensor_tune(hgdg_ctx* c) {
             best = c->adv_tile_ms[i];
             best_tiles = kCand[i];
         }
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
     HGDG_HIP(hipStreamSynchronize(c->stream));
     c->adv_tiles = best_tiles;
 }
