@@ -9,7 +9,8 @@
 // It follows the reference's own algorithm, function by function:
 //   * Gauss rules (kernel/Integration/QuadratureRules/*): tensor Gauss-Legendre
 //     for line/square/cube (the reference's literal tables agree with these to
-//     <=1e-15); simplex rules are supplied as data (see orc_set_simplex_rule).
+//     <=1e-15); simplex rules are supplied as data (see orc_set_simplex_rule),
+//     pinned by the closed-form monomial moments of the reference simplex.
 //   * H1 hierarchical DG bases in the reference's own ordering
 //     (kernel/FE/BasisFunctions2DH1ConformingSquare.cpp:164,
 //      BasisFunctions3DH1ConformingCube.cpp:434,
