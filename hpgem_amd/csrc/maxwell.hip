@@ -44,7 +44,6 @@ struct RefTetRule {
 };
 #include "tet_rules.inc"
 
-constexpr int kMaxHcurl = 30;  // Ainsworth-Coyle functions at order 2
 
 struct MaxwellParams {
     int64_t n_elem;

@@ -64,36 +64,6 @@ struct RefTetRule {
 #include "tet_rules.inc"
 static_assert(kFaceQ == 16, "the table index reserves 16 face points");
 
-// Inverse of a small SPD matrix (Gauss-Jordan with pivoting, long double).
-std::vector<long double> inverse_ld(const std::vector<double>& A, int n) {
-    std::vector<long double> M(A.begin(), A.end()), inv(size_t(n) * n, 0.0L);
-    for (int i = 0; i < n; ++i) inv[size_t(i) * n + i] = 1.0L;
-    for (int c = 0; c < n; ++c) {
-        int piv = c;
-        for (int r = c + 1; r < n; ++r)
-            if (fabsl(M[size_t(r) * n + c]) > fabsl(M[size_t(piv) * n + c])) piv = r;
-        for (int k = 0; k < n; ++k) {
-            std::swap(M[size_t(c) * n + k], M[size_t(piv) * n + k]);
-            std::swap(inv[size_t(c) * n + k], inv[size_t(piv) * n + k]);
-        }
-        const long double d = M[size_t(c) * n + c];
-        for (int k = 0; k < n; ++k) {
-            M[size_t(c) * n + k] /= d;
-            inv[size_t(c) * n + k] /= d;
-        }
-        for (int r = 0; r < n; ++r) {
-            if (r == c) continue;
-            const long double f = M[size_t(r) * n + c];
-            if (f == 0.0L) continue;
-            for (int k = 0; k < n; ++k) {
-                M[size_t(r) * n + k] -= f * M[size_t(c) * n + k];
-                inv[size_t(r) * n + k] -= f * inv[size_t(c) * n + k];
-            }
-        }
-    }
-    return inv;
-}
-
 }  // namespace
 
 struct SimplexParams {
