@@ -433,7 +433,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the operator shards by element partition with a face-trace halo; see DESIGN.md)",
+              "(the operator would shard by element partition with a face-trace halo exchange)",
               file=sys.stderr)
         sys.exit(2)
 
