@@ -97,32 +97,36 @@ __host__ __device__ constexpr int simplex_tpe(int p) { return simplex_nb(p) > 32
 __host__ __device__ constexpr int simplex_tpe_alt(int p) { return p == 1 ? 32 : 64; }
 __host__ __device__ constexpr int simplex_tpe_of(int p, int variant) { return variant ? simplex_tpe_alt(p) : simplex_tpe(p); }
 
-template <int P, int TPE_ = simplex_tpe(P)>
+template <int P, int TPE_ = simplex_tpe(P), int TB = 8>
 struct SimplexShape {
     static_assert(TPE_ >= simplex_nb(P) && TPE_ <= 256, "a row phase needs one thread per basis function");
     static constexpr int NB = simplex_nb(P);
     static constexpr int NCOL = simplex_ncol(P);
     static constexpr int TPE = TPE_;  // threads per element
     static constexpr int EPB = kBlock / TPE;
-    // LDS carve-up of the RHS kernel (in doubles; T-typed regions use the
-    // first half of theirs in FP32). FM, FI and U live through the whole
-    // kernel; the work area W holds in turn the volume data (G, DR) and the
-    // face data (SO, SN, SF).
+    // doubles taken by n values of the arithmetic type (TB bytes), rounded
+    // up to an even count (regions start double2-aligned)
+    static constexpr int tw(int n) { return (n * TB + 15) / 16 * 2; }
+    // LDS carve-up of the RHS kernel, in doubles. FM, FI and U live through
+    // the whole kernel; the work area W holds in turn the volume data (G, DR)
+    // and the face data (SO, SN, SF); LIFT follows.
     static constexpr int FM = 0;                            // [EPB][4 faces][4]: unit normal, sigma
     static constexpr int FI = FM + EPB * 16;                // [EPB][4 faces] (int2): neighbour, packed face info
     static constexpr int U = FI + EPB * 4;                  // [EPB][4][NB] stage input
     static constexpr int W = U + EPB * 4 * NB;
     static constexpr int G = W;                             // [EPB][4][NB] (T): v, then g_r = sum_d (J^{-1})_{r,d} s_d
-    static constexpr bool DR_LDS = P < 4;                   // DrT staged in LDS (order 4: read through the caches)
-    static constexpr int DR = G + EPB * 4 * NB;             // [3][NB][NB] (T): DrT
-    static constexpr int VOL_END = DR + (DR_LDS ? 3 * NB * NB : 0);
+    // DrT staged in LDS below order 4, and at order 4 in FP32 (in FP64 its
+    // 29 KB are read through the caches)
+    static constexpr bool DR_LDS = P < 4 || TB == 4;
+    static constexpr int DR = G + tw(EPB * 4 * NB);         // [3][NB][NB] (T): DrT
+    static constexpr int VOL_END = DR + (DR_LDS ? tw(3 * NB * NB) : 0);
     static constexpr int SO = W;                            // [EPB][4 faces][NCOL] (double2): n.s and v at own face columns
     static constexpr int SN = SO + EPB * 8 * NCOL;          // [EPB][4 faces][kMaxCols] (double2): the same for the neighbours
     static constexpr int SF = SN + EPB * 8 * kMaxCols;      // [EPB][4 faces][NCOL] (T pairs): flux coefficients
-    static constexpr int FACE_END = SF + EPB * 8 * NCOL;
+    static constexpr int FACE_END = SF + tw(EPB * 8 * NCOL);
     static constexpr bool LIFT_LDS = P < 4;                 // lift table staged in LDS (order 4: read through the caches)
     static constexpr int LIFT = FACE_END > VOL_END ? FACE_END : VOL_END;  // [4 f][NCOL m][NB i] (T), the whole kernel
-    static constexpr int TOTAL = LIFT + (LIFT_LDS ? 4 * NCOL * NB : 0);
+    static constexpr int TOTAL = LIFT + (LIFT_LDS ? tw(4 * NCOL * NB) : 0);
     static_assert(FI % 2 == 0 && U % 2 == 0 && W % 2 == 0 && SN % 2 == 0 && SF % 2 == 0, "double2 alignment");
 };
 
@@ -170,7 +174,7 @@ __device__ __forceinline__ void dot_pairs(const double* col, const double2* x, d
 template <int P, int TPE, typename T>
 __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int32_t* tix, int64_t first, int count,
                                               double* lds) {
-    using S = SimplexShape<P, TPE>;
+    using S = SimplexShape<P, TPE, int(sizeof(T))>;
     constexpr int NB = S::NB;
     constexpr int NC = S::NCOL;
     const int tid = threadIdx.x;
@@ -387,11 +391,12 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
     }
 }
 
+template <int TB>
 constexpr int simplex_lds_doubles() {
-    constexpr int t[8] = {SimplexShape<1>::TOTAL, SimplexShape<2>::TOTAL, SimplexShape<3>::TOTAL,
-                          SimplexShape<4>::TOTAL, SimplexShape<1, simplex_tpe_alt(1)>::TOTAL,
-                          SimplexShape<2, simplex_tpe_alt(2)>::TOTAL, SimplexShape<3, simplex_tpe_alt(3)>::TOTAL,
-                          SimplexShape<4, simplex_tpe_alt(4)>::TOTAL};
+    constexpr int t[8] = {SimplexShape<1, simplex_tpe(1), TB>::TOTAL,     SimplexShape<2, simplex_tpe(2), TB>::TOTAL,
+                          SimplexShape<3, simplex_tpe(3), TB>::TOTAL,     SimplexShape<4, simplex_tpe(4), TB>::TOTAL,
+                          SimplexShape<1, simplex_tpe_alt(1), TB>::TOTAL, SimplexShape<2, simplex_tpe_alt(2), TB>::TOTAL,
+                          SimplexShape<3, simplex_tpe_alt(3), TB>::TOTAL, SimplexShape<4, simplex_tpe_alt(4), TB>::TOTAL};
     int m = 0;
     for (int k = 0; k < 8; ++k) m = t[k] > m ? t[k] : m;
     return m;
@@ -406,7 +411,7 @@ constexpr int simplex_lds_doubles() {
 // for five blocks per CU; the register budget is set to match.
 template <typename T, int V>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void simplex_acoustic_rhs(const SimplexParams p) {
-    __shared__ __align__(16) double lds[simplex_lds_doubles()];  // regions start at even doubles: double2-aligned
+    __shared__ __align__(16) double lds[simplex_lds_doubles<int(sizeof(T))>()];  // regions start at even doubles
     __shared__ int32_t tix[kTixMax];                              // table offsets, orientations, face column lists
     for (int k = threadIdx.x; k < p.n_tix; k += kBlock) tix[k] = p.tix[k];
     const int4 bl = p.blocks[xcd_block(blockIdx.x, gridDim.x)];
