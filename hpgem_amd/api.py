@@ -217,6 +217,20 @@ class HpgemAPIDevice:
         check(self.lib.hgdg_get_advection_tiles(self.h, ctypes.byref(t), dptr(ms)))
         return t.value, ms
 
+    # ---- single-launch RK steps for meshes that fit in LDS -----------------
+    def setFusedTimeStepping(self, enable):
+        """True (default): computeOneTimeStep may run all its steps in one
+        single-workgroup launch when the state fits in LDS (small advection
+        meshes); False: always replay the captured step graph."""
+        check(self.lib.hgdg_set_rk_fused(self.h, 1 if enable else 0))
+
+    def getFusedTimeStepping(self):
+        """(enabled, single-workgroup RK launches made so far)."""
+        e = ctypes.c_int()
+        n = ctypes.c_int64()
+        check(self.lib.hgdg_get_rk_fused(self.h, ctypes.byref(e), ctypes.byref(n)))
+        return bool(e.value), n.value
+
     # ---- threads per element of the tetrahedral kernel ---------------------
     def setSimplexThreads(self, threads):
         """Threads per element for orders 1..4 (four values): each 0 to time

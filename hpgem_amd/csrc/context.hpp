@@ -84,6 +84,8 @@ struct hgdg_ctx {
     // no prefetch); chosen by timing at operator set-up unless fixed by the caller
     int adv_tiles = 1;
     int adv_tiles_fixed = 0;
+    int rk_fused = 1;  // RK steps of meshes that fit in LDS in one single-workgroup launch (0: always the graph)
+    int64_t rk_fused_launches = 0;
     double adv_tile_ms[4] = {0, 0, 0, 0};  // timed candidates 1, 2, 4, 8 (0 = not timed)
 
     bool simplex() const { return shape == HGDG_SHAPE_TETRAHEDRON; }
@@ -111,6 +113,10 @@ void tensor_project_device(hgdg_ctx* c, Vec& v, int field, int n_modes, const do
                            const double* gauss = nullptr);
 void tensor_operator_set(hgdg_ctx* c, const hgdg_op_desc* op);
 void tensor_rhs(hgdg_ctx* c, const StageIO& io);
+// n_steps RK steps in one single-workgroup launch when the mesh is small
+// enough for LDS (tensor advection); false: not eligible, nothing launched
+bool tensor_rk_tiny(hgdg_ctx* c, int solution, const int* aux, int stages, const double* a, const double* b,
+                    double dt, int n_steps);
 // Chooses c->adv_tiles for the advection operator by timing each candidate
 // on scratch vectors of the mesh's size (or takes c->adv_tiles_fixed).
 void tensor_tune(hgdg_ctx* c);

@@ -267,6 +267,7 @@ void rk_steps(hgdg_ctx* c, int solution, const int* aux, int stages, const doubl
               int n_steps) {
     require(n_steps >= 0, HGDG_ERR_ARG, "negative step count");
     check_rk_args(c, solution, aux, stages);
+    if (tensor_rk_tiny(c, solution, aux, stages, a, b, dt, n_steps)) return;
     GraphKey key{solution, std::vector<int>(aux, aux + stages), std::vector<double>(a, a + stages * stages),
                  std::vector<double>(b, b + stages), dt};
     if (!(c->graph_valid && c->graph_key == key)) {
@@ -529,6 +530,22 @@ int hgdg_get_advection_tiles(hgdg_ctx* ctx, int* tiles, double* ms) {
         *tiles = ctx->adv_tiles;
         if (ms)
             for (int i = 0; i < 4; ++i) ms[i] = ctx->adv_tile_ms[i];
+    });
+}
+
+int hgdg_set_rk_fused(hgdg_ctx* ctx, int enable) {
+    return run([&] {
+        require(ctx != nullptr, HGDG_ERR_ARG, "ctx is NULL");
+        require(enable == 0 || enable == 1, HGDG_ERR_ARG, "enable must be 0 or 1");
+        ctx->rk_fused = enable;
+    });
+}
+
+int hgdg_get_rk_fused(hgdg_ctx* ctx, int* enable, int64_t* launches) {
+    return run([&] {
+        require(ctx != nullptr && enable != nullptr, HGDG_ERR_ARG, "ctx and enable must be non-NULL");
+        *enable = ctx->rk_fused;
+        if (launches) *launches = ctx->rk_fused_launches;
     });
 }
 

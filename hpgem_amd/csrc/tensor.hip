@@ -364,6 +364,103 @@ __device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk)
     advection_finish<DIM, N1, T>(p, blk, w, meta, s_w, s_D, s_Mi);
 }
 
+// Whole RK steps of a small advection mesh in one workgroup: the solution,
+// the stage input, every stage derivative and the per-element face data stay
+// in LDS for all n_steps steps, with workgroup barriers between the phases,
+// so one launch replaces 5 n_steps launches (meshes this small are bound by
+// launch latency, not by HBM). The arithmetic is the graph path's, operation
+// for operation: the stage input as combine() forms it, the RHS through
+// advection_compute with the neighbour traces read from LDS, the update as
+// multi_axpy forms it; results are bitwise the same (tested).
+// The one workgroup: 16 wavefronts where the kernel fits in 128 registers,
+// else 8 (256 registers).
+template <int DIM, int N1>
+constexpr int rk_tiny_block() {
+    return (DIM == 2 ? N1 <= 4 : N1 <= 3) ? 1024 : 512;
+}
+
+struct RkTinyParams {
+    RhsParams rhs;          // geometry, tables; io unused
+    double* u;              // [n_dof] solution (read, then written after the last step)
+    void* k[kMaxIn];        // stage vectors (written with the last step's derivatives)
+    int32_t stages;
+    int32_t n_steps;
+    double ca[kMaxIn * kMaxIn];  // dt a_ij, row i (0: the term is skipped, as the graph path skips it)
+    double cb[kMaxIn];           // dt b_j
+};
+
+template <int DIM, int N1, typename T>
+__global__ __launch_bounds__((rk_tiny_block<DIM, N1>())) void tensor_advection_rk_tiny(const RkTinyParams p) {
+    constexpr int kTinyBlock = rk_tiny_block<DIM, N1>();
+    using S = Pencil<DIM, N1>;
+    extern __shared__ __align__(16) double dyn[];
+    __shared__ T s_D[N1 * N1];
+    __shared__ T s_Mi[2 * N1];
+    const int64_t E = p.rhs.n_elem, n = E * S::NB;
+    const int st = p.stages;
+    double* su = dyn;                                  // [n] solution
+    double* sw = su + n;                               // [n] stage input
+    uint32_t* smd = reinterpret_cast<uint32_t*>(sw + n);  // [E] upwind modes
+    int32_t* snb = reinterpret_cast<int32_t*>(smd + E);   // [2 DIM][E] neighbours
+    double* sah = reinterpret_cast<double*>(snb + 2 * DIM * E + ((E * (1 + 2 * DIM)) & 1));  // [DIM][E]
+    T* sk = reinterpret_cast<T*>(sah + DIM * E);       // [stages][n] stage derivatives
+    advection_tables<DIM, N1, T>(p.rhs, s_D, s_Mi);
+    for (int64_t i = threadIdx.x; i < n; i += kTinyBlock) su[i] = p.u[i];
+    for (int64_t e = threadIdx.x; e < E; e += kTinyBlock) {
+        smd[e] = p.rhs.mode[e];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) sah[d * E + e] = p.rhs.ahat[d * E + e];
+#pragma unroll
+        for (int f = 0; f < 2 * DIM; ++f) snb[f * E + e] = p.rhs.nbr[f * E + e];
+    }
+    __syncthreads();
+    RhsParams pp = p.rhs;
+    pp.io.out_f32 = sizeof(T) == 4 ? 1 : 0;
+    for (int step = 0; step < p.n_steps; ++step) {
+        for (int i = 0; i < st; ++i) {
+            // stage input: 1 * u, then + (dt a_ij) k_j in order of j
+            for (int64_t q = threadIdx.x; q < n; q += kTinyBlock) {
+                double v = 1.0 * su[q];
+                for (int j = 0; j < i; ++j)
+                    if (p.ca[i * kMaxIn + j] != 0.0) v += p.ca[i * kMaxIn + j] * double(sk[int64_t(j) * n + q]);
+                sw[q] = v;
+            }
+            __syncthreads();
+            pp.io.out = sk + int64_t(i) * n;
+            for (int64_t q = threadIdx.x; q < E * S::NP; q += kTinyBlock) {
+                const int64_t e = q / S::NP;
+                const int pc = int(q % S::NP);
+                double w[N1];
+#pragma unroll
+                for (int a = 0; a < N1; ++a) w[a] = sw[e * S::NB + pc * N1 + a];
+                double ah[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) ah[d] = sah[d * E + e];
+                auto xo = [&](int s) -> double { return sw[int64_t(snb[s * E + e]) * S::NB + pc * N1 + (1 - s)]; };
+                auto yz = [&](int, int, int f, int nbr_pc, double (&other)[N1]) {
+                    const int64_t nb = snb[f * E + e];
+#pragma unroll
+                    for (int a = 0; a < N1; ++a) other[a] = sw[nb * S::NB + nbr_pc * N1 + a];
+                };
+                advection_compute<DIM, N1, T>(pp, e, pc, w, smd[e], ah, sw + e * S::NB, s_D, s_Mi, xo, yz);
+            }
+            __syncthreads();
+        }
+        // update: u + (dt b_j) k_j in order of j
+        for (int64_t q = threadIdx.x; q < n; q += kTinyBlock) {
+            double y = su[q];
+            for (int j = 0; j < st; ++j)
+                if (p.cb[j] != 0.0) y += p.cb[j] * double(sk[int64_t(j) * n + q]);
+            su[q] = y;
+        }
+        __syncthreads();
+    }
+    for (int64_t q = threadIdx.x; q < n; q += kTinyBlock) {
+        p.u[q] = su[q];
+        for (int j = 0; j < st; ++j) static_cast<T*>(p.k[j])[q] = sk[int64_t(j) * n + q];
+    }
+}
+
 // Minimum waves per SIMD the compiler is asked to keep for the one-tile
 // kernel: 3D p=3 in FP64 needs a few registers beyond the 72 of seven waves
 // otherwise, and fits them without spilling.
@@ -1541,6 +1638,57 @@ void tensor_tune(hgdg_ctx* c) {
         }
     HGDG_HIP(hipStreamSynchronize(c->stream));
     c->adv_tiles = best_tiles;
+}
+
+// Largest dynamic LDS the fused RK kernel asks for (MI355X: 160 KB per
+// workgroup; the kernel's tables take a few hundred bytes of static LDS).
+constexpr size_t kRkTinyMaxLds = 150 * 1024;
+
+bool tensor_rk_tiny(hgdg_ctx* c, int solution, const int* aux, int stages, const double* a, const double* b,
+                    double dt, int n_steps) {
+    if (c->rk_fused == 0 || c->simplex() || c->op.kind != HGDG_OP_ADVECTION_UPWIND || n_steps < 1) return false;
+    const bool mixed = c->op.precision == HGDG_PREC_MIXED_FP32;
+    const int64_t E = c->n_elem, n = c->n_dof();
+    const size_t tb = mixed ? 4 : 8;
+    const size_t bytes = size_t(16 * n) + size_t(4 * E * (1 + 2 * c->dim) + 4) + size_t(8 * c->dim * E) +
+                         size_t(stages) * size_t(n) * tb;
+    if (bytes > kRkTinyMaxLds) return false;
+    RkTinyParams p{};
+    StageIO io{};
+    p.rhs = advection_params(c, io);
+    Vec& u = vec_at(c, solution);
+    p.u = u.f64->ptr;
+    for (int j = 0; j < stages; ++j) {
+        Vec& k = vec_at(c, aux[j]);
+        require(k.is_f32 == mixed, HGDG_ERR_ARG, "stage vector storage does not match the operator's precision");
+        p.k[j] = k.ptr();
+    }
+    p.stages = stages;
+    p.n_steps = n_steps;
+    for (int i = 0; i < stages; ++i)
+        for (int j = 0; j < i; ++j) p.ca[i * kMaxIn + j] = dt * a[i * stages + j];
+    for (int j = 0; j < stages; ++j) p.cb[j] = dt * b[j];
+    bool launched = false;
+    dispatch_tensor(c->dim, c->tm.n1, [&](auto D, auto N) {
+        constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
+        const void* fn = mixed ? reinterpret_cast<const void*>(&tensor_advection_rk_tiny<DIMc, N1c, float>)
+                               : reinterpret_cast<const void*>(&tensor_advection_rk_tiny<DIMc, N1c, double>);
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)) != hipSuccess) {
+            (void)hipGetLastError();  // this device cannot give the LDS: the graph path runs instead
+            return;
+        }
+        constexpr unsigned kThreads = unsigned(rk_tiny_block<DIMc, N1c>());
+        if (mixed)
+            hipLaunchKernelGGL((tensor_advection_rk_tiny<DIMc, N1c, float>), dim3(1), dim3(kThreads), bytes, c->stream,
+                               p);
+        else
+            hipLaunchKernelGGL((tensor_advection_rk_tiny<DIMc, N1c, double>), dim3(1), dim3(kThreads), bytes, c->stream,
+                               p);
+        HGDG_HIP(hipGetLastError());
+        launched = true;
+        ++c->rk_fused_launches;
+    });
+    return launched;
 }
 
 void tensor_rhs(hgdg_ctx* c, const StageIO& io) {

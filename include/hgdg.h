@@ -127,6 +127,17 @@ int hgdg_set_advection_tiles(hgdg_ctx* ctx, int tiles);
  * 1, 2, 4, 8 (0 where not timed). ms may be NULL. */
 int hgdg_get_advection_tiles(hgdg_ctx* ctx, int* tiles, double* ms);
 
+/* Whether hgdg_rk_steps may run all its steps in one single-workgroup launch
+ * when the whole state fits in LDS (tensor advection meshes of a few thousand
+ * degrees of freedom; 1, the default) or always replays the captured step
+ * graph (0). Results are bitwise the same. No counterpart in the reference
+ * (a launch choice of this implementation). */
+int hgdg_set_rk_fused(hgdg_ctx* ctx, int enable);
+
+/* The setting above and how many single-workgroup RK launches the context
+ * has made. launches may be NULL. */
+int hgdg_get_rk_fused(hgdg_ctx* ctx, int* enable, int64_t* launches);
+
 /* Threads per element of the tetrahedral RHS kernel, per order 1..4
  * (threads[p-1]): 0 (the default) lets hgdg_operator_set time both candidates
  * of that order on the mesh and keep the faster; otherwise one of the two

@@ -410,3 +410,43 @@ def test_adaptive_step_rejects_until_the_error_bound_holds():
     print(f"adaptive DP: {len(dts)} steps, dt {dts}, L2 error {l2:.3e} (projection error {l2_0:.3e})")
     assert l2 < 1.5 * l2_0  # time error negligible next to the spatial one
     dev.close()
+
+
+FUSED_CASES = [
+    # (cells, p, velocity, periodic, precision, tableau, steps)
+    ([16, 16], 2, [1.0, 0.5], True, "fp64", "RK4", 20),   # the 2D quad p=2 workload
+    ([16, 16], 2, [1.0, 0.5], True, "mixed", "RK4", 20),
+    ([4, 4], 2, [1.0, 0.5], False, "fp64", "RK4", 7),     # walls: inflow / outflow / no-flux faces
+    ([5, 3], 3, [-0.4, 1.0], True, "fp64", "RK3_TVD", 9),
+    ([3, 3], 1, [1.0, 0.3], True, "fp64", "DORMAND_PRINCE", 6),  # seven stages, zero tableau entries
+    ([3, 3, 3], 2, [1.0, 0.7, 0.3], True, "fp64", "RK4", 5),
+    ([3, 3, 3], 2, [1.0, -0.7, 0.3], False, "mixed", "RK4", 5),
+]
+
+
+@pytest.mark.parametrize("n,p,a,periodic,precision,tab,steps", FUSED_CASES)
+def test_fused_rk_steps_bitwise_equal_graph_replay(n, p, a, periodic, precision, tab, steps):
+    """RK steps of a mesh whose state fits in LDS run in one single-workgroup
+    launch that repeats the graph path's arithmetic operation for operation:
+    the solution and every stage vector come out bitwise the same as from the
+    replayed step graph."""
+    print(f"[case] fused RK n={n} p={p} {precision} {tab} steps={steps}", flush=True)
+    tableau = getattr(hpgem_amd.tableaux, tab)
+    mesh = Mesh.structured(n, periodic=[int(periodic)] * len(n))
+    u0 = None
+    out = {}
+    for fused in (False, True):
+        dev = HpgemAPIDevice(mesh, p, velocity=a, precision=precision, tableau=tableau, n_extra_vectors=2)
+        dev.setFusedTimeStepping(fused)
+        if u0 is None:
+            u0 = seeded_state(dev.n_dof, 5)
+        dev.setTimeIntegrationVector(0, u0)
+        h = 1.0 / max(n)
+        dev.computeOneTimeStep(0.0, 0.05 * h / np.linalg.norm(a), steps)
+        enabled, launches = dev.getFusedTimeStepping()
+        assert enabled == fused and (launches > 0) == fused
+        out[fused] = [dev.getTimeIntegrationVector(v) for v in range(tableau.stages + 1)]
+        dev.close()
+    assert np.all(np.isfinite(out[True][0]))
+    for g, f in zip(out[False], out[True]):
+        np.testing.assert_array_equal(f, g)
