@@ -1643,6 +1643,7 @@ void tensor_tune(hgdg_ctx* c) {
 // Largest dynamic LDS the fused RK kernel asks for (MI355X: 160 KB per
 // workgroup; the kernel's tables take a few hundred bytes of static LDS).
 constexpr size_t kRkTinyMaxLds = 150 * 1024;
+constexpr int kRkTinyChunk = 500;  // steps per launch
 
 bool tensor_rk_tiny(hgdg_ctx* c, int solution, const int* aux, int stages, const double* a, const double* b,
                     double dt, int n_steps) {
@@ -1678,15 +1679,21 @@ bool tensor_rk_tiny(hgdg_ctx* c, int solution, const int* aux, int stages, const
             return;
         }
         constexpr unsigned kThreads = unsigned(rk_tiny_block<DIMc, N1c>());
-        if (mixed)
-            hipLaunchKernelGGL((tensor_advection_rk_tiny<DIMc, N1c, float>), dim3(1), dim3(kThreads), bytes, c->stream,
-                               p);
-        else
-            hipLaunchKernelGGL((tensor_advection_rk_tiny<DIMc, N1c, double>), dim3(1), dim3(kThreads), bytes, c->stream,
-                               p);
-        HGDG_HIP(hipGetLastError());
+        // at most kRkTinyChunk steps per launch, so that no launch runs for
+        // long; the solution goes through HBM between launches exactly (FP64)
+        for (int done = 0; done < n_steps; done += kRkTinyChunk) {
+            RkTinyParams q = p;
+            q.n_steps = std::min(kRkTinyChunk, n_steps - done);
+            if (mixed)
+                hipLaunchKernelGGL((tensor_advection_rk_tiny<DIMc, N1c, float>), dim3(1), dim3(kThreads), bytes,
+                                   c->stream, q);
+            else
+                hipLaunchKernelGGL((tensor_advection_rk_tiny<DIMc, N1c, double>), dim3(1), dim3(kThreads), bytes,
+                                   c->stream, q);
+            HGDG_HIP(hipGetLastError());
+            ++c->rk_fused_launches;
+        }
         launched = true;
-        ++c->rk_fused_launches;
     });
     return launched;
 }

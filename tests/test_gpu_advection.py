@@ -419,6 +419,7 @@ FUSED_CASES = [
     ([4, 4], 2, [1.0, 0.5], False, "fp64", "RK4", 7),     # walls: inflow / outflow / no-flux faces
     ([5, 3], 3, [-0.4, 1.0], True, "fp64", "RK3_TVD", 9),
     ([3, 3], 1, [1.0, 0.3], True, "fp64", "DORMAND_PRINCE", 6),  # seven stages, zero tableau entries
+    ([3, 3], 1, [1.0, 0.3], True, "fp64", "RK4", 1100),  # more steps than one launch takes
     ([3, 3, 3], 2, [1.0, 0.7, 0.3], True, "fp64", "RK4", 5),
     ([3, 3, 3], 2, [1.0, -0.7, 0.3], False, "mixed", "RK4", 5),
 ]
