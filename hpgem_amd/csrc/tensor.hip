@@ -385,8 +385,10 @@ struct RkTinyParams {
     void* k[kMaxIn];        // stage vectors (written with the last step's derivatives)
     int32_t stages;
     int32_t n_steps;
-    double ca[kMaxIn * kMaxIn];  // dt a_ij, row i (0: the term is skipped, as the graph path skips it)
+    double ca[kMaxIn * kMaxIn];  // dt a_ij, row i
     double cb[kMaxIn];           // dt b_j
+    uint32_t amask[kMaxIn];      // bit j of row i: a_ij != 0 (the graph path skips zero entries)
+    uint32_t bmask;              // bit j: b_j != 0
 };
 
 template <int DIM, int N1, typename T>
@@ -422,7 +424,7 @@ __global__ __launch_bounds__((rk_tiny_block<DIM, N1>())) void tensor_advection_r
             for (int64_t q = threadIdx.x; q < n; q += kTinyBlock) {
                 double v = 1.0 * su[q];
                 for (int j = 0; j < i; ++j)
-                    if (p.ca[i * kMaxIn + j] != 0.0) v += p.ca[i * kMaxIn + j] * double(sk[int64_t(j) * n + q]);
+                    if ((p.amask[i] >> j) & 1u) v += p.ca[i * kMaxIn + j] * double(sk[int64_t(j) * n + q]);
                 sw[q] = v;
             }
             __syncthreads();
@@ -450,7 +452,7 @@ __global__ __launch_bounds__((rk_tiny_block<DIM, N1>())) void tensor_advection_r
         for (int64_t q = threadIdx.x; q < n; q += kTinyBlock) {
             double y = su[q];
             for (int j = 0; j < st; ++j)
-                if (p.cb[j] != 0.0) y += p.cb[j] * double(sk[int64_t(j) * n + q]);
+                if ((p.bmask >> j) & 1u) y += p.cb[j] * double(sk[int64_t(j) * n + q]);
             su[q] = y;
         }
         __syncthreads();
@@ -1667,8 +1669,14 @@ bool tensor_rk_tiny(hgdg_ctx* c, int solution, const int* aux, int stages, const
     p.stages = stages;
     p.n_steps = n_steps;
     for (int i = 0; i < stages; ++i)
-        for (int j = 0; j < i; ++j) p.ca[i * kMaxIn + j] = dt * a[i * stages + j];
-    for (int j = 0; j < stages; ++j) p.cb[j] = dt * b[j];
+        for (int j = 0; j < i; ++j) {
+            p.ca[i * kMaxIn + j] = dt * a[i * stages + j];
+            if (a[i * stages + j] != 0.0) p.amask[i] |= 1u << j;
+        }
+    for (int j = 0; j < stages; ++j) {
+        p.cb[j] = dt * b[j];
+        if (b[j] != 0.0) p.bmask |= 1u << j;
+    }
     bool launched = false;
     dispatch_tensor(c->dim, c->tm.n1, [&](auto D, auto N) {
         constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
