@@ -118,8 +118,9 @@ struct SimplexShape {
     // DrT staged in LDS below order 4, and at order 4 in FP32 (in FP64 its
     // 29 KB are read through the caches)
     static constexpr bool DR_LDS = P < 4 || TB == 4;
+    // otherwise (FP64, order 4) one direction's table at a time, [NB][NB]
     static constexpr int DR = G + tw(EPB * 4 * NB);         // [3][NB][NB] (T): DrT
-    static constexpr int VOL_END = DR + (DR_LDS ? tw(3 * NB * NB) : 0);
+    static constexpr int VOL_END = DR + (DR_LDS ? tw(3 * NB * NB) : tw(NB * NB));
     static constexpr int SO = W;                            // [EPB][4 faces][NCOL] (double2): n.s and v at own face columns
     static constexpr int SN = SO + EPB * 8 * NCOL;          // [EPB][4 faces][kMaxCols] (double2): the same for the neighbours
     static constexpr int SF = SN + EPB * 8 * kMaxCols;      // [EPB][4 faces][NCOL] (T pairs): flux coefficients
@@ -217,9 +218,10 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
         for (int r = 0; r < 3; ++r) sg[(1 + r) * NB + i] = T(Ji[r * 3] * sv[1] + Ji[r * 3 + 1] * sv[2] + Ji[r * 3 + 2] * sv[3]);
     }
     const T* gDr = tabT + p.tix[(P - 1) * kTixStride + kTixDrT];
-    if constexpr (S::DR_LDS) {
+    {
+        constexpr int nd = S::DR_LDS ? 3 * NB * NB : NB * NB;  // all directions, or the first
 #pragma unroll 4
-        for (int k = tid; k < 3 * NB * NB; k += kBlock) sDr[k] = gDr[k];
+        for (int k = tid; k < nd; k += kBlock) sDr[k] = gDr[k];
     }
     const T* gLift = tabT + p.tix[(P - 1) * kTixStride + kTixLift];
     T* sLift = reinterpret_cast<T*>(lds + S::LIFT);
@@ -231,22 +233,45 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
 
     // ---- phase 2: volume, k_v = sum_r Dr_r g_r ; k_s_d = (1/c_inv) sum_r (J^{-1})_{r,d} Dr_r v
     Acc<T> kv, ks[3];
+    T a[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
+    if constexpr (S::DR_LDS) {
+        if (row_ok) {
+#pragma unroll 4
+            for (int j = 0; j < NB; ++j) {
+                const T v = sg[j];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const T d = sDr[(r * NB + j) * NB + i];
+                    a[r] += d * sg[(1 + r) * NB + j];
+                    b[r] += d * v;
+                }
+            }
+        }
+    } else {
+        // one direction at a time through LDS (every thread of the block
+        // takes part in the barriers and loads); the sums over j run in the
+        // same order as above
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            if (r > 0) {
+                __syncthreads();  // the block is done with direction r - 1
+                for (int k = tid; k < NB * NB; k += kBlock) sDr[k] = gDr[r * NB * NB + k];
+                __syncthreads();
+            }
+            if (row_ok) {
+#pragma unroll 5
+                for (int j = 0; j < NB; ++j) {
+                    const T d = sDr[j * NB + i];
+                    a[r] += d * sg[(1 + r) * NB + j];
+                    b[r] += d * sg[j];
+                }
+            }
+        }
+    }
     if (row_ok) {
         T Jt[9];  // J^{-1} again (not kept in registers across the barrier)
 #pragma unroll
         for (int q = 0; q < 9; ++q) Jt[q] = T(p.jinv[q * E + e]);
-        T a[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
-        const T* Dr = S::DR_LDS ? sDr : gDr;
-#pragma unroll 4
-        for (int j = 0; j < NB; ++j) {
-            const T v = sg[j];
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                const T d = Dr[(r * NB + j) * NB + i];
-                a[r] += d * sg[(1 + r) * NB + j];
-                b[r] += d * v;
-            }
-        }
         kv = Acc<T>(a[0] + a[1] + a[2]);
         const T ci = T(1.0 / cinv);
 #pragma unroll
