@@ -125,7 +125,10 @@ struct SimplexShape {
     static constexpr int SN = SO + EPB * 8 * NCOL;          // [EPB][4 faces][kMaxCols] (double2): the same for the neighbours
     static constexpr int SF = SN + EPB * 8 * kMaxCols;      // [EPB][4 faces][NCOL] (T pairs): flux coefficients
     static constexpr int FACE_END = SF + tw(EPB * 8 * NCOL);
-    static constexpr bool LIFT_LDS = P < 4;                 // lift table staged in LDS (order 4: read through the caches)
+    static constexpr bool LIFT_LDS = P < 4;                 // lift table staged in LDS for the whole kernel
+    // order 4 in FP64: one face's lift table at a time in the stage-input
+    // area (in FP32 it is read through the caches, which measured faster)
+    static constexpr bool LIFT_ONE = !LIFT_LDS && TB == 8;
     static constexpr int LIFT = FACE_END > VOL_END ? FACE_END : VOL_END;  // [4 f][NCOL m][NB i] (T), the whole kernel
     static constexpr int TOTAL = LIFT + (LIFT_LDS ? tw(4 * NCOL * NB) : 0);
     static_assert(FI % 2 == 0 && U % 2 == 0 && W % 2 == 0 && SN % 2 == 0 && SF % 2 == 0, "double2 alignment");
@@ -381,19 +384,38 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
             sf[f * NC + m] = FluxPair<T>{T(Fv), T(Fs)};
         }
     }
+    T* sL1 = reinterpret_cast<T*>(lds + S::U);  // one face's lift table at order 4
+    if constexpr (S::LIFT_ONE) {
+        static_assert(NC * NB * sizeof(T) <= sizeof(double) * S::EPB * 4 * NB, "a face's lift table fits the stage-input area");
+        for (int k = tid; k < NC * NB; k += kBlock) sL1[k] = gLift[k];
+    }
     __syncthreads();
 
     // ---- phase 5: k += sigma_f sum_m (M_ref^{-1} int_f phi chi_m)_i F_m
-    if (row_ok) {
-        const T ci = T(1.0 / cinv);
-        const T* lift = (S::LIFT_LDS ? sLift : gLift) + i;
+    // (order 4 in FP64: the lift table passes through the stage-input area,
+    // dead since phase 3, one face at a time; face 0's was loaded in phase 4)
+    const T ci = T(1.0 / cinv);
 #pragma unroll 1
-        for (int f = 0; f < 4; ++f) {
+    for (int f = 0; f < 4; ++f) {
+        const T* lift;
+        if constexpr (S::LIFT_LDS) {
+            lift = sLift + f * NC * NB + i;
+        } else if constexpr (!S::LIFT_ONE) {
+            lift = gLift + f * NC * NB + i;
+        } else {
+            if (f > 0) {
+                __syncthreads();  // the block is done with face f - 1
+                for (int k = tid; k < NC * NB; k += kBlock) sL1[k] = gLift[f * NC * NB + k];
+                __syncthreads();
+            }
+            lift = sL1 + i;
+        }
+        if (row_ok) {
             const FluxPair<T>* fl = sf + f * NC;
             T lv = T(0), ls = T(0);
 #pragma unroll 5
             for (int m = 0; m < NC; ++m) {
-                const T l = lift[(f * NC + m) * NB];
+                const T l = lift[m * NB];
                 const FluxPair<T> x = fl[m];
                 lv += l * x.v;
                 ls += l * x.s;
@@ -405,6 +427,8 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
             ks[1].add(c * T(fm[f * 4 + 1]));
             ks[2].add(c * T(fm[f * 4 + 2]));
         }
+    }
+    if (row_ok) {
         const double vals[4] = {double(kv.value()), double(ks[0].value()), double(ks[1].value()),
                                 double(ks[2].value())};
 #pragma unroll
