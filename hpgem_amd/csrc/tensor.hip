@@ -677,9 +677,12 @@ __device__ __forceinline__ void acoustic_block_quad(const AcousticParams& p, int
     // ---- faces: x faces one value per pencil, y/z faces a whole pencil.
     // The neighbours' traces of all faces first, every global load issued
     // before any is used (a neighbour in the block is read from LDS; a face
-    // that loads nothing points at the thread's own pencil).
+    // that loads nothing points at the thread's own pencil). In the FP32
+    // instance at p = 3 the registers that holds cost more occupancy than
+    // the shorter wait gains (measured), so there each face loads its own.
+    constexpr bool kHoist = !(N1 == 4 && sizeof(T) == 4);
     double oth[2 * DIM][N1];
-    {
+    if constexpr (kHoist) {
         int64_t at[2 * DIM];
 #pragma unroll
         for (int f = 0; f < 2 * DIM; ++f) {
@@ -714,6 +717,22 @@ __device__ __forceinline__ void acoustic_block_quad(const AcousticParams& p, int
         double jmp[N1];
 #pragma unroll
         for (int a = 0; a < N1; ++a) jmp[a] = 0.0;
+        if constexpr (!kHoist) {
+            if (lane && inner) {
+                const int nbr_pc = df == 0 ? pc : df == 1 ? (1 - s) + N1 * ic : ib + N1 * (1 - s);
+                const int noff = df == 0 ? 1 - s : 0;
+                const int64_t lnb = int64_t(nbi[f]) - blk * S::EPB;
+                if (lnb >= 0 && lnb < S::EPB) {
+#pragma unroll
+                    for (int a = 0; a < N1; ++a)
+                        if (a < na) oth[f][a] = s_w[(lnb * NF + fld) * NB + nbr_pc * N1 + noff + a];
+                } else if (df == 0) {
+                    oth[f][0] = combine(p.io, int64_t(nbi[f]) * NF * NB + fld * NB + nbr_pc * N1 + noff);
+                } else {
+                    combine_pencil<N1>(p.io, int64_t(nbi[f]) * NF * NB + fld * NB + nbr_pc * N1, oth[f]);
+                }
+            }
+        }
         if (lane) {
 #pragma unroll
             for (int a = 0; a < N1; ++a)
