@@ -677,10 +677,10 @@ __device__ __forceinline__ void acoustic_block_quad(const AcousticParams& p, int
     // ---- faces: x faces one value per pencil, y/z faces a whole pencil.
     // The neighbours' traces of all faces first, every global load issued
     // before any is used (a neighbour in the block is read from LDS; a face
-    // that loads nothing points at the thread's own pencil). In the FP32
-    // instance at p = 3 the registers that holds cost more occupancy than
-    // the shorter wait gains (measured), so there each face loads its own.
-    constexpr bool kHoist = !(N1 == 4 && sizeof(T) == 4);
+    // that loads nothing points at the thread's own pencil). At p = 3
+    // (N1 = 4) the registers that holds cost more than the shorter wait
+    // gains (measured in both precisions), so there each face loads its own.
+    constexpr bool kHoist = N1 != 4;
     double oth[2 * DIM][N1];
     if constexpr (kHoist) {
         int64_t at[2 * DIM];
