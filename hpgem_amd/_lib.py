@@ -51,6 +51,9 @@ SIGNATURES = [
     ("hgdg_state_set_storage", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ("hgdg_time_derivative", ctypes.c_int, [ctypes.c_void_p, _INT, _D, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_double]),
+    ("hgdg_right_hand_side", ctypes.c_int, [ctypes.c_void_p, _INT, _D, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_double]),
+    ("hgdg_solve_mass", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hgdg_scale_and_add", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
     ("hgdg_rk_steps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _INT, ctypes.c_int, _D, _D,
                                      ctypes.c_double, ctypes.c_int]),

@@ -313,6 +313,16 @@ class HpgemAPIDevice:
         co = np.ascontiguousarray(coefficientsInputVectors, dtype=np.float64)
         check(self.lib.hgdg_time_derivative(self.h, ids, dptr(co), len(inputVectorIds), resultVectorId, time))
 
+    def computeRightHandSide(self, inputVectorIds, coefficientsInputVectors, resultVectorId, time=0.0):
+        """r = S w before the inverse mass (HpgemAPILinear::computeRightHandSide)."""
+        ids = (ctypes.c_int * len(inputVectorIds))(*inputVectorIds)
+        co = np.ascontiguousarray(coefficientsInputVectors, dtype=np.float64)
+        check(self.lib.hgdg_right_hand_side(self.h, ids, dptr(co), len(inputVectorIds), resultVectorId, time))
+
+    def solveMassMatrixEquations(self, timeIntegrationVectorId):
+        """vector = M^{-1} vector, element by element (HpgemAPILinear::solveMassMatrixEquations)."""
+        check(self.lib.hgdg_solve_mass(self.h, timeIntegrationVectorId))
+
     def scaleAndAddVector(self, vectorToChangeId, vectorToAddId, scale):
         check(self.lib.hgdg_scale_and_add(self.h, vectorToChangeId, vectorToAddId, scale))
 

@@ -46,6 +46,8 @@ struct SimplexMesh {
     int32_t tpe_fixed[4] = {0, 0, 0, 0};         // per order: 0 = timed choice, else the fixed threads per element
     double tpe_ms[4][2] = {};                    // per order and candidate: timed RHS launch (ms), 0 if not timed
     DevBuf<int32_t> d_tab_index;                 // offsets and column lists into d_tables
+    DevBuf<double> d_mass;                       // per order: M_ref [nb][nb], then M_ref^{-1}
+    DevBuf<int32_t> d_order;                     // per device element: its order
     DevBuf<int64_t> d_elem_off;                  // [E + 1] first DOF of each device element
     DevBuf<double> d_geo;                        // J^{-1} [9][E], then c_inv [E], sigma [4][E], normal [3][4][E]
     DevBuf<int32_t> d_face;                      // [4][E]: neighbour device element or -1, then packed info [4][E]
@@ -121,6 +123,8 @@ bool tensor_rk_tiny(hgdg_ctx* c, int solution, const int* aux, int stages, const
 // on scratch vectors of the mesh's size (or takes c->adv_tiles_fixed).
 void tensor_tune(hgdg_ctx* c);
 void tensor_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* l2, double* max_abs);
+// v = M v (inverse = false) or v = M^{-1} v (inverse = true), in place, FP64
+void tensor_mass_apply(hgdg_ctx* c, Vec& v, bool inverse);
 void tensor_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, const double* amp, const double* k,
                         const double* shift, double* l2, double* max_abs);
 void tensor_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev);
@@ -162,6 +166,7 @@ void simplex_tune(hgdg_ctx* c);
 int simplex_threads(int order, int variant);
 void simplex_rhs(hgdg_ctx* c, const StageIO& io);
 void simplex_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* l2, double* max_abs);
+void simplex_mass_apply(hgdg_ctx* c, Vec& v, bool inverse);
 void simplex_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, const double* amp, const double* k,
                          const double* shift, double* l2, double* max_abs);
 void simplex_project_host(hgdg_ctx* c, Vec& v, int field, int n_modes, const double* amp, const double* k,

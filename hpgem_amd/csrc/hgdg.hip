@@ -647,6 +647,32 @@ int hgdg_time_derivative(hgdg_ctx* ctx, const int* in_ids, const double* coeffs,
     });
 }
 
+int hgdg_right_hand_side(hgdg_ctx* ctx, const int* in_ids, const double* coeffs, int n_in, int out_id,
+                         double time) {
+    (void)time;  // the operators are autonomous
+    return run([&] {
+        require(ctx != nullptr && in_ids != nullptr && coeffs != nullptr, HGDG_ERR_ARG, "NULL argument");
+        HGDG_HIP(hipSetDevice(ctx->device));
+        hgdg::Vec& out = hgdg::vec_at(ctx, out_id);
+        require(!out.is_f32, HGDG_ERR_UNSUPPORTED, "the right-hand side is written to a vector stored in FP64");
+        // r = M k: the fused kernel's k, multiplied back by the block-diagonal mass
+        hgdg::launch_rhs(ctx, in_ids, coeffs, n_in, out_id);
+        if (ctx->simplex()) hgdg::simplex_mass_apply(ctx, out, false);
+        else hgdg::tensor_mass_apply(ctx, out, false);
+    });
+}
+
+int hgdg_solve_mass(hgdg_ctx* ctx, int vec) {
+    return run([&] {
+        require(ctx != nullptr, HGDG_ERR_ARG, "ctx is NULL");
+        require(ctx->has_op, HGDG_ERR_STATE, "set an operator before solving with its mass matrix");
+        HGDG_HIP(hipSetDevice(ctx->device));
+        hgdg::Vec& v = hgdg::vec_at(ctx, vec);
+        if (ctx->simplex()) hgdg::simplex_mass_apply(ctx, v, true);
+        else hgdg::tensor_mass_apply(ctx, v, true);
+    });
+}
+
 int hgdg_scale_and_add(hgdg_ctx* ctx, int to, int add, double scale) {
     return run([&] {
         require(ctx != nullptr, HGDG_ERR_ARG, "ctx is NULL");

@@ -101,6 +101,15 @@ struct hgdg_server {
     hipStream_t prep_stream = nullptr;  // the preparer's: building solves
     std::mutex mu;
     std::condition_variable cv;
+    // Serialises the two threads' allocation-heavy HIP work: the preparer
+    // builds a solve (hipMalloc/hipFree, uploads, launches on prep_stream)
+    // while holding it; the worker holds it while it re-stages (parameter
+    // blob reallocation, stream capture, graph instantiation) and while it
+    // retires solves (downloads, hipFree). hipFree waits for the whole device,
+    // so without this lock it could run while the worker's stream is in
+    // capture; the steady-state ticks (one graph launch and a stream
+    // synchronisation) take no lock and overlap with building.
+    std::mutex hip_mu;
     bool stop = false;
     std::deque<hgdg::Request> pending;
     std::deque<hgdg::Solve*> ready;
@@ -207,6 +216,7 @@ void hgdg_server::prepare_loop() {
         s->req = std::move(r);
         s->bytes = need;
         try {
+            std::lock_guard<std::mutex> hk(hip_mu);
             s->ctx = hgdg::build_solve(device, mixed, s->req, prep_stream);
         } catch (const std::exception& ex) {
             std::lock_guard<std::mutex> lk(mu);
@@ -328,7 +338,10 @@ void hgdg_server::tick(std::vector<hgdg::Solve*>& solves, int n_steps) {
     using namespace hgdg;
     bool same = solves.size() == staged.size();
     for (size_t i = 0; same && i < solves.size(); ++i) same = solves[i]->req.ticket == staged[i];
-    if (!same) stage(solves);
+    if (!same) {
+        std::lock_guard<std::mutex> hk(hip_mu);
+        stage(solves);
+    }
     for (int step = 0; step < n_steps; ++step) HGDG_HIP(hipGraphLaunch(step_exec, stream));
     HGDG_HIP(hipStreamSynchronize(stream));
 }
@@ -377,11 +390,14 @@ void hgdg_server::worker_loop() {
             } catch (const std::exception& ex) {
                 err = ex.what();
                 finished = mine;  // a failed tick ends every resident solve
+                std::lock_guard<std::mutex> hk(hip_mu);
+                (void)hipStreamSynchronize(stream);  // nothing of the old graph may still run
                 staged.clear();
                 drop_step_graph();
             }
         }
         for (hgdg::Solve* s : finished) {
+            std::unique_lock<std::mutex> hk(hip_mu);
             std::vector<double> state;
             double norm = 0;
             int64_t n_dof = 0;
@@ -404,6 +420,7 @@ void hgdg_server::worker_loop() {
             }
             hgdg::destroy_ctx(s->ctx);
             s->ctx = nullptr;
+            hk.unlock();
             std::lock_guard<std::mutex> lk(mu);
             auto& res = results[s->req.ticket];
             res.t_end = hgdg::now_s();
@@ -429,6 +446,7 @@ void hgdg_server::worker_loop() {
             cv.notify_all();
         }
     }
+    std::lock_guard<std::mutex> hk(hip_mu);
     for (hgdg::Solve* s : mine) {
         hgdg::destroy_ctx(s->ctx);
         delete s;

@@ -546,6 +546,42 @@ __global__ __launch_bounds__(kBlock) void simplex_diff_norm(const SimplexDiffPar
     }
 }
 
+// The block-diagonal mass matrix of the acoustic system
+// (AcousticWaveLinear_Impl.h:102-129: |det J| M_ref on v, c_inv |det J| M_ref
+// on each s_d) applied to a vector or solved against it, in place. One
+// wavefront per element; the element's four fields are staged in LDS.
+struct SimplexMassParams {
+    const int64_t* elem_off;  // [n_elem + 1]
+    const int32_t* order;     // [n_elem]
+    const double* tab;        // per order: M_ref, then M_ref^{-1}
+    int32_t tab_off[2][4];    // [apply / solve][order - 1]
+    int32_t inverse;
+    double* v;
+    const double* absdet;     // [n_elem]
+    const double* cinv;       // [n_elem]
+};
+
+__global__ __launch_bounds__(64) void simplex_mass(const SimplexMassParams p) {
+    __shared__ double s[4 * SimplexShape<4>::NB];
+    const int64_t e = blockIdx.x;
+    const int P = p.order[e];
+    const int nb = (P + 1) * (P + 2) * (P + 3) / 6;
+    const int n = 4 * nb;
+    const int64_t off = p.elem_off[e];
+    for (int i = threadIdx.x; i < n; i += 64) s[i] = p.v[off + i];
+    __syncthreads();
+    const double* A = p.tab + p.tab_off[p.inverse][P - 1];
+    const double w0 = p.absdet[e];
+    const double ws = w0 * p.cinv[e];
+    for (int r = threadIdx.x; r < n; r += 64) {
+        const int f = r / nb, i = r - f * nb;
+        double acc = 0;
+        for (int j = 0; j < nb; ++j) acc += A[i * nb + j] * s[f * nb + j];
+        const double w = f > 0 ? ws : w0;
+        p.v[off + r] = p.inverse ? acc / w : acc * w;
+    }
+}
+
 // ---------------------------------------------------------------- host side
 namespace {
 
@@ -785,6 +821,11 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     simplex_schedule(c);
 
     sm.d_tables.upload(tab, c->stream);
+    std::vector<double> mass;  // lives until the synchronize below
+    for (int q = 1; q <= kMaxSimplexOrder; ++q) mass.insert(mass.end(), sm.ops[q - 1].Mref.begin(), sm.ops[q - 1].Mref.end());
+    for (int q = 1; q <= kMaxSimplexOrder; ++q) mass.insert(mass.end(), sm.ops[q - 1].Minv.begin(), sm.ops[q - 1].Minv.end());
+    sm.d_mass.upload(mass, c->stream);
+    sm.d_order.upload(sm.order, c->stream);
     const std::vector<float> tab32(tab.begin(), tab.end());  // lives until the synchronize below
     sm.d_tables32.upload(tab32, c->stream);
     sm.d_tab_index.upload(tix, c->stream);
@@ -987,6 +1028,30 @@ void simplex_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, d
     }
     *l2 = std::sqrt(std::max(s, 0.0));
     *max_abs = mx;
+}
+
+void simplex_mass_apply(hgdg_ctx* c, Vec& v, bool inverse) {
+    require(!v.is_f32, HGDG_ERR_UNSUPPORTED, "mass apply/solve needs a vector stored in FP64");
+    require(c->n_fields == 4, HGDG_ERR_STATE, "the tetrahedral mass matrix belongs to the acoustic operator");
+    SimplexMesh& sm = c->sm;
+    SimplexMassParams p{};
+    int32_t at = 0;
+    for (int k = 0; k < 2; ++k)
+        for (int q = 1; q <= kMaxSimplexOrder; ++q) {
+            p.tab_off[k][q - 1] = at;
+            at += tet_nb(q) * tet_nb(q);
+        }
+    require(sm.d_mass.n == size_t(at), HGDG_ERR_STATE, "set the operator before the mass matrix is used");
+    p.elem_off = sm.d_elem_off.ptr;
+    p.order = sm.d_order.ptr;
+    p.tab = sm.d_mass.ptr;
+    p.inverse = inverse ? 1 : 0;
+    p.v = v.f64->ptr;
+    p.absdet = sm.d_absdet.ptr;
+    p.cinv = sm.d_geo.ptr + size_t(9) * c->n_elem;
+    if (c->n_elem == 0) return;
+    hipLaunchKernelGGL(simplex_mass, dim3(unsigned(c->n_elem)), dim3(64), 0, c->stream, p);
+    HGDG_HIP(hipGetLastError());
 }
 
 namespace {

@@ -417,6 +417,7 @@ SimplexOps simplex_ops(int p) {
     }
     const std::vector<long double> Mi = invert(nb, M);
     ops.Mref.assign(M.begin(), M.end());
+    ops.Minv.assign(Mi.begin(), Mi.end());
     ops.Dr.assign(3 * size_t(nb) * nb, 0.0);
     for (int e = 0; e < 3; ++e)
         for (int i = 0; i < nb; ++i)

@@ -40,6 +40,7 @@ struct SimplexOps {
     int p = 0, nb = 0;
     std::vector<double> Dr;    // [3][nb][nb]: reference derivative d/dxi_e, exact in P_p
     std::vector<double> Mref;  // [nb][nb]
+    std::vector<double> Minv;  // [nb][nb]: M_ref^{-1}, inverted in long double
     // face tables per local face f and orientation o (trace points mapped
     // through orientation o before the face map):
     std::array<std::vector<int>, 4> cols;            // nonzero trace columns of face f

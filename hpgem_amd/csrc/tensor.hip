@@ -1194,6 +1194,62 @@ __global__ __launch_bounds__(kBlock) void tensor_diff_norm(const DiffParams p) {
     }
 }
 
+// The block-diagonal mass matrix (HpgemAPILinear::createMassMatrices, with
+// the acoustic weighting of AcousticWaveLinear_Impl.h:102-129) applied to a
+// vector, or solved against it, in place: per element and field the 1D factor
+// A1 = M1 (apply) or Mi = M1^{-1} (solve) along every direction, then times
+// |det J| w_f (apply) or divided by it (solve); w_f = c_inv on the s fields of
+// the acoustic system, else 1.
+struct MassParams {
+    int64_t n_blocks;      // elements x fields, field-major inside an element
+    int32_t fields;
+    int32_t inverse;
+    double* v;
+    const double* absdet;  // [n_elem]
+    const double* cinv;    // [n_elem], or NULL
+    const double* A1;      // [n1][n1]
+};
+
+template <int DIM, int N1>
+__global__ __launch_bounds__(kBlock) void tensor_mass(const MassParams p) {
+    using S = Tensor<DIM, N1>;
+    __shared__ double s_a[S::EPB * S::NB];
+    __shared__ double s_b[S::EPB * S::NB];
+    __shared__ double s_M[N1 * N1];
+    const int tid = threadIdx.x;
+    if (tid < N1 * N1) s_M[tid] = p.A1[tid];
+    const int le = tid / S::L;
+    const int dof = tid % S::L;
+    const int64_t b = int64_t(blockIdx.x) * S::EPB + le;
+    const bool active = (b < p.n_blocks) && (dof < S::NB);
+    if (active) s_a[le * S::NB + dof] = p.v[b * S::NB + dof];
+    __syncthreads();
+    const int idx[3] = {dof % N1, (dof / N1) % N1, DIM == 3 ? dof / (N1 * N1) : 0};
+    constexpr int stride[3] = {1, N1, N1 * N1};
+    double* src = s_a;
+    double* dst = s_b;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        if (active) {
+            const int row = dof - idx[d] * stride[d];
+            double s = 0;
+            for (int j = 0; j < N1; ++j) s += s_M[idx[d] * N1 + j] * src[le * S::NB + row + j * stride[d]];
+            dst[le * S::NB + dof] = s;
+        }
+        __syncthreads();
+        double* t = src;
+        src = dst;
+        dst = t;
+    }
+    if (active) {
+        const int64_t e = b / p.fields;
+        double w = p.absdet[e];
+        if (p.cinv != nullptr && b % p.fields > 0) w *= p.cinv[e];
+        const double x = src[le * S::NB + dof];
+        p.v[b * S::NB + dof] = p.inverse ? x / w : x * w;
+    }
+}
+
 template <typename F>
 void dispatch_tensor(int dim, int n1, F&& f) {
 #define HGDG_CASE(D, N) \
@@ -1783,6 +1839,25 @@ void tensor_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, do
     }
     *l2 = std::sqrt(std::max(s, 0.0));
     *max_abs = m;
+}
+
+void tensor_mass_apply(hgdg_ctx* c, Vec& v, bool inverse) {
+    require(!v.is_f32, HGDG_ERR_UNSUPPORTED, "mass apply/solve needs a vector stored in FP64");
+    MassParams p{};
+    p.n_blocks = c->n_elem * c->n_fields;
+    p.fields = c->n_fields;
+    p.inverse = inverse ? 1 : 0;
+    p.v = v.f64->ptr;
+    p.absdet = c->tm.d_absdet.ptr;
+    p.cinv = c->op.kind == HGDG_OP_ACOUSTIC_CENTRAL ? c->tm.d_geo.ptr + size_t(c->dim) * c->dim * c->n_elem : nullptr;
+    p.A1 = inverse ? Mi_ptr(c) : M1_ptr(c);
+    dispatch_tensor(c->dim, c->tm.n1, [&](auto D, auto N) {
+        constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
+        using S = Tensor<DIMc, N1c>;
+        const int64_t grid = (p.n_blocks + S::EPB - 1) / S::EPB;
+        hipLaunchKernelGGL((tensor_mass<DIMc, N1c>), dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
+    });
+    HGDG_HIP(hipGetLastError());
 }
 
 void tensor_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, const double* amp, const double* k,

@@ -174,6 +174,26 @@ int hgdg_state_set_storage(hgdg_ctx* ctx, int vec, int precision);
 int hgdg_time_derivative(hgdg_ctx* ctx, const int* in_ids, const double* coeffs, int n_in, int out_id,
                          double time);
 
+/* out = r( sum_i coeffs[i] * vec[in_ids[i]] ): the stage combination and the
+ * volume + face integrals, before the inverse mass (r = M k, the block-diagonal
+ * mass of the operator, acoustic: c_inv on the s fields; the fused kernel's k
+ * multiplied back by M). out is stored in FP64. The operators are autonomous:
+ * time is taken for the signature's sake; a time-dependent source is added to
+ * r by the caller (hgdg_scale_and_add of a source vector), as
+ * HpgemAPILinear::addSourceTerm does.
+ * Replaces HpgemAPILinear::computeRightHandSide(ids, coeffs, out, t)
+ * (kernel/Base/HpgemAPILinear_Impl.h:485-498 -> multiplyStiffnessMatrices,
+ * :360-434). */
+int hgdg_right_hand_side(hgdg_ctx* ctx, const int* in_ids, const double* coeffs, int n_in, int out_id,
+                         double time);
+
+/* vec = M^{-1} vec in place, M the operator's block-diagonal mass matrix
+ * (acoustic: c_inv on the s fields), with the reference element's inverse
+ * mass formed in long double; vec stored in FP64.
+ * Replaces HpgemAPILinear::solveMassMatrixEquations(id)
+ * (kernel/Base/HpgemAPILinear_Impl.h:143-156). */
+int hgdg_solve_mass(hgdg_ctx* ctx, int vec);
+
 /* vec[to] += scale * vec[add]  (HpgemAPISimplified::scaleAndAddVector, :707-717). */
 int hgdg_scale_and_add(hgdg_ctx* ctx, int to, int add, double scale);
 

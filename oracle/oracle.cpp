@@ -1074,8 +1074,12 @@ void assemble(Problem& P) {
     P.assembled = true;
 }
 
-// getLinearCombinationOfVectors + multiplyStiffnessMatrices + mass solve
-void time_derivative(Problem& P, int nin, const Real* const* vecs, const Real* coef, Real* out) {
+// getLinearCombinationOfVectors + multiplyStiffnessMatrices
+// (HpgemAPILinear::computeRightHandSide, HpgemAPILinear_Impl.h:360-434,
+// 485-498): r = S w with w the stage combination; with solve = true, then the
+// per-element mass solve (solveMassMatrixEquations, :143-156) as
+// computeTimeDerivative does.
+void time_derivative(Problem& P, int nin, const Real* const* vecs, const Real* coef, Real* out, bool solve = true) {
     if (!P.assembled) assemble(P);
     int64_t ndof = P.offset[P.nE];
     std::vector<Real> w(ndof);
@@ -1108,10 +1112,18 @@ void time_derivative(Problem& P, int nin, const Real* const* vecs, const Real* c
             gemv_add(NL, NL, P.F[fi][0], &w[P.offset[f.L]], &out[P.offset[f.L]], false);
         }
     }
+    if (!solve) return;
     for (int64_t e = 0; e < P.nE; ++e) {
         int N = P.nB(e) * P.nfields;
         lu_solve_copy(N, P.M[e], &out[P.offset[e]]);
     }
+}
+
+// HpgemAPILinear::solveMassMatrixEquations (HpgemAPILinear_Impl.h:143-156):
+// every element's block solved in place with its mass matrix by LU.
+void solve_mass(Problem& P, Real* v) {
+    if (!P.assembled) assemble(P);
+    for (int64_t e = 0; e < P.nE; ++e) lu_solve_copy(P.nB(e) * P.nfields, P.M[e], &v[P.offset[e]]);
 }
 
 // RK tableau: classic RK4 (RK4Methods.cpp:45-63)
@@ -1225,6 +1237,35 @@ Real l2_error(Problem& P, const Real* u, const Modes& m, Real t) {
         }
     }
     return std::sqrt(total);
+}
+
+// computeIntegrandTotalError / integrateErrorAtElement (HpgemAPISimplified_Impl.h:
+// 293-366, rule order 2p+2) split by field: err2[v] = sum_e int_e |f_v(x - shift)
+// - u_h,v|^2, where field v's exact function is sum_m amp[v][m] sin(2 pi k_m .
+// (x - shift)). The caller combines the fields (plain sum, or the acoustic
+// example's weighting of AcousticWaveLinear_Impl.h:285-312).
+void l2_error_fields(Problem& P, const Real* u, int n_modes, const Real* amp, const Real* k, const Real* shift,
+                     Real* err2) {
+    int dim = P.dim;
+    for (int v = 0; v < P.nfields; ++v) err2[v] = 0;
+    for (int64_t e = 0; e < P.nE; ++e) {
+        int nB = P.nB(e);
+        Rule rule = get_rule(P.shape, 2 * P.order[e] + 2);
+        ElemPoint ep;
+        for (int q = 0; q < rule.n(); ++q) {
+            elem_point(P, e, &rule.pts[q * dim], rule.w[q], ep);
+            for (int v = 0; v < P.nfields; ++v) {
+                Modes m;
+                m.n = n_modes;
+                m.amp = amp + size_t(v) * n_modes;
+                m.k = k;
+                Real num = 0;
+                for (int j = 0; j < nB; ++j) num += ep.phi[j] * u[P.offset[e] + v * nB + j];
+                Real err = m.eval(ep.x, dim, shift) - num;
+                err2[v] += ep.w * err * err;
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------ mesh generator
@@ -1730,6 +1771,31 @@ int orc_time_derivative(void* h, int n_in, const double* const* vecs, const doub
     })
 }
 
+int orc_right_hand_side(void* h, int n_in, const double* const* vecs, const double* coef, double* out) {
+    ORC_TRY({
+        Problem* P = static_cast<Problem*>(h);
+        const size_t n = size_t(P->offset[P->nE]);
+        std::vector<std::vector<Real>> in(n_in);
+        std::vector<const Real*> ptrs(n_in);
+        std::vector<Real> c(coef, coef + n_in), o(n);
+        for (int i = 0; i < n_in; ++i) {
+            in[i].assign(vecs[i], vecs[i] + n);
+            ptrs[i] = in[i].data();
+        }
+        time_derivative(*P, n_in, ptrs.data(), c.data(), o.data(), false);
+        std::copy(o.begin(), o.end(), out);
+    })
+}
+
+int orc_solve_mass(void* h, double* v) {
+    ORC_TRY({
+        Problem* P = static_cast<Problem*>(h);
+        std::vector<Real> w(v, v + P->offset[P->nE]);
+        solve_mass(*P, w.data());
+        std::copy(w.begin(), w.end(), v);
+    })
+}
+
 int orc_rk4(void* h, double* u, double dt, int n_steps) {
     ORC_TRY({
         Problem* P = static_cast<Problem*>(h);
@@ -1774,6 +1840,18 @@ int orc_l2_error_modes(void* h, const double* u, int n_modes, const double* amp,
         m.amp = ar.data();
         m.k = kr.data();
         *err = double(l2_error(*P, ur.data(), m, Real(t)));
+    })
+}
+
+int orc_l2_error_fields(void* h, const double* u, int n_modes, const double* amp, const double* k,
+                        const double* shift, double* err2) {
+    ORC_TRY({
+        Problem* P = static_cast<Problem*>(h);
+        std::vector<Real> ar(amp, amp + size_t(P->nfields) * n_modes), kr(k, k + 3 * n_modes),
+            ur(u, u + P->offset[P->nE]), e2(P->nfields);
+        Real sh[3] = {shift[0], shift[1], shift[2]};
+        l2_error_fields(*P, ur.data(), n_modes, ar.data(), kr.data(), sh, e2.data());
+        for (int v = 0; v < P->nfields; ++v) err2[v] = double(e2[v]);
     })
 }
 

@@ -221,6 +221,23 @@ class Problem:
         check(self.lib.orc_time_derivative(self.h, len(vecs), arr, _dp(c), _dp(out)))
         return out
 
+    def right_hand_side(self, vecs, coefs):
+        """r = S w before the mass solve (HpgemAPILinear::computeRightHandSide)."""
+        vecs = [np.ascontiguousarray(v, dtype=np.float64) for v in vecs]
+        arr = (_D * len(vecs))(*[_dp(v) for v in vecs])
+        c = np.ascontiguousarray(coefs, dtype=np.float64)
+        out = np.zeros(self.n_dof)
+        self.lib.orc_right_hand_side.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(_D), _D, _D]
+        check(self.lib.orc_right_hand_side(self.h, len(vecs), arr, _dp(c), _dp(out)))
+        return out
+
+    def solve_mass(self, v):
+        """M^{-1} v per element by LU (solveMassMatrixEquations)."""
+        out = np.array(v, dtype=np.float64, copy=True)
+        self.lib.orc_solve_mass.argtypes = [ctypes.c_void_p, _D]
+        check(self.lib.orc_solve_mass(self.h, _dp(out)))
+        return out
+
     def rk4(self, u, dt, n_steps):
         u = np.array(u, dtype=np.float64, copy=True)
         check(self.lib.orc_rk4(self.h, _dp(u), dt, n_steps))
@@ -272,6 +289,20 @@ class Problem:
         M = M.reshape(self.n_elem, n, n).transpose(0, 2, 1)
         F = F.reshape(nf, 4, n, n).transpose(0, 1, 3, 2)
         return S, M, F
+
+    def l2_error_fields(self, u, amp, k, shift):
+        """Per-field squared L2 errors against f_v(x - shift), f_v = sum_m
+        amp[v][m] sin(2 pi k_m . x) (computeTotalError's integrand, rule 2p+2)."""
+        amp = np.ascontiguousarray(np.broadcast_to(amp, (self.n_fields, len(k))), dtype=np.float64)
+        kk = np.zeros((len(k), 3))
+        kk[:, : np.asarray(k).shape[1]] = k
+        sh = np.zeros(3)
+        sh[: len(shift)] = shift
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros(self.n_fields)
+        self.lib.orc_l2_error_fields.argtypes = [ctypes.c_void_p, _D, ctypes.c_int, _D, _D, _D, _D]
+        check(self.lib.orc_l2_error_fields(self.h, _dp(u), len(kk), _dp(amp), _dp(kk), _dp(sh), _dp(out)))
+        return out
 
     def mass_matrix(self, e):
         nb = self.n_dof // self.n_elem if np.all(self.orders == self.orders[0]) else None

@@ -11,3 +11,15 @@ if TESTS not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu)")
+
+
+def pytest_sessionfinish(session, exitstatus):
+    """Keep the parity distances the GPU suites measured (tests/_parity.py)."""
+    mod = sys.modules.get("_parity")
+    out = os.path.join(ROOT, "gpurun_out")
+    if mod is None or not mod.DISTANCES or not os.path.isdir(out):
+        return
+    import json
+
+    with open(os.path.join(out, "parity_distances.json"), "w") as f:
+        json.dump(mod.DISTANCES, f, indent=1)

@@ -1,29 +1,16 @@
-"""GPU parity of the tetrahedral acoustic-wave RHS (AcousticWaveLinear:
-central flux, reflecting walls, per-element c_inv, per-element order 1..4)
-against the oracle, same bars as the advection suite: <=1e-12 against the
-oracle run in long double, and within the double oracle's own rounding."""
+"""GPU parity of the acoustic-wave RHS (AcousticWaveLinear: central flux,
+reflecting walls, per-element c_inv) on tetrahedra of per-element order 1..4
+and on squares/cubes, against the oracle, with the FP64 bar of
+tests/_parity.py (tetrahedra: <= 1e-12 against the double and the
+long-double oracle at every order)."""
 import numpy as np
 import pytest
 
 import _oracle as orc
+from _parity import assert_parity, rel
 from hpgem_amd import OP_ACOUSTIC_CENTRAL, RK4, HpgemAPIDevice, Mesh
 
 pytestmark = pytest.mark.gpu
-
-FP64_TOL = 1e-12
-
-
-def rel(a, b):
-    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
-
-
-def assert_parity(got, ref64, ref_ld):
-    own = rel(ref64, ref_ld)
-    e_ld = rel(got, ref_ld)
-    e_64 = rel(got, ref64)
-    print(f"  vs long double {e_ld:.2e}, vs double {e_64:.2e} (double vs long double {own:.2e})")
-    assert e_ld < FP64_TOL
-    assert e_64 < max(FP64_TOL, 2 * own)
 
 
 def make(n, orders, seed, periodic=False, extended=True):
@@ -63,11 +50,13 @@ def test_acoustic_rhs_matches_oracle(n, orders, periodic):
     dev.setTimeIntegrationVector(0, u)
     dev.setTimeIntegrationVector(1, k)
     dev.computeTimeDerivative([0], [1.0], 2)
+    pm = int(ref.orders.max())
+    label = f"tet acoustic n={n} p={orders} periodic={periodic}"
     assert_parity(dev.getTimeIntegrationVector(2), ref.time_derivative([u], [1.0]),
-                  ref_ld.time_derivative([u], [1.0]))
+                  ref_ld.time_derivative([u], [1.0]), pm, ref, label)
     dev.computeTimeDerivative([0, 1], [1.0, -0.2], 3)
     assert_parity(dev.getTimeIntegrationVector(3), ref.time_derivative([u, k], [1.0, -0.2]),
-                  ref_ld.time_derivative([u, k], [1.0, -0.2]))
+                  ref_ld.time_derivative([u, k], [1.0, -0.2]), pm, ref, label + " u-0.2k")
     dev.close()
 
 
@@ -78,7 +67,8 @@ def test_acoustic_rk4_matches_oracle_hp():
     dev.setTimeIntegrationVector(0, u0)
     dt = 0.01
     dev.computeOneTimeStep(0.0, dt, 4)
-    assert_parity(dev.getTimeIntegrationVector(0), ref.rk4(u0, dt, 4), ref_ld.rk4(u0, dt, 4))
+    assert_parity(dev.getTimeIntegrationVector(0), ref.rk4(u0, dt, 4), ref_ld.rk4(u0, dt, 4), int(ref.orders.max()),
+                  ref, "tet acoustic hp RK4 x4")
     dev.close()
 
 
@@ -132,7 +122,7 @@ def test_tensor_acoustic_rhs_matches_oracle(n, p, periodic):
     dev.setTimeIntegrationVector(1, k)
     dev.computeTimeDerivative([0, 1], [1.0, 0.3], 2)
     assert_parity(dev.getTimeIntegrationVector(2), ref.time_derivative([u, k], [1.0, 0.3]),
-                  ref_ld.time_derivative([u, k], [1.0, 0.3]))
+                  ref_ld.time_derivative([u, k], [1.0, 0.3]), p, ref, f"tensor acoustic n={n} p={p} periodic={periodic}")
     dev.close()
 
 
@@ -178,7 +168,8 @@ def test_tet_acoustic_on_reference_mesh_file(name):
     u = rng.standard_normal(dev.n_dof)
     dev.setTimeIntegrationVector(0, u)
     dev.computeTimeDerivative([0], [1.0], 1)
-    assert_parity(dev.getTimeIntegrationVector(1), ref.time_derivative([u], [1.0]), ref_ld.time_derivative([u], [1.0]))
+    assert_parity(dev.getTimeIntegrationVector(1), ref.time_derivative([u], [1.0]), ref_ld.time_derivative([u], [1.0]),
+                  int(p.max()), ref, f"tet acoustic mesh file {name}")
     dev.close()
 
 
@@ -268,7 +259,8 @@ def test_tet_alternative_threads_match_oracle():
     dev.setTimeIntegrationVector(1, k)
     dev.computeTimeDerivative([0, 1], [1.0, -0.2], 2)
     assert_parity(dev.getTimeIntegrationVector(2), ref.time_derivative([u, k], [1.0, -0.2]),
-                  ref_ld.time_derivative([u, k], [1.0, -0.2]))
+                  ref_ld.time_derivative([u, k], [1.0, -0.2]), int(ref.orders.max()), ref,
+                  "tet acoustic n=[3, 2, 2] threads=[32, 64, 64, 64]")
     dev.close()
 
 
@@ -363,4 +355,43 @@ def test_tet_error_norm_matches_oracle_to_quadrature_accuracy():
     print(f"tet L2 error: library {l2:.6e}, oracle {e_ref:.6e}; max {mx:.3e}")
     assert abs(l2 - e_ref) < 1e-12 * e_ref
     assert mx > 0
+    dev.close()
+
+
+@pytest.mark.parametrize("kind", ["tet", "cube"])
+def test_right_hand_side_and_mass_solve_match_oracle(kind):
+    """hgdg_right_hand_side (r before the inverse mass,
+    HpgemAPILinear::computeRightHandSide) and hgdg_solve_mass
+    (solveMassMatrixEquations) against the oracle's two halves of
+    computeTimeDerivative; their composition equals the fused path."""
+    print(f"[case] right-hand side and mass solve, acoustic {kind}", flush=True)
+    if kind == "tet":
+        dev, (ref, ref_ld) = make([2, 2, 2], "random", seed=17)
+    else:
+        mesh = Mesh.structured([3, 3, 3], periodic=[0, 0, 0])
+        cinv = np.random.default_rng(3).uniform(0.25, 4.0, size=mesh.n_elements)
+        dev = HpgemAPIDevice(mesh, 3, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=RK4, n_extra_vectors=2)
+        ref = orc.Problem(mesh.shape, mesh.coords, mesh.node_ids, 3, n_fields=4)
+        ref.set_acoustic(cinv)
+        ref_ld = orc.Problem(mesh.shape, mesh.coords, mesh.node_ids, 3, n_fields=4, extended=True)
+        ref_ld.set_acoustic(cinv)
+    pm = int(ref.orders.max())
+    rng = np.random.default_rng(23)
+    u = rng.standard_normal(dev.n_dof)
+    k = rng.standard_normal(dev.n_dof)
+    dev.setTimeIntegrationVector(0, u)
+    dev.setTimeIntegrationVector(1, k)
+    dev.computeRightHandSide([0, 1], [1.0, 0.25], 2)
+    r = dev.getTimeIntegrationVector(2)
+    assert_parity(r, ref.right_hand_side([u, k], [1.0, 0.25]), ref_ld.right_hand_side([u, k], [1.0, 0.25]), pm, ref,
+                  f"acoustic {kind} right-hand side")
+    # mass solve of an arbitrary vector
+    dev.setTimeIntegrationVector(3, k)
+    dev.solveMassMatrixEquations(3)
+    assert_parity(dev.getTimeIntegrationVector(3), ref.solve_mass(k), ref_ld.solve_mass(k), pm, ref,
+                  f"acoustic {kind} mass solve")
+    # r then M^{-1} = the fused time derivative
+    dev.solveMassMatrixEquations(2)
+    dev.computeTimeDerivative([0, 1], [1.0, 0.25], 4)
+    assert rel(dev.getTimeIntegrationVector(2), dev.getTimeIntegrationVector(4)) < 1e-12
     dev.close()

@@ -1,40 +1,29 @@
 """GPU parity of the advection RHS (volume + upwind faces + inverse mass, with
 the RK stage combination) and of RK time stepping against the oracle.
 
-Bars (relative = max abs difference / max abs value):
-* FP64 path vs the oracle's algorithm run in long double (the exact operator
-  to ~1e-18): <= 1e-12.
-* FP64 path vs the oracle in double (hpGEM's algorithm with its FP64
-  rounding): <= max(1e-12, 2 x the double oracle's own distance from the long
-  double one). hpGEM solves each element's hierarchical mass matrix by LU;
-  its condition number grows like 57^3 (p=3) and 145^3 (p=4), so the
-  reference's own FP64 result carries ~1e-12 (p=3) and ~2e-11 (p=4) of
-  rounding, which no other implementation can reproduce digit for digit.
-* mixed FP32 path within 1e-6 relative L2 of the FP64 solution.
+FP64 bar (tests/_parity.py): <= 1e-12 relative (max abs coefficient
+difference / max abs value) against the oracle run in long double for every
+case; <= 1e-12 against the oracle in double (hpGEM's own rounding) for every
+case of order <= 3, including the 32^3 hex p=3 workload's element on 4^3
+meshes; at p = 4, where hpGEM's per-call LU of a mass matrix of condition
+~3e6 leaves the double oracle itself 2e-11 from the exact operator, <= 1e-12
+relative in the mass-weighted L2 norm. Mixed FP32 path: within 1e-6 relative
+L2 of the FP64 solution.
 """
 import numpy as np
 import pytest
 
 import _oracle as orc
 import hpgem_amd
+from _parity import assert_parity as _assert_parity
+from _parity import rel
 from hpgem_amd import HpgemAPIDevice, Mesh, RK4
 
 pytestmark = pytest.mark.gpu
 
-FP64_TOL = 1e-12
 
-
-def rel(a, b):
-    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
-
-
-def assert_parity(got, ref64, ref_ld):
-    own = rel(ref64, ref_ld)
-    e_ld = rel(got, ref_ld)
-    e_64 = rel(got, ref64)
-    print(f"  vs long double {e_ld:.2e}, vs double {e_64:.2e} (double vs long double {own:.2e})")
-    assert e_ld < FP64_TOL
-    assert e_64 < max(FP64_TOL, 2 * own)
+def assert_parity(got, ref64, ref_ld, p, oracle, label=""):
+    _assert_parity(got, ref64, ref_ld, p, oracle, label)
 
 
 def make_pair(n, p, a, periodic=True, shear=None, precision="fp64", tableau=RK4, extended=False):
@@ -85,11 +74,13 @@ def test_time_derivative_matches_oracle(n, p, a, periodic, shear):
     # single input
     dev.computeTimeDerivative([0], [1.0], 2)
     got = dev.getTimeIntegrationVector(2)
-    assert_parity(got, ref.time_derivative([u], [1.0]), ref_ld.time_derivative([u], [1.0]))
+    label = f"advection rhs n={n} p={p} periodic={periodic} shear={shear is not None}"
+    assert_parity(got, ref.time_derivative([u], [1.0]), ref_ld.time_derivative([u], [1.0]), p, ref, label)
     # RK-style stage combination u + 0.37 k
     dev.computeTimeDerivative([0, 1], [1.0, 0.37], 3)
     got = dev.getTimeIntegrationVector(3)
-    assert_parity(got, ref.time_derivative([u, k], [1.0, 0.37]), ref_ld.time_derivative([u, k], [1.0, 0.37]))
+    assert_parity(got, ref.time_derivative([u, k], [1.0, 0.37]), ref_ld.time_derivative([u, k], [1.0, 0.37]), p, ref,
+                  label + " u+0.37k")
     dev.close()
 
 
@@ -113,10 +104,10 @@ def test_rk4_steps_match_oracle_2d_quads_p2():
     dt = 0.1 * (1 / 16) / np.hypot(*a)
     u0 = ref.project_modes(amp, kvec)
     dev.setInitialSolutionModes(amp, kvec)
-    assert_parity(dev.getTimeIntegrationVector(0), u0, ref_ld.project_modes(amp, kvec))
+    assert_parity(dev.getTimeIntegrationVector(0), u0, ref_ld.project_modes(amp, kvec), 2, ref, "quad16 p=2 projection")
     dev.computeOneTimeStep(0.0, dt, 100)
     got = dev.getTimeIntegrationVector(0)
-    assert_parity(got, ref.rk4(u0, dt, 100), ref_ld.rk4(u0, dt, 100))
+    assert_parity(got, ref.rk4(u0, dt, 100), ref_ld.rk4(u0, dt, 100), 2, ref, "quad16 p=2 RK4 x100")
     dev.close()
 
 
@@ -129,9 +120,10 @@ def test_rk4_steps_match_oracle_3d_hex_p3():
     dt = 0.1 * (1 / 4) / np.linalg.norm(a)
     u0 = ref.project_modes(amp, kvec)
     dev.setInitialSolutionModes(amp, kvec)
-    assert_parity(dev.getTimeIntegrationVector(0), u0, ref_ld.project_modes(amp, kvec))
+    assert_parity(dev.getTimeIntegrationVector(0), u0, ref_ld.project_modes(amp, kvec), 3, ref, "hex4 p=3 projection")
     dev.computeOneTimeStep(0.0, dt, 5)
-    assert_parity(dev.getTimeIntegrationVector(0), ref.rk4(u0, dt, 5), ref_ld.rk4(u0, dt, 5))
+    assert_parity(dev.getTimeIntegrationVector(0), ref.rk4(u0, dt, 5), ref_ld.rk4(u0, dt, 5), 3, ref,
+                  "hex4 p=3 RK4 x5")
     dev.close()
 
 
@@ -347,7 +339,8 @@ def test_rhs_on_reference_mesh_file(name, p, a):
     u = seeded_state(dev.n_dof, 7)
     dev.setTimeIntegrationVector(0, u)
     dev.computeTimeDerivative([0], [1.0], 1)
-    assert_parity(dev.getTimeIntegrationVector(1), ref.time_derivative([u], [1.0]), ref_ld.time_derivative([u], [1.0]))
+    assert_parity(dev.getTimeIntegrationVector(1), ref.time_derivative([u], [1.0]), ref_ld.time_derivative([u], [1.0]),
+                  p, ref, f"mesh file {name} p={p}")
     dev.close()
 
 
@@ -451,3 +444,23 @@ def test_fused_rk_steps_bitwise_equal_graph_replay(n, p, a, periodic, precision,
     assert np.all(np.isfinite(out[True][0]))
     for g, f in zip(out[False], out[True]):
         np.testing.assert_array_equal(f, g)
+
+
+@pytest.mark.parametrize("n,p,periodic", [([4, 4], 2, True), ([3, 3, 3], 3, False)])
+def test_right_hand_side_and_mass_solve_match_oracle(n, p, periodic):
+    """hgdg_right_hand_side and hgdg_solve_mass (the two halves of
+    computeTimeDerivative in HpgemAPILinear) against the oracle."""
+    print(f"[case] advection right-hand side and mass solve n={n} p={p}", flush=True)
+    a = [1.0, 0.7, 0.3][: len(n)]
+    dev, ref, ref_ld = make_pair(n, p, a, periodic, extended=True)
+    rng = np.random.default_rng(31)
+    u = rng.standard_normal(dev.n_dof)
+    dev.setTimeIntegrationVector(0, u)
+    dev.computeRightHandSide([0], [1.0], 1)
+    assert_parity(dev.getTimeIntegrationVector(1), ref.right_hand_side([u], [1.0]), ref_ld.right_hand_side([u], [1.0]),
+                  p, ref, f"advection right-hand side n={n} p={p}")
+    dev.setTimeIntegrationVector(2, u)
+    dev.solveMassMatrixEquations(2)
+    assert_parity(dev.getTimeIntegrationVector(2), ref.solve_mass(u), ref_ld.solve_mass(u), p, ref,
+                  f"advection mass solve n={n} p={p}")
+    dev.close()
