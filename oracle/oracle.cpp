@@ -56,6 +56,9 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace orc {
 
@@ -768,6 +771,7 @@ struct Problem {
     std::vector<std::vector<Real>> M;      // per element, column-major (nB*nf)^2
     std::vector<std::vector<Real>> S;      // per element stiffness, column-major
     std::vector<std::array<std::vector<Real>, 4>> F;  // per face: LL, LR, RL, RR column-major
+    std::vector<std::vector<std::pair<int64_t, int>>> elem_faces;  // (face, side) per element, all-core path
     bool assembled = false;
 
     const Basis& basis(int p) {
@@ -950,6 +954,10 @@ void assemble(Problem& P) {
     int nfld = P.nfields;
     P.M.assign(P.nE, {});
     P.S.assign(P.nE, {});
+    // every basis in use exists before the loops (read-only inside them; the
+    // CPU-baseline build runs the loops on several threads)
+    for (int64_t e = 0; e < P.nE; ++e) P.basis(P.order[e]);
+#pragma omp parallel for schedule(dynamic, 16)
     for (int64_t e = 0; e < P.nE; ++e) {
         int nB = P.nB(e);
         int N = nB * nfld;
@@ -989,6 +997,7 @@ void assemble(Problem& P) {
         }
     }
     P.F.assign(P.faces.size(), {});
+#pragma omp parallel for schedule(dynamic, 16)
     for (size_t fi = 0; fi < P.faces.size(); ++fi) {
         const FaceRec& f = P.faces[fi];
         int nBL = P.nB(f.L) * nfld, nBR = f.internal ? P.nB(f.R) * nfld : 0;
@@ -1116,6 +1125,57 @@ void time_derivative(Problem& P, int nin, const Real* const* vecs, const Real* c
     for (int64_t e = 0; e < P.nE; ++e) {
         int N = P.nB(e) * P.nfields;
         lu_solve_copy(N, P.M[e], &out[P.offset[e]]);
+    }
+}
+
+// The same operator with the loops spread over threads, for the all-core CPU
+// baseline only (the reference runs one thread per MPI rank; this stands in
+// for its multi-rank scaling): each element gathers the face blocks that act
+// on it (its own side's rows of every face it lies on), so the threads share
+// no output; the element's sum then runs in a different order than the
+// face-ordered serial loop (same terms, different rounding).
+void time_derivative_par(Problem& P, int nin, const Real* const* vecs, const Real* coef, Real* out) {
+    if (!P.assembled) assemble(P);
+    const int64_t ndof = P.offset[P.nE];
+    std::vector<Real> w(ndof);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < ndof; ++i) {
+        Real v = vecs[0][i] * coef[0];
+        for (int k = 1; k < nin; ++k) v += coef[k] * vecs[k][i];
+        w[i] = v;
+    }
+    if (P.elem_faces.empty()) {  // (face, side) pairs per element, built once
+        P.elem_faces.assign(P.nE, {});
+        for (size_t fi = 0; fi < P.faces.size(); ++fi) {
+            P.elem_faces[P.faces[fi].L].push_back({int64_t(fi), 0});
+            if (P.faces[fi].internal) P.elem_faces[P.faces[fi].R].push_back({int64_t(fi), 1});
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t e = 0; e < P.nE; ++e) {
+        const int N = P.nB(e) * P.nfields;
+        std::vector<Real> t(N, 0.0);
+        auto gemv = [&](int cols, const std::vector<Real>& A, const Real* x) {
+            for (int c = 0; c < cols; ++c) {
+                const Real xc = x[c];
+                for (int r = 0; r < N; ++r) t[r] += A[r + size_t(c) * N] * xc;
+            }
+        };
+        gemv(N, P.S[e], &w[P.offset[e]]);
+        for (const auto& fs : P.elem_faces[e]) {
+            const FaceRec& f = P.faces[fs.first];
+            const int NL = P.nB(f.L) * P.nfields;
+            const int NR = f.internal ? P.nB(f.R) * P.nfields : 0;
+            if (fs.second == 0) {
+                gemv(NL, P.F[fs.first][0], &w[P.offset[f.L]]);
+                if (f.internal) gemv(NR, P.F[fs.first][1], &w[P.offset[f.R]]);
+            } else {
+                gemv(NL, P.F[fs.first][2], &w[P.offset[f.L]]);
+                gemv(NR, P.F[fs.first][3], &w[P.offset[f.R]]);
+            }
+        }
+        lu_solve_copy(N, P.M[e], t.data());
+        std::copy(t.begin(), t.end(), out + P.offset[e]);
     }
 }
 
@@ -1769,6 +1829,30 @@ int orc_time_derivative(void* h, int n_in, const double* const* vecs, const doub
         time_derivative(*P, n_in, ptrs.data(), c.data(), o.data());
         std::copy(o.begin(), o.end(), out);
     })
+}
+
+int orc_time_derivative_par(void* h, int n_in, const double* const* vecs, const double* coef, double* out) {
+    ORC_TRY({
+        Problem* P = static_cast<Problem*>(h);
+        const size_t n = size_t(P->offset[P->nE]);
+        std::vector<std::vector<Real>> in(n_in);
+        std::vector<const Real*> ptrs(n_in);
+        std::vector<Real> c(coef, coef + n_in), o(n);
+        for (int i = 0; i < n_in; ++i) {
+            in[i].assign(vecs[i], vecs[i] + n);
+            ptrs[i] = in[i].data();
+        }
+        time_derivative_par(*P, n_in, ptrs.data(), c.data(), o.data());
+        std::copy(o.begin(), o.end(), out);
+    })
+}
+
+int orc_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 int orc_right_hand_side(void* h, int n_in, const double* const* vecs, const double* coef, double* out) {

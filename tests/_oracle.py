@@ -13,6 +13,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 LIB_PATH_LD = os.path.join(ROOT, "oracle", "liboracle_ld.so")
+LIB_PATH_BENCH = os.path.join(ROOT, "oracle", "liboracle_bench.so")
 FIXTURE = os.path.join(ROOT, "tests", "golden", "ref_tables.json")
 
 LINE, SQUARE, TRIANGLE, CUBE, TETRAHEDRON = 1, 2, 3, 4, 5
@@ -32,10 +33,11 @@ def _dp(a):
 
 def load(extended=False):
     """The double-precision oracle, or (extended=True) the same algorithm in
-    long double."""
+    long double, or (extended="bench") the double build at -O3 with OpenMP
+    that bench.py times as the CPU baseline."""
     if extended in _libs:
         return _libs[extended]
-    path = LIB_PATH_LD if extended else LIB_PATH
+    path = LIB_PATH_BENCH if extended == "bench" else LIB_PATH_LD if extended else LIB_PATH
     if not os.path.exists(path):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True,
                        stdout=subprocess.DEVNULL)
@@ -54,6 +56,10 @@ def load(extended=False):
     lib.orc_set_acoustic.argtypes = [ctypes.c_void_p, _D]
     lib.orc_assemble.argtypes = [ctypes.c_void_p]
     lib.orc_time_derivative.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(_D), _D, _D]
+    lib.orc_time_derivative_par.restype = ctypes.c_int
+    lib.orc_time_derivative_par.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(_D), _D, _D]
+    lib.orc_threads.restype = ctypes.c_int
+    lib.orc_threads.argtypes = []
     lib.orc_rk4.argtypes = [ctypes.c_void_p, _D, ctypes.c_double, ctypes.c_int]
     lib.orc_project_modes.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D]
     lib.orc_l2_error_modes.argtypes = [ctypes.c_void_p, _D, ctypes.c_int, _D, _D, ctypes.c_double, _D]
@@ -220,6 +226,19 @@ class Problem:
         out = np.zeros(self.n_dof)
         check(self.lib.orc_time_derivative(self.h, len(vecs), arr, _dp(c), _dp(out)))
         return out
+
+    def time_derivative_par(self, vecs, coefs):
+        """The same operator with the element loop on all OpenMP threads (the
+        bench build): each element gathers its faces' blocks."""
+        vecs = [np.ascontiguousarray(v, dtype=np.float64) for v in vecs]
+        arr = (_D * len(vecs))(*[_dp(v) for v in vecs])
+        c = np.ascontiguousarray(coefs, dtype=np.float64)
+        out = np.zeros(self.n_dof)
+        check(self.lib.orc_time_derivative_par(self.h, len(vecs), arr, _dp(c), _dp(out)))
+        return out
+
+    def assemble(self):
+        check(self.lib.orc_assemble(self.h))
 
     def right_hand_side(self, vecs, coefs):
         """r = S w before the mass solve (HpgemAPILinear::computeRightHandSide)."""

@@ -5,21 +5,23 @@ with its FP64 rounding), ref_ld = the same algorithm in long double (the exact
 operator to ~1e-18). "Relative" = max abs difference / max abs value of the
 reference, over all coefficients.
 
-* got vs ref_ld: <= 1e-12 for every case.
+* got vs ref_ld: <= 1e-12 for every case (measured: 1e-16 .. 4e-16).
 * got vs ref64: <= 1e-12 for every tetrahedral case and every square/cube
-  case whose highest order is <= 3.
-* got vs ref64 on squares/cubes at order 4: <= 1e-12 relative in the
-  mass-weighted L2 norm
-  (the DG function-space norm, sqrt(sum_e x_e^T M_e x_e)), and the coefficient
-  distance is printed. hpGEM solves every element's hierarchical mass matrix
-  by LU on every call; at p = 4 its condition number is ~3e6 on cubes, so the
-  double oracle's own coefficients sit up to 2e-11 from the exact operator
-  (measured: 2.0e-11 coefficient-wise, 3.0e-13 in the mass-weighted norm, on
-  3^3 cubes): its rounding lives in directions of tiny L2 size, which the
-  function-space norm sees at their true weight. At p <= 3 the double
-  oracle's own distance is <= 8.4e-13 (4^3 cubes, p = 3), so the plain
-  coefficient bar holds there. hpGEM's tetrahedral mass matrices are far
-  better conditioned: the double oracle stays within 7e-14 of the exact
+  case whose highest order is <= 2.
+* got vs ref64 on squares/cubes at order >= 3: <= 1e-12 relative in the
+  mass-weighted L2 norm (the DG function-space norm,
+  sqrt(sum_e x_e^T M_e x_e)); the coefficient distance is printed and kept.
+  hpGEM solves every element's hierarchical mass matrix by LU on every call,
+  and the tensor-product sets are ill-conditioned (kappa(M_e) ~ 1.9e5 at
+  p = 3, ~3e6 at p = 4 on cubes), so the double oracle's own coefficients
+  sit up to 1.35e-12 (p = 3, the reference's unitCubeN4Per mesh) and 2.0e-11
+  (p = 4, 3^3 cubes) from the exact operator, while in the mass-weighted norm
+  they sit within 4e-14 and 3e-13: that rounding lives in directions of tiny
+  L2 size, which the function-space norm sees at their true weight. A plain
+  1e-12 coefficient bar against the double oracle would test the
+  reference's own rounding there, not this implementation, which stays
+  within 4e-16 of the exact operator. hpGEM's tetrahedral mass matrices are
+  far better conditioned: the double oracle stays within 7e-14 of the exact
   operator up to p = 4, so tetrahedra take the plain bar at every order.
 
 Every call prints the three distances and appends them to DISTANCES, which
@@ -64,13 +66,13 @@ def mass_rel(masses, x, ref):
 
 def assert_parity(got, ref64, ref_ld, p_max, oracle, label=""):
     """The bar above. p_max = the highest element order of the case; oracle =
-    the double oracle problem (its shape, and its mass matrices at p = 4)."""
+    the double oracle problem (its shape, and its mass matrices at p >= 3)."""
     own = rel(ref64, ref_ld)
     e_ld = rel(got, ref_ld)
     e_64 = rel(got, ref64)
     rec = {"case": label, "p_max": int(p_max), "vs_long_double": e_ld, "vs_double": e_64, "double_vs_long_double": own}
     msg = f"  [{label}] vs long double {e_ld:.2e}, vs double {e_64:.2e} (double vs long double {own:.2e})"
-    conditioning_limited = p_max >= 4 and oracle.shape in (orc.SQUARE, orc.CUBE)
+    conditioning_limited = p_max >= 3 and oracle.shape in (orc.SQUARE, orc.CUBE)
     if conditioning_limited:
         masses = element_masses(oracle)
         m64 = mass_rel(masses, got, ref64)
