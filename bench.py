@@ -145,17 +145,23 @@ def run_gpu(steps, warmup):
             events.record(c1, stream)
             copy_ms = events.elapsed_ms(c0, c1) / steps
             copy_gbs = 16 * api.n_dof / (copy_ms * 1e-3) / 1e9
-        # whole RK4 run: 50 steps, hipGraph-replayed (the graph is captured
-        # by a one-step warm-up, outside the timed region)
-        api.computeOneTimeStep(0.0, dt, 1)
-        api.setInitialSolutionModes(amp, k)
+        # whole RK4 run: 50 steps, hipGraph-replayed (the graphs are captured
+        # by a three-step warm-up, outside the timed region), with the RK
+        # update fused into each step's last stage (pairs of steps, the
+        # default) and with the update as a separate kernel
         stream = api.stream()
-        e0, e1 = events.event(), events.event()
-        api.synchronize()
-        events.record(e0, stream)
-        api.computeOneTimeStep(0.0, dt, 50)
-        events.record(e1, stream)
-        run_ms = events.elapsed_ms(e0, e1)
+        run = {}
+        for fused in (False, True):
+            api.setFusedTimeStepping(fused)
+            api.computeOneTimeStep(0.0, dt, 3)
+            api.setInitialSolutionModes(amp, k)
+            e0, e1 = events.event(), events.event()
+            api.synchronize()
+            events.record(e0, stream)
+            api.computeOneTimeStep(0.0, dt, 50)
+            events.record(e1, stream)
+            run[fused] = events.elapsed_ms(e0, e1)
+        run_ms = run[True]
         finals[prec] = api.getTimeIntegrationVector(0)
         results[prec] = {
             "gdofs": api.n_dof / (ms * 1e-3) / 1e9,
@@ -164,6 +170,8 @@ def run_gpu(steps, warmup):
             "achieved_gbs": bytes_alg / (ms * 1e-3) / 1e9,
             "rk4_50_steps_ms": run_ms,
             "rk4_step_ms": run_ms / 50,
+            "rk4_step_gdofs": 4 * api.n_dof / (run_ms / 50 * 1e-3) / 1e9,
+            "rk4_step_ms_separate_update": run[False] / 50,
             "copy_gbs": copy_gbs if prec == "fp64" else None,
             "host_wall_s": wall,
         }

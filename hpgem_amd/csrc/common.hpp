@@ -77,6 +77,22 @@ constexpr int kQuadXor1 = 0xB1;  // lanes (1, 0, 3, 2)
 constexpr int kQuadXor2 = 0x4E;  // lanes (2, 3, 0, 1)
 __host__ __device__ constexpr int quad_bcast(int l) { return l * 0x55; }  // lanes (l, l, l, l)
 
+// The RK update of a step, fused into its last stage's kernel:
+// y_new = u + sum_j c_j x_j, the terms added one after another in FP64 (the
+// operations of multi_axpy, so the result is bitwise the same), where term
+// `self` is the stage derivative this kernel has just computed, taken as
+// stored (rounded to the output's storage type), and the others are read from
+// memory. y_new is another vector than u: the kernel's neighbours still read u.
+struct RkUpdate {
+    const double* u;  // NULL: no update
+    double* y_new;
+    const void* x[kMaxIn];
+    double c[kMaxIn];
+    uint32_t x_f32;  // bit j set: term j is stored in FP32
+    int32_t n_x;
+    int32_t self;    // the term that is this kernel's output, or -1
+};
+
 // Inputs and output of one stage evaluation: out = f(sum_i coef_i in_i).
 struct StageIO {
     const void* in[kMaxIn];
@@ -85,7 +101,62 @@ struct StageIO {
     int32_t n_in;
     void* out;
     int32_t out_f32;
+    RkUpdate upd;     // fused RK update (upd.u == NULL: none)
 };
+
+// N consecutive values as doubles (N even: 16-byte vector loads, src being
+// aligned to N values).
+template <int N, typename T>
+__device__ __forceinline__ void load_run(const T* src, double (&x)[N]) {
+    if constexpr (N % 2 == 0 && sizeof(T) == 8) {
+#pragma unroll
+        for (int a = 0; a < N; a += 2) {
+            const double2 t = *reinterpret_cast<const double2*>(src + a);
+            x[a] = t.x;
+            x[a + 1] = t.y;
+        }
+    } else if constexpr (N % 2 == 0) {
+#pragma unroll
+        for (int a = 0; a < N; a += 2) {
+            const float2 t = *reinterpret_cast<const float2*>(src + a);
+            x[a] = t.x;
+            x[a + 1] = t.y;
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < N; ++a) x[a] = double(src[a]);
+    }
+}
+
+// The fused update of N consecutive coefficients at o (a multiple of N), with
+// `own` the values of this kernel's output there (as stored).
+template <int N>
+__device__ __forceinline__ void rk_update_run(const RkUpdate& q, int64_t o, const double (&own)[N]) {
+    double y[N];
+    load_run<N>(q.u + o, y);
+#pragma unroll
+    for (int j = 0; j < kMaxIn; ++j) {
+        if (j >= q.n_x) break;
+        double x[N];
+        if (j == q.self) {
+#pragma unroll
+            for (int a = 0; a < N; ++a) x[a] = own[a];
+        } else if ((q.x_f32 >> j) & 1u) {
+            load_run<N>(static_cast<const float*>(q.x[j]) + o, x);
+        } else {
+            load_run<N>(static_cast<const double*>(q.x[j]) + o, x);
+        }
+#pragma unroll
+        for (int a = 0; a < N; ++a) y[a] += q.c[j] * x[a];
+    }
+    if constexpr (N % 2 == 0) {
+#pragma unroll
+        for (int a = 0; a < N; a += 2) *reinterpret_cast<double2*>(q.y_new + o + a) = make_double2(y[a], y[a + 1]);
+    } else {
+#pragma unroll
+        for (int a = 0; a < N; ++a) q.y_new[o + a] = y[a];
+    }
+}
 
 __device__ __forceinline__ double load_in(const StageIO& p, int i, int64_t idx) {
     return ((p.in_f32 >> i) & 1u) ? double(static_cast<const float*>(p.in[i])[idx])

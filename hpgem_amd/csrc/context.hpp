@@ -80,24 +80,44 @@ struct hgdg_ctx {
     hgdg::GraphKey graph_key{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // two steps with the RK update fused into each last stage, ping-ponging
+    // the solution through u_spare (captured for one solution buffer)
+    bool graph2_valid = false;
+    hgdg::GraphKey graph2_key{};
+    const double* graph2_u = nullptr;
+    hipGraph_t graph2 = nullptr;
+    hipGraphExec_t graph2_exec = nullptr;
+    hgdg::DevBuf<double> u_spare;
+    int64_t rk_pair_launches = 0;
     double dt_estimate = 0;  // adaptive stepping: next step size (0 = not started)
     double last_kernel_ms = 0;  // duration of the last assembly kernel (HIP events)
     // tensor advection: tiles of elements each workgroup takes (1 = one tile,
     // no prefetch); chosen by timing at operator set-up unless fixed by the caller
     int adv_tiles = 1;
     int adv_tiles_fixed = 0;
-    int rk_fused = 1;  // RK steps of meshes that fit in LDS in one single-workgroup launch (0: always the graph)
+    // 1: RK steps of meshes that fit in LDS in one single-workgroup launch, and
+    // elsewhere (tensor advection) the RK update fused into each step's last
+    // stage; 0: always the graph of separate launches
+    int rk_fused = 1;
     int64_t rk_fused_launches = 0;
     double adv_tile_ms[4] = {0, 0, 0, 0};  // timed candidates 1, 2, 4, 8 (0 = not timed)
 
     bool simplex() const { return shape == HGDG_SHAPE_TETRAHEDRON; }
     int64_t n_dof() const { return dof; }
+    void drop_graph2() {
+        if (graph2_exec) (void)hipGraphExecDestroy(graph2_exec);
+        if (graph2) (void)hipGraphDestroy(graph2);
+        graph2_exec = nullptr;
+        graph2 = nullptr;
+        graph2_valid = false;
+    }
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
         graph_exec = nullptr;
         graph = nullptr;
         graph_valid = false;
+        drop_graph2();
     }
 };
 

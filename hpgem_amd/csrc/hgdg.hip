@@ -187,7 +187,7 @@ void operator_set(hgdg_ctx* c, const hgdg_op_desc* op) {
     if (old_fields != c->n_fields) c->vecs.clear();  // the field count changed the vector size
 }
 
-void launch_rhs(hgdg_ctx* c, const int* in_ids, const double* coeffs, int n_in, int out_id) {
+StageIO make_io(hgdg_ctx* c, const int* in_ids, const double* coeffs, int n_in, int out_id) {
     require(c->has_op, HGDG_ERR_STATE, "set an operator before computing time derivatives");
     require(n_in >= 1 && n_in <= kMaxIn, HGDG_ERR_ARG, "between 1 and 8 input vectors");
     StageIO io{};
@@ -202,8 +202,21 @@ void launch_rhs(hgdg_ctx* c, const int* in_ids, const double* coeffs, int n_in, 
     Vec& out = vec_at(c, out_id);
     io.out = out.ptr();
     io.out_f32 = out.is_f32 ? 1 : 0;
+    return io;
+}
+
+void dispatch_rhs(hgdg_ctx* c, const StageIO& io) {
     if (c->simplex()) simplex_rhs(c, io);
     else tensor_rhs(c, io);
+}
+
+void launch_rhs(hgdg_ctx* c, const int* in_ids, const double* coeffs, int n_in, int out_id) {
+    dispatch_rhs(c, make_io(c, in_ids, coeffs, n_in, out_id));
+}
+
+// Whether the operator's RHS kernel performs a fused RK update (StageIO::upd).
+bool rhs_fuses_update(const hgdg_ctx* c) {
+    return !c->simplex() && c->op.kind == HGDG_OP_ADVECTION_UPWIND;
 }
 
 void launch_axpy(hgdg_ctx* c, int to, const std::vector<int>& add, const std::vector<double>& scale) {
@@ -250,6 +263,42 @@ void enqueue_rk_step(hgdg_ctx* c, int solution, const int* aux, int stages, cons
     if (!add.empty()) launch_axpy(c, solution, add, sc);
 }
 
+// One step as enqueue_rk_step performs it, on the solution held at u_cur,
+// with the update fused into the last stage's kernel and written to u_next
+// (u_cur stays as it was: the last stage's neighbours still read it). The
+// FP64 operations are those of enqueue_rk_step, so the result is bitwise the
+// same.
+void enqueue_rk_step_fused(hgdg_ctx* c, int solution, const int* aux, int stages, const double* a, const double* b,
+                           double dt, double* u_cur, double* u_next) {
+    for (int i = 0; i < stages; ++i) {
+        std::vector<int> ids{solution};
+        std::vector<double> co{1.0};
+        for (int j = 0; j < i; ++j)
+            if (a[i * stages + j] != 0.0) {
+                ids.push_back(aux[j]);
+                co.push_back(dt * a[i * stages + j]);
+            }
+        StageIO io = make_io(c, ids.data(), co.data(), int(ids.size()), aux[i]);
+        io.in[0] = u_cur;
+        if (i == stages - 1) {
+            RkUpdate& q = io.upd;
+            q.u = u_cur;
+            q.y_new = u_next;
+            q.self = -1;
+            for (int j = 0; j < stages; ++j) {
+                if (b[j] == 0.0) continue;
+                Vec& x = vec_at(c, aux[j]);
+                if (j == stages - 1) q.self = q.n_x;
+                q.x[q.n_x] = x.ptr();
+                q.c[q.n_x] = dt * b[j];
+                if (x.is_f32) q.x_f32 |= 1u << q.n_x;
+                ++q.n_x;
+            }
+        }
+        dispatch_rhs(c, io);
+    }
+}
+
 void check_rk_args(hgdg_ctx* c, int solution, const int* aux, int stages) {
     require(c->has_op, HGDG_ERR_STATE, "set an operator before time stepping");
     require(stages >= 1 && stages <= kMaxIn - 1, HGDG_ERR_ARG, "1..7 stages supported");
@@ -270,6 +319,35 @@ void rk_steps(hgdg_ctx* c, int solution, const int* aux, int stages, const doubl
     if (tensor_rk_tiny(c, solution, aux, stages, a, b, dt, n_steps)) return;
     GraphKey key{solution, std::vector<int>(aux, aux + stages), std::vector<double>(a, a + stages * stages),
                  std::vector<double>(b, b + stages), dt};
+    if (c->rk_fused != 0 && rhs_fuses_update(c) && n_steps >= 2) {
+        // pairs of steps with the update fused into each last stage: the
+        // solution goes to the context's spare vector and back
+        double* u0 = vec_at(c, solution).f64->ptr;
+        if (c->u_spare.n != size_t(c->n_dof())) c->u_spare.alloc(size_t(c->n_dof()));
+        if (!(c->graph2_valid && c->graph2_key == key && c->graph2_u == u0)) {
+            c->drop_graph2();
+            HGDG_HIP(hipStreamSynchronize(c->stream));
+            HGDG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_rk_step_fused(c, solution, aux, stages, a, b, dt, u0, c->u_spare.ptr);
+                enqueue_rk_step_fused(c, solution, aux, stages, a, b, dt, c->u_spare.ptr, u0);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(c->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HGDG_HIP(hipStreamEndCapture(c->stream, &c->graph2));
+            HGDG_HIP(hipGraphInstantiate(&c->graph2_exec, c->graph2, nullptr, nullptr, 0));
+            c->graph2_key = key;
+            c->graph2_u = u0;
+            c->graph2_valid = true;
+        }
+        for (int s = 0; s < n_steps / 2; ++s) HGDG_HIP(hipGraphLaunch(c->graph2_exec, c->stream));
+        c->rk_pair_launches += n_steps / 2;
+        n_steps %= 2;
+        if (n_steps == 0) return;
+    }
     if (!(c->graph_valid && c->graph_key == key)) {
         c->drop_graph();
         HGDG_HIP(hipStreamSynchronize(c->stream));

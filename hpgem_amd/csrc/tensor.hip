@@ -222,7 +222,7 @@ __device__ __forceinline__ bool advection_load(const RhsParams& p, int64_t blk, 
 // (y/z face: the neighbour's pencil nbr_pc), called only for faces whose mode
 // needs the neighbour; every kernel variant shares this code, so they agree
 // bit for bit.
-template <int DIM, int N1, typename T, typename XO, typename YZ>
+template <int DIM, int N1, typename T, bool UPD, typename XO, typename YZ>
 __device__ __forceinline__ void advection_compute(const RhsParams& p, int64_t e, int pc, const double (&w)[N1],
                                                   uint32_t md, const double* ahd, const double* sw, const T* s_D,
                                                   const T* s_Mi, XO xo, YZ yz) {
@@ -296,20 +296,30 @@ __device__ __forceinline__ void advection_compute(const RhsParams& p, int64_t e,
         }
     }
     const int64_t o = e * S::NB + pc * N1;
+    double kv[N1];
     if (p.io.out_f32) {
         float* out = static_cast<float*>(p.io.out) + o;
 #pragma unroll
-        for (int a = 0; a < N1; ++a) out[a] = float(acc[a].value());
+        for (int a = 0; a < N1; ++a) {
+            const float v = float(acc[a].value());
+            out[a] = v;
+            kv[a] = double(v);
+        }
     } else {
         double* out = static_cast<double*>(p.io.out) + o;
 #pragma unroll
-        for (int a = 0; a < N1; ++a) out[a] = double(acc[a].value());
+        for (int a = 0; a < N1; ++a) {
+            const double v = double(acc[a].value());
+            out[a] = v;
+            kv[a] = v;
+        }
     }
+    if constexpr (UPD) rk_update_run<N1>(p.io.upd, o, kv);
 }
 
 // General meshes: neighbour ids from the per-element face data; a neighbour
 // inside the block is read from LDS, any other from HBM.
-template <int DIM, int N1, typename T>
+template <int DIM, int N1, typename T, bool UPD = false>
 __device__ __forceinline__ void advection_finish(const RhsParams& p, int64_t blk, const double (&w)[N1],
                                                  const ElemMeta<DIM>& meta, const double* s_w, const T* s_D,
                                                  const T* s_Mi) {
@@ -336,7 +346,7 @@ __device__ __forceinline__ void advection_finish(const RhsParams& p, int64_t blk
             combine_pencil<N1>(p.io, nb * S::NB + nbr_pc * N1, other);
         }
     };
-    advection_compute<DIM, N1, T>(p, e, pc, w, meta.md, meta.ah, s_w + le * S::NB, s_D, s_Mi, xo, yz);
+    advection_compute<DIM, N1, T, UPD>(p, e, pc, w, meta.md, meta.ah, s_w + le * S::NB, s_D, s_Mi, xo, yz);
 }
 
 template <int DIM, int N1, typename T>
@@ -346,7 +356,7 @@ __device__ __forceinline__ void advection_tables(const RhsParams& p, T* s_D, T* 
     if (tid < 2 * N1) s_Mi[tid] = T(p.Mi[(tid % N1) * N1 + tid / N1]);  // s_Mi[s * N1 + a] = Mi(a, s)
 }
 
-template <int DIM, int N1, typename T>
+template <int DIM, int N1, typename T, bool UPD = false>
 __device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk) {
     using S = Pencil<DIM, N1>;
     __shared__ double s_w[S::EPB * S::NB];  // FP64 stage input of the block's elements
@@ -361,7 +371,7 @@ __device__ __forceinline__ void advection_block(const RhsParams& p, int64_t blk)
         for (int a = 0; a < N1; ++a) s_w[le * S::NB + pc * N1 + a] = w[a];
     }
     __syncthreads();
-    advection_finish<DIM, N1, T>(p, blk, w, meta, s_w, s_D, s_Mi);
+    advection_finish<DIM, N1, T, UPD>(p, blk, w, meta, s_w, s_D, s_Mi);
 }
 
 // Whole RK steps of a small advection mesh in one workgroup: the solution,
@@ -444,7 +454,7 @@ __global__ __launch_bounds__((rk_tiny_block<DIM, N1>())) void tensor_advection_r
 #pragma unroll
                     for (int a = 0; a < N1; ++a) other[a] = sw[nb * S::NB + nbr_pc * N1 + a];
                 };
-                advection_compute<DIM, N1, T>(pp, e, pc, w, smd[e], ah, sw + e * S::NB, s_D, s_Mi, xo, yz);
+                advection_compute<DIM, N1, T, false>(pp, e, pc, w, smd[e], ah, sw + e * S::NB, s_D, s_Mi, xo, yz);
             }
             __syncthreads();
         }
@@ -471,10 +481,11 @@ constexpr int adv_min_waves() {
     return (DIM == 3 && N1 == 4 && sizeof(T) == 8) ? 7 : 1;
 }
 
-template <int DIM, int N1, typename T>
+// UPD: the RK update fused into this (last) stage, StageIO::upd.
+template <int DIM, int N1, typename T, bool UPD = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(adv_min_waves<DIM, N1, T>())))
 void tensor_advection_rhs(const RhsParams p) {
-    advection_block<DIM, N1, T>(p, xcd_block(blockIdx.x, gridDim.x));
+    advection_block<DIM, N1, T, UPD>(p, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // The same work with each block taking `tiles` consecutive tiles of EPB
@@ -1641,6 +1652,15 @@ static void launch_advection(hgdg_ctx* c, const RhsParams& p, int tiles) {
         constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
         using S = Pencil<DIMc, N1c>;
         const int64_t n_tiles = (c->n_elem + S::EPB - 1) / S::EPB;
+        if (p.io.upd.u != nullptr) {  // the fused RK update runs in the one-tile kernel
+            if (mixed)
+                hipLaunchKernelGGL((tensor_advection_rhs<DIMc, N1c, float, true>), dim3(unsigned(n_tiles)),
+                                   dim3(kBlock), 0, c->stream, p);
+            else
+                hipLaunchKernelGGL((tensor_advection_rhs<DIMc, N1c, double, true>), dim3(unsigned(n_tiles)),
+                                   dim3(kBlock), 0, c->stream, p);
+            return;
+        }
         if (tiles > 1) {
             const int64_t grid = (n_tiles + tiles - 1) / tiles;
             if (mixed)

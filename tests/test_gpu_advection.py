@@ -464,3 +464,42 @@ def test_right_hand_side_and_mass_solve_match_oracle(n, p, periodic):
     assert_parity(dev.getTimeIntegrationVector(2), ref.solve_mass(u), ref_ld.solve_mass(u), p, ref,
                   f"advection mass solve n={n} p={p}")
     dev.close()
+
+
+PAIR_CASES = [
+    # (cells, p, velocity, periodic, precision, tableau, steps): meshes too large for the single-workgroup path
+    ([8, 8, 8], 3, [1.0, 0.7, 0.3], True, "fp64", "RK4", 6),
+    ([8, 8, 8], 3, [1.0, 0.7, 0.3], True, "mixed", "RK4", 7),    # three pairs and one separate step
+    ([10, 6, 4], 2, [-0.5, 1.0, 0.2], False, "fp64", "RK3_TVD", 5),
+    ([12, 8, 6], 1, [0.3, -0.2, 1.0], True, "fp64", "DORMAND_PRINCE", 4),  # last stage has b = 0
+    ([40, 30], 3, [1.0, 0.5], True, "mixed", "RK4", 2),
+]
+
+
+@pytest.mark.parametrize("n,p,a,periodic,precision,tab,steps", PAIR_CASES)
+def test_fused_update_steps_bitwise_equal_graph_replay(n, p, a, periodic, precision, tab, steps):
+    """Pairs of RK steps with the update fused into each step's last stage
+    (the solution ping-pongs through a spare vector) perform the FP64
+    operations of the separate update kernel: the solution and every stage
+    vector come out bitwise the same as from the replayed one-step graph."""
+    print(f"[case] fused update n={n} p={p} {precision} {tab} steps={steps}", flush=True)
+    tableau = getattr(hpgem_amd.tableaux, tab)
+    mesh = Mesh.structured(n, periodic=[int(periodic)] * len(n))
+    u0 = None
+    out = {}
+    for fused in (False, True):
+        dev = HpgemAPIDevice(mesh, p, velocity=a, precision=precision, tableau=tableau, n_extra_vectors=2)
+        dev.setFusedTimeStepping(fused)
+        if u0 is None:
+            u0 = seeded_state(dev.n_dof, 6)
+        dev.setTimeIntegrationVector(0, u0)
+        dt = 0.05 / max(n) / np.linalg.norm(a)
+        dev.computeOneTimeStep(0.0, dt, steps)
+        dev.computeOneTimeStep(0.0, dt, 3)  # the same graphs again, odd count
+        enabled, launches = dev.getFusedTimeStepping()
+        assert enabled == fused and launches == 0
+        out[fused] = [dev.getTimeIntegrationVector(v) for v in range(tableau.stages + 1)]
+        dev.close()
+    assert np.all(np.isfinite(out[False][0]))
+    for v in range(tableau.stages + 1):
+        np.testing.assert_array_equal(out[True][v], out[False][v])
