@@ -59,6 +59,8 @@ SIGNATURES = [
                                      ctypes.c_double, ctypes.c_int]),
     ("hgdg_project_modes", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, _D]),
     ("hgdg_project_field_modes", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D]),
+    ("hgdg_step_error_noalloc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _INT, ctypes.c_int, _D,
+                                               ctypes.c_double, _D, _D]),
     ("hgdg_step_error", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _INT, ctypes.c_int, _D, ctypes.c_double, _D,
                                        _D]),
     ("hgdg_rk_adaptive_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _INT, ctypes.c_int, _D, _D, _D,

@@ -12,6 +12,7 @@
 #include <functional>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -457,6 +458,115 @@ void step_error_and_norm(hgdg_ctx* c, int solution, const int* aux, int stages, 
     *norm = nrm;
 }
 
+// ---- the step error without allocation -------------------------------------
+// hgdg_step_error keeps its per-block partial sums in the context's buffer,
+// which it allocates and grows on demand, and reduces them on the host in a
+// std::vector. hgdg_step_error_noalloc runs the same per-element kernel with
+// the partials in `scratch`, a file-scope device array of fixed size, then a
+// one-block kernel reduces them into its last two entries and 16 bytes come
+// back to the caller's stack. Each GPU holds its own copy of the array (at its
+// own address, looked up once per device when a context is created on it);
+// every call on one GPU shares it, so calls that overlap on one GPU race.
+constexpr int64_t kScratchBlocks = 65536;  // partial pairs: meshes of at most 4 * 65536 elements
+constexpr int64_t kScratchMaxElements = 4 * kScratchBlocks;
+static __device__ __align__(256) double scratch[2 * kScratchBlocks + 2];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};  // device address of `scratch` on each device
+
+// The current device's address of `scratch`, looked up once per device.
+void setup_scratch(int device) {
+    require(device >= 0 && device < kMaxDevices, HGDG_ERR_UNSUPPORTED, "device index above 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device] != nullptr) return;
+    void* ptr = nullptr;
+    HGDG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    require(ptr != nullptr, HGDG_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch[device] = static_cast<double*>(ptr);
+}
+
+// out[0] = sum of partial[2 i], out[1] = sum of partial[2 i + 1], i < n.
+__global__ __launch_bounds__(kBlock) void reduce_pairs(const double* partial, int64_t n, double* out) {
+    __shared__ double s[2][kBlock];
+    const int t = threadIdx.x;
+    double a = 0, b = 0;
+    for (int64_t i = t; i < n; i += kBlock) {
+        a += partial[2 * i];
+        b += partial[2 * i + 1];
+    }
+    s[0][t] = a;
+    s[1][t] = b;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            s[0][t] += s[0][t + w];
+            s[1][t] += s[1][t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = s[0][0];
+        out[1] = s[1][0];
+    }
+}
+
+// Failure of a call that must not allocate on its success path: the message
+// is built only here.
+int fail_noalloc(int code, const char* msg) {
+    g_error = msg;
+    return code;
+}
+
+int step_error_noalloc(hgdg_ctx* c, int solution, const int* aux, int stages, const double* err_coef, double dt,
+                       double* error, double* norm) {
+    if (c == nullptr || aux == nullptr || err_coef == nullptr || error == nullptr || norm == nullptr)
+        return fail_noalloc(HGDG_ERR_ARG, "NULL argument");
+    if (stages < 1 || stages > kMaxIn) return fail_noalloc(HGDG_ERR_ARG, "1..8 stage vectors");
+    if (!c->has_op) return fail_noalloc(HGDG_ERR_STATE, "set an operator first");
+    const int nv = int(c->vecs.size());
+    if (solution < 0 || solution >= nv) return fail_noalloc(HGDG_ERR_ARG, "solution vector id out of range");
+    for (int j = 0; j < stages; ++j)
+        if (aux[j] < 0 || aux[j] >= nv) return fail_noalloc(HGDG_ERR_ARG, "stage vector id out of range");
+    if (c->vecs[solution].is_f32) return fail_noalloc(HGDG_ERR_ARG, "the solution vector must be stored in FP64");
+    if (c->n_elem > kScratchMaxElements)
+        return fail_noalloc(HGDG_ERR_UNSUPPORTED,
+                            "hgdg_step_error_noalloc: at most 262144 elements (its fixed scratch array holds "
+                            "65536 partial pairs); use hgdg_step_error for larger meshes");
+    if (c->device < 0 || c->device >= kMaxDevices || g_scratch[c->device] == nullptr)
+        return fail_noalloc(HGDG_ERR_STATE, "no scratch address for this device (create the context with "
+                                            "hgdg_ctx_create)");
+    if (hipSetDevice(c->device) != hipSuccess) return fail_noalloc(HGDG_ERR_HIP, "hipSetDevice failed");
+    double* ptr = g_scratch[c->device];
+    StepNormParams p{};
+    p.n_elem = c->n_elem;
+    p.off = c->simplex() ? c->sm.d_elem_off.ptr : nullptr;
+    p.block = c->n_elem > 0 ? c->dof / c->n_elem : 0;
+    p.u = c->vecs[solution].f64->ptr;
+    for (int j = 0; j < stages; ++j) {
+        if (err_coef[j] == 0.0) continue;
+        const Vec& k = c->vecs[aux[j]];
+        p.k[p.n_k] = k.ptr();
+        p.c[p.n_k] = dt * err_coef[j];
+        if (k.is_f32) p.k_f32 |= 1u << p.n_k;
+        ++p.n_k;
+    }
+    const int64_t per = kBlock / 64;
+    const int64_t grid = (c->n_elem + per - 1) / per;
+    p.partial = ptr;
+    double h[2] = {0.0, 0.0};
+    if (grid > 0) {
+        hipLaunchKernelGGL(step_error_norm, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
+        hipLaunchKernelGGL(reduce_pairs, dim3(1), dim3(kBlock), 0, c->stream, ptr, grid, ptr + 2 * kScratchBlocks);
+        if (hipGetLastError() != hipSuccess) return fail_noalloc(HGDG_ERR_HIP, "kernel launch failed");
+        if (hipMemcpyAsync(h, ptr + 2 * kScratchBlocks, sizeof(h), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return fail_noalloc(HGDG_ERR_HIP, "reading back the reduced sums failed");
+    }
+    *error = std::sqrt(h[0]);
+    *norm = h[1];
+    return HGDG_OK;
+}
+
 // HpgemAPISimplified::computeOneTimeStep(time, maximumRelativeError, dtMax)
 // (kernel/Base/HpgemAPISimplified_Impl.h:822-861): steps with the current
 // estimate (capped by dt_max), rewinds and shrinks dt while the embedded
@@ -534,6 +644,7 @@ int hgdg_ctx_create(hgdg_ctx** out, int device) {
         HGDG_HIP(hipGetDeviceCount(&count));
         require(device >= 0 && device < count, HGDG_ERR_ARG, "no HIP device " + std::to_string(device));
         HGDG_HIP(hipSetDevice(device));
+        hgdg::setup_scratch(device);
         std::unique_ptr<hgdg_ctx> c(new hgdg_ctx);
         c->device = device;
         HGDG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -797,6 +908,11 @@ int hgdg_step_error(hgdg_ctx* ctx, int solution, const int* aux_ids, int stages,
         require(!hgdg::vec_at(ctx, solution).is_f32, HGDG_ERR_ARG, "the solution vector must be stored in FP64");
         hgdg::step_error_and_norm(ctx, solution, aux_ids, stages, err_coef, dt, error, norm);
     });
+}
+
+int hgdg_step_error_noalloc(hgdg_ctx* ctx, int solution, const int* aux_ids, int stages, const double* err_coef,
+                            double dt, double* error, double* norm) {
+    return hgdg::step_error_noalloc(ctx, solution, aux_ids, stages, err_coef, dt, error, norm);
 }
 
 int hgdg_project_modes(hgdg_ctx* ctx, int vec, int n_modes, const double* amp, const double* k) {

@@ -213,6 +213,19 @@ int hgdg_rk_steps(hgdg_ctx* ctx, int solution, const int* aux_ids, int stages, c
 int hgdg_step_error(hgdg_ctx* ctx, int solution, const int* aux_ids, int stages, const double* err_coef, double dt,
                     double* error, double* norm);
 
+/* hgdg_step_error with the same inputs and results, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. It calls no hipMalloc,
+ * hipFree or other allocating routine: the per-block partial sums live in a
+ * fixed device array of 1 MiB that the library holds on every GPU it runs on
+ * until the process ends, reduced on the device, and 16 bytes are read back.
+ * Limit: meshes of at most 262144 elements (HGDG_ERR_UNSUPPORTED beyond).
+ * All calls on one GPU share that array: calls that overlap on one GPU race.
+ * The sums are reduced in a different order than hgdg_step_error's, so the
+ * two agree to rounding (~1e-16 relative), not bit for bit. */
+int hgdg_step_error_noalloc(hgdg_ctx* ctx, int solution, const int* aux_ids, int stages, const double* err_coef,
+                            double dt, double* error, double* norm);
+
 /* One adaptive step with an embedded tableau (err_coef = b - b_low, order =
  * the tableau's order): try dt = min(estimate, dt_max), rewind and retry with
  * dt *= 0.8 (tol * norm / error)^(1/order) while error > max_rel_error * norm,
