@@ -47,6 +47,9 @@ struct SimplexMesh {
     double tpe_ms[4][2] = {};                    // per order and candidate: timed RHS launch (ms), 0 if not timed
     DevBuf<int32_t> d_tab_index;                 // offsets and column lists into d_tables
     DevBuf<double> d_mass;                       // per order: M_ref [nb][nb], then M_ref^{-1}
+    DevBuf<double> d_qtab;                       // projection / error-norm quadrature tables (built on first use)
+    std::vector<int32_t> qtab_index;             // their offsets and sizes per order
+    DevBuf<double> d_xj;                         // [E][12]: vertex 0 and J (row-major), device order
     DevBuf<int32_t> d_order;                     // per device element: its order
     DevBuf<int64_t> d_elem_off;                  // [E + 1] first DOF of each device element
     DevBuf<double> d_geo;                        // J^{-1} [9][E], then c_inv [E], sigma [4][E], normal [3][4][E]
@@ -189,8 +192,8 @@ void simplex_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* l2, dou
 void simplex_mass_apply(hgdg_ctx* c, Vec& v, bool inverse);
 void simplex_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, const double* amp, const double* k,
                          const double* shift, double* l2, double* max_abs);
-void simplex_project_host(hgdg_ctx* c, Vec& v, int field, int n_modes, const double* amp, const double* k,
-                         const double* gauss = nullptr);
+void simplex_project(hgdg_ctx* c, Vec& v, int field, int n_modes, const double* amp, const double* k,
+                     const double* gauss = nullptr);
 void simplex_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev);
 void simplex_from_device(hgdg_ctx* c, const std::vector<double>& dev, double* host);
 

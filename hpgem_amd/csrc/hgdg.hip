@@ -927,7 +927,7 @@ int hgdg_project_field_modes(hgdg_ctx* ctx, int vec, int field, int n_modes, con
         require(ctx->has_mesh, HGDG_ERR_STATE, "no mesh");
         require(field >= -1 && field < ctx->n_fields, HGDG_ERR_ARG, "field out of range");
         hgdg::Vec& v = hgdg::vec_at(ctx, vec);
-        if (ctx->simplex()) hgdg::simplex_project_host(ctx, v, field, n_modes, amp, k);
+        if (ctx->simplex()) hgdg::simplex_project(ctx, v, field, n_modes, amp, k);
         else hgdg::tensor_project_device(ctx, v, field, n_modes, amp, k);
     });
 }
@@ -957,7 +957,7 @@ int hgdg_project_gaussian(hgdg_ctx* ctx, int vec, int field, const double* cente
         require(field >= -1 && field < ctx->n_fields, HGDG_ERR_ARG, "field out of range");
         hgdg::Vec& v = hgdg::vec_at(ctx, vec);
         const double g[5] = {center[0], center[1], ctx->dim == 3 ? center[2] : 0.0, width, amplitude};
-        if (ctx->simplex()) hgdg::simplex_project_host(ctx, v, field, 0, nullptr, nullptr, g);
+        if (ctx->simplex()) hgdg::simplex_project(ctx, v, field, 0, nullptr, nullptr, g);
         else hgdg::tensor_project_device(ctx, v, field, 0, nullptr, nullptr, g);
     });
 }

@@ -582,6 +582,125 @@ __global__ __launch_bounds__(64) void simplex_mass(const SimplexMassParams p) {
     }
 }
 
+// Quadrature tables of the on-device projection and error norms, per order
+// (index p - 1), as offsets into one table array: the element rule of order
+// 2p+1 (points; the projection operator R = M_ref^{-1} [w_q phi_i(x_q)], [nb][nq];
+// the basis at its points, [nq][nb], for the maximum error) and the rule of
+// order 2p+2 (points, weights and the basis at its points, for the L2 error),
+// all hpGEM's tetrahedral rules.
+struct SimplexQuadTab {
+    int32_t nb[4], nq_p[4], nq_e[4];
+    int32_t pts_p[4], R[4], phi_p[4], pts_e[4], w_e[4], phi_e[4];
+};
+
+struct SimplexFnParams {
+    const double* tab;
+    SimplexQuadTab t;
+    const double* xj;         // [E][12]: vertex 0, then J row-major (device order)
+    const int64_t* elem_off;  // [E + 1]
+    const int32_t* order;     // [E]
+    int32_t n_fields, field;  // field < 0: every field
+    int32_t n_modes;
+    const double* modes;      // [n_modes][4]: amplitude, k
+    int32_t gaussian;         // 1: f = g[4] exp(-|x - g[0..2]|^2 / g[3]^2) instead of the modes
+    double g[5];
+    double shift[3];          // error: f(x - shift)
+    void* u;                  // projection: output; error: input
+    int32_t u_f32;
+    const double* absdet;
+    double* partial;          // error: [E][2] (squared L2, max)
+};
+
+__device__ __forceinline__ void tet_point(const double* xj, const double* xi, double (&x)[3]) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) x[r] = xj[r] + xj[3 + r * 3] * xi[0] + xj[3 + r * 3 + 1] * xi[1] + xj[3 + r * 3 + 2] * xi[2];
+}
+
+__device__ __forceinline__ double tet_fn(const SimplexFnParams& p, const double (&x)[3]) {
+    if (p.gaussian) {
+        const double r2 = (x[0] - p.g[0]) * (x[0] - p.g[0]) + (x[1] - p.g[1]) * (x[1] - p.g[1]) +
+                          (x[2] - p.g[2]) * (x[2] - p.g[2]);
+        return p.g[4] * exp(-r2 / (p.g[3] * p.g[3]));
+    }
+    double f = 0;
+    for (int m = 0; m < p.n_modes; ++m) {
+        const double* md = p.modes + 4 * m;
+        f += md[0] * sin(2 * M_PI * (md[1] * (x[0] - p.shift[0]) + md[2] * (x[1] - p.shift[1]) +
+                                     md[3] * (x[2] - p.shift[2])));
+    }
+    return f;
+}
+
+// L2 projection (HpgemAPISimplified::setInitialSolution with hpGEM's element
+// rule of order 2p+1): one wavefront per element, lanes over points, then
+// over basis functions.
+__global__ __launch_bounds__(64) void simplex_project(const SimplexFnParams p) {
+    __shared__ double fq[64];
+    const int64_t e = blockIdx.x;
+    const int P = p.order[e], nb = p.t.nb[P - 1], nq = p.t.nq_p[P - 1];
+    const double* xj = p.xj + 12 * e;
+    const int l = threadIdx.x;
+    if (l < nq) {
+        double x[3];
+        tet_point(xj, p.tab + p.t.pts_p[P - 1] + 3 * l, x);
+        fq[l] = tet_fn(p, x);
+    }
+    __syncthreads();
+    if (l < nb) {
+        const double* R = p.tab + p.t.R[P - 1] + l * nq;
+        double s = 0;
+        for (int q = 0; q < nq; ++q) s += R[q] * fq[q];
+        const int64_t o = p.elem_off[e];
+        for (int f = 0; f < p.n_fields; ++f) {
+            if (p.field >= 0 && f != p.field) continue;
+            if (p.u_f32) static_cast<float*>(p.u)[o + f * nb + l] = float(s);
+            else static_cast<double*>(p.u)[o + f * nb + l] = s;
+        }
+    }
+}
+
+// computeTotalError (rule 2p+2) and computeMaxError (the points of the rule
+// 2p+1) of one field or all, per element: partial[2e] = |det J| sum_q w_q
+// err^2, partial[2e+1] = max |err|.
+__global__ __launch_bounds__(64) void simplex_error(const SimplexFnParams p) {
+    __shared__ double cf[4 * SimplexShape<4>::NB];
+    const int64_t e = blockIdx.x;
+    const int P = p.order[e], nb = p.t.nb[P - 1];
+    const double* xj = p.xj + 12 * e;
+    const int64_t o = p.elem_off[e];
+    const int l = threadIdx.x;
+    for (int k = l; k < p.n_fields * nb; k += 64)
+        cf[k] = p.u_f32 ? double(static_cast<const float*>(p.u)[o + k]) : static_cast<const double*>(p.u)[o + k];
+    __syncthreads();
+    double s2 = 0, mx = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int nq = pass == 0 ? p.t.nq_e[P - 1] : p.t.nq_p[P - 1];
+        const double* pts = p.tab + (pass == 0 ? p.t.pts_e[P - 1] : p.t.pts_p[P - 1]);
+        const double* phi = p.tab + (pass == 0 ? p.t.phi_e[P - 1] : p.t.phi_p[P - 1]);
+        for (int q = l; q < nq; q += 64) {
+            double x[3];
+            tet_point(xj, pts + 3 * q, x);
+            const double ex = tet_fn(p, x);
+            for (int f = 0; f < p.n_fields; ++f) {
+                if (p.field >= 0 && f != p.field) continue;
+                double val = 0;
+                for (int i = 0; i < nb; ++i) val += cf[f * nb + i] * phi[q * nb + i];
+                const double err = ex - val;
+                if (pass == 0) s2 += p.tab[p.t.w_e[P - 1] + q] * err * err;
+                else mx = fmax(mx, fabs(err));
+            }
+        }
+    }
+    for (int w = 32; w > 0; w /= 2) {
+        s2 += __shfl_xor(s2, w, 64);
+        mx = fmax(mx, __shfl_xor(mx, w, 64));
+    }
+    if (l == 0) {
+        p.partial[2 * e] = s2 * p.absdet[e];
+        p.partial[2 * e + 1] = mx;
+    }
+}
+
 // ---------------------------------------------------------------- host side
 namespace {
 
@@ -1130,100 +1249,138 @@ TetProjector tet_projector(int p) {
 // L2 projection of sine modes or a Gaussian pulse (see tensor_project_device)
 // into one field or all fields of a tetrahedral vector, computed on the host
 // element by element and uploaded; the other fields keep their values.
-void simplex_project_host(hgdg_ctx* c, Vec& v, int field, int n_modes, const double* amp, const double* k,
-                          const double* gauss) {
+namespace {
+
+// The quadrature tables of the on-device projection and error norms, and the
+// elements' affine maps, uploaded on first use.
+SimplexQuadTab simplex_quad_tables(hgdg_ctx* c) {
     SimplexMesh& sm = c->sm;
-    std::vector<double> dev;
-    download_vector(c, v, dev);
-    TetProjector pr[kMaxSimplexOrder];
-    std::vector<double> fq;
-    for (int64_t d = 0; d < c->n_elem; ++d) {
-        const int p = sm.order[d];
-        TetProjector& P = pr[p - 1];
-        if (P.nb == 0) P = tet_projector(p);
-        fq.assign(size_t(P.nq), 0.0);
-        const double* x0 = &sm.x0[size_t(d) * 3];
-        const double* J = &sm.jac[size_t(d) * 9];
-        for (int q = 0; q < P.nq; ++q) {
-            const double* xi = &P.pts[size_t(q) * 3];
-            double x[3];
-            for (int r = 0; r < 3; ++r) x[r] = x0[r] + J[r * 3 + 0] * xi[0] + J[r * 3 + 1] * xi[1] + J[r * 3 + 2] * xi[2];
-            double f = 0;
-            if (gauss) {
-                const double r2 = (x[0] - gauss[0]) * (x[0] - gauss[0]) + (x[1] - gauss[1]) * (x[1] - gauss[1]) +
-                                  (x[2] - gauss[2]) * (x[2] - gauss[2]);
-                f = gauss[4] * std::exp(-r2 / (gauss[3] * gauss[3]));
-            } else {
-                for (int m = 0; m < n_modes; ++m)
-                    f += amp[m] * std::sin(2 * M_PI * (k[3 * m] * x[0] + k[3 * m + 1] * x[1] + k[3 * m + 2] * x[2]));
+    SimplexQuadTab t{};
+    if (sm.qtab_index.empty()) {
+        std::vector<double> tab;
+        std::vector<int32_t> ix(40, 0);
+        for (int p = 1; p <= kMaxSimplexOrder; ++p) {
+            const int g = p - 1, nb = tet_nb(p);
+            const TetProjector P = tet_projector(p);
+            ix[g] = nb;
+            ix[4 + g] = P.nq;
+            ix[12 + g] = int32_t(tab.size());
+            tab.insert(tab.end(), P.pts.begin(), P.pts.end());
+            ix[16 + g] = int32_t(tab.size());
+            tab.insert(tab.end(), P.R.begin(), P.R.end());
+            std::vector<long double> phi(static_cast<size_t>(nb)), grad(static_cast<size_t>(nb) * 3);
+            ix[20 + g] = int32_t(tab.size());
+            for (int q = 0; q < P.nq; ++q) {
+                const long double xi[3] = {P.pts[3 * q], P.pts[3 * q + 1], P.pts[3 * q + 2]};
+                tet_basis(p, xi, phi.data(), grad.data());
+                for (int i = 0; i < nb; ++i) tab.push_back(double(phi[i]));
             }
-            fq[q] = f;
+            std::vector<long double> pts, w;
+            ref_tet_rule(2 * p + 2, pts, w);
+            const int nq = int(w.size());
+            ix[8 + g] = nq;
+            ix[24 + g] = int32_t(tab.size());
+            tab.insert(tab.end(), pts.begin(), pts.end());
+            ix[28 + g] = int32_t(tab.size());
+            tab.insert(tab.end(), w.begin(), w.end());
+            ix[32 + g] = int32_t(tab.size());
+            for (int q = 0; q < nq; ++q) {
+                tet_basis(p, &pts[size_t(q) * 3], phi.data(), grad.data());
+                for (int i = 0; i < nb; ++i) tab.push_back(double(phi[i]));
+            }
+            require(P.nq <= 64, HGDG_ERR_UNSUPPORTED, "projection rule with more than 64 points");
         }
-        for (int i = 0; i < P.nb; ++i) {
-            double s = 0;
-            for (int q = 0; q < P.nq; ++q) s += P.R[size_t(i) * P.nq + q] * fq[q];
-            for (int fld = 0; fld < c->n_fields; ++fld)
-                if (field < 0 || field == fld) dev[size_t(sm.dof_off[d] + int64_t(fld) * P.nb + i)] = s;
+        std::vector<double> xj(size_t(c->n_elem) * 12);
+        for (int64_t d = 0; d < c->n_elem; ++d) {
+            for (int r = 0; r < 3; ++r) xj[size_t(d) * 12 + r] = sm.x0[size_t(d) * 3 + r];
+            for (int r = 0; r < 9; ++r) xj[size_t(d) * 12 + 3 + r] = sm.jac[size_t(d) * 9 + r];
         }
+        sm.d_qtab.upload(tab, c->stream);
+        sm.d_xj.upload(xj, c->stream);
+        HGDG_HIP(hipStreamSynchronize(c->stream));  // the host vectors go out of scope
+        sm.qtab_index = ix;
     }
-    upload_vector(c, v, dev);
+    const std::vector<int32_t>& ix = sm.qtab_index;
+    for (int g = 0; g < 4; ++g) {
+        t.nb[g] = ix[g];
+        t.nq_p[g] = ix[4 + g];
+        t.nq_e[g] = ix[8 + g];
+        t.pts_p[g] = ix[12 + g];
+        t.R[g] = ix[16 + g];
+        t.phi_p[g] = ix[20 + g];
+        t.pts_e[g] = ix[24 + g];
+        t.w_e[g] = ix[28 + g];
+        t.phi_e[g] = ix[32 + g];
+    }
+    return t;
+}
+
+SimplexFnParams simplex_fn_params(hgdg_ctx* c, const Vec& v, int field, int n_modes, const double* amp,
+                                  const double* k, const double* gauss, DevBuf<double>& d_modes) {
+    SimplexMesh& sm = c->sm;
+    SimplexFnParams p{};
+    p.t = simplex_quad_tables(c);
+    p.tab = sm.d_qtab.ptr;
+    p.xj = sm.d_xj.ptr;
+    p.elem_off = sm.d_elem_off.ptr;
+    p.order = sm.d_order.ptr;
+    p.n_fields = c->n_fields;
+    p.field = field;
+    p.u = v.ptr();
+    p.u_f32 = v.is_f32;
+    p.absdet = sm.d_absdet.ptr;
+    if (gauss) {
+        p.gaussian = 1;
+        for (int q = 0; q < 5; ++q) p.g[q] = gauss[q];
+    } else if (n_modes > 0) {
+        std::vector<double> md(size_t(4) * n_modes);
+        for (int m = 0; m < n_modes; ++m) {
+            md[4 * m] = amp[m];
+            for (int d = 0; d < 3; ++d) md[4 * m + 1 + d] = k[3 * m + d];
+        }
+        d_modes.upload(md, c->stream);
+        HGDG_HIP(hipStreamSynchronize(c->stream));
+        p.n_modes = n_modes;
+        p.modes = d_modes.ptr;
+    }
+    return p;
+}
+
+}  // namespace
+
+// L2 projection of sine modes or a Gaussian pulse, on the device.
+void simplex_project(hgdg_ctx* c, Vec& v, int field, int n_modes, const double* amp, const double* k,
+                     const double* gauss) {
+    require(c->sm.d_order.n == size_t(c->n_elem), HGDG_ERR_STATE, "set the operator before projecting");
+    DevBuf<double> d_modes;
+    const SimplexFnParams p = simplex_fn_params(c, v, field, n_modes, amp, k, gauss, d_modes);
+    if (c->n_elem > 0) hipLaunchKernelGGL(simplex_project, dim3(unsigned(c->n_elem)), dim3(64), 0, c->stream, p);
+    HGDG_HIP(hipGetLastError());
+    HGDG_HIP(hipStreamSynchronize(c->stream));  // before the modes table goes
 }
 
 // Error against f(x - shift) for sine modes f, as tensor_error_modes: L2 with
 // hpGEM's tetrahedral rule of order 2p+2, maximum over the points of the
-// element rule (order 2p+1). Host side, after a download.
+// element rule (order 2p+1), on the device; the per-element sums are added
+// on the host in element order.
 void simplex_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, const double* amp, const double* k,
                          const double* shift, double* l2, double* max_abs) {
-    SimplexMesh& sm = c->sm;
-    std::vector<double> dev;
-    download_vector(c, const_cast<Vec&>(v), dev);
-    struct Pts {
-        int nq = 0, nb = 0;
-        std::vector<double> x, w, phi;  // [nq][3], [nq], [nq][nb]
-    };
-    auto make = [](int p, int degree) {
-        Pts t;
-        std::vector<long double> pts, w;
-        ref_tet_rule(degree, pts, w);
-        t.nq = int(w.size());
-        t.nb = tet_nb(p);
-        t.x.assign(pts.begin(), pts.end());
-        t.w.assign(w.begin(), w.end());
-        std::vector<long double> phi(size_t(t.nb)), grad(size_t(t.nb) * 3);
-        for (int q = 0; q < t.nq; ++q) {
-            tet_basis(p, &pts[size_t(q) * 3], phi.data(), grad.data());
-            for (int i = 0; i < t.nb; ++i) t.phi.push_back(double(phi[i]));
-        }
-        return t;
-    };
-    Pts rules[2][kMaxSimplexOrder];
+    require(c->sm.d_order.n == size_t(c->n_elem), HGDG_ERR_STATE, "set the operator before computing errors");
+    DevBuf<double> d_modes;
+    SimplexFnParams p = simplex_fn_params(c, v, field, n_modes, amp, k, nullptr, d_modes);
+    for (int d = 0; d < 3; ++d) p.shift[d] = shift[d];
+    const int64_t ne = c->n_elem;
+    if (c->d_partial.n < size_t(2 * ne)) c->d_partial.alloc(size_t(2 * ne));
+    p.partial = c->d_partial.ptr;
+    if (ne > 0) hipLaunchKernelGGL(simplex_error, dim3(unsigned(ne)), dim3(64), 0, c->stream, p);
+    HGDG_HIP(hipGetLastError());
+    std::vector<double> h(size_t(2 * ne));
+    HGDG_HIP(hipMemcpyAsync(h.data(), c->d_partial.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HGDG_HIP(hipStreamSynchronize(c->stream));
     double sum = 0, mx = 0;
-    for (int64_t d = 0; d < c->n_elem; ++d) {
-        const int p = sm.order[d];
-        const double* x0 = &sm.x0[size_t(d) * 3];
-        const double* J = &sm.jac[size_t(d) * 9];
-        for (int pass = 0; pass < 2; ++pass) {
-            Pts& R = rules[pass][p - 1];
-            if (R.nq == 0) R = make(p, pass == 0 ? 2 * p + 2 : 2 * p + 1);
-            for (int q = 0; q < R.nq; ++q) {
-                const double* xi = &R.x[size_t(q) * 3];
-                double x[3];
-                for (int r = 0; r < 3; ++r) x[r] = x0[r] + J[r * 3] * xi[0] + J[r * 3 + 1] * xi[1] + J[r * 3 + 2] * xi[2];
-                double ex = 0;
-                for (int m = 0; m < n_modes; ++m)
-                    ex += amp[m] * std::sin(2 * M_PI * (k[3 * m] * (x[0] - shift[0]) + k[3 * m + 1] * (x[1] - shift[1]) +
-                                                        k[3 * m + 2] * (x[2] - shift[2])));
-                for (int fld = 0; fld < c->n_fields; ++fld) {
-                    if (field >= 0 && fld != field) continue;
-                    double val = 0;
-                    const double* cf = &dev[size_t(sm.dof_off[d] + int64_t(fld) * R.nb)];
-                    for (int i = 0; i < R.nb; ++i) val += cf[i] * R.phi[size_t(q) * R.nb + i];
-                    const double err = ex - val;
-                    if (pass == 0) sum += R.w[q] * sm.absdet[d] * err * err;
-                    else mx = std::max(mx, std::fabs(err));
-                }
-            }
-        }
+    for (int64_t d = 0; d < ne; ++d) {
+        sum += h[2 * d];
+        mx = std::max(mx, h[2 * d + 1]);
     }
     *l2 = std::sqrt(sum);
     *max_abs = mx;

@@ -240,8 +240,7 @@ int hgdg_rk_adaptive_step(hgdg_ctx* ctx, int solution, const int* aux_ids, int s
 
 /* vec = L2 projection of f(x) = sum_m amp[m] sin(2 pi k[m] . x), k given as
  * [n_modes][3] (unused components ignored), with hpGEM's element rule of
- * order 2p+1 on squares and cubes (a collapsed Gauss rule exact to degree
- * 2p+3 on tetrahedra, where the projection is formed on the host). Replaces HpgemAPISimplified::setInitialSolution
+ * order 2p+1 (squares, cubes and tetrahedra), on the device. Replaces HpgemAPISimplified::setInitialSolution
  * (kernel/Base/HpgemAPISimplified_Impl.h:227-291,751-758) for this family of
  * initial conditions. */
 int hgdg_project_modes(hgdg_ctx* ctx, int vec, int n_modes, const double* amp, const double* k);
@@ -255,8 +254,8 @@ int hgdg_project_field_modes(hgdg_ctx* ctx, int vec, int field, int n_modes, con
  * all fields (field = -1): l2 = sqrt(sum_e int_e |f - u_h|^2) with the rule of
  * order 2p+2, max_abs = the largest |f - u_h| at the element rule's points.
  * Replaces HpgemAPISimplified::computeTotalError / computeMaxError
- * (kernel/Base/HpgemAPISimplified_Impl.h:293-530). On the device for squares
- * and cubes; on the host, with collapsed Gauss rules, for tetrahedra. */
+ * (kernel/Base/HpgemAPISimplified_Impl.h:293-530), with hpGEM's rules, on the
+ * device (squares, cubes and tetrahedra). */
 int hgdg_error_modes(hgdg_ctx* ctx, int vec, int field, int n_modes, const double* amp, const double* k,
                      const double* shift, double* l2, double* max_abs);
 

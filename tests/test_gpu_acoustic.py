@@ -296,7 +296,7 @@ def test_tet_mixed_within_1e6_after_rk4():
 
 def test_tet_projection_is_the_l2_projection():
     """Sine modes projected into the pressure-like field of an hp tet mesh
-    (host-side, with hpGEM's tetrahedral rule of order 2p+1) equal the
+    (on the device, with hpGEM's tetrahedral rule of order 2p+1) equal the
     oracle's projection, and the other fields stay zero."""
     mesh = Mesh.structured([3, 3, 3], periodic=[0, 0, 0], simplices=True)
     rng = np.random.default_rng(8)
@@ -339,8 +339,8 @@ def test_gaussian_pulse_projection_keeps_its_norm(simplices):
 
 
 def test_tet_error_norm_matches_oracle_to_quadrature_accuracy():
-    """computeTotalError on an hp tet mesh (host-side, hpGEM's tetrahedral
-    rule of order 2p+2) against the oracle's."""
+    """computeTotalError on an hp tet mesh (on the device, hpGEM's
+    tetrahedral rule of order 2p+2) against the oracle's."""
     mesh = Mesh.structured([3, 3, 3], periodic=[0, 0, 0], simplices=True)
     rng = np.random.default_rng(9)
     p = rng.integers(1, 5, size=mesh.n_elements).astype(np.int32)
