@@ -37,11 +37,11 @@ SEED = 1
 # HBM bytes per launch of tensor_advection_rhs<3,4,T> on this workload, from
 # `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of
 # `bench.py --profile-workload`, each scaled by what the counter reads for a
-# plain copy of one 16 MiB vector (FETCH_SIZE x 1.9977, WRITE_SIZE x 1); the
+# plain copy of one 16 MiB vector (FETCH_SIZE x 1.9979, WRITE_SIZE x 1); the
 # commands and per-kernel figures are in
 # profiles/rhs_kernels_32cube_p3_and_tet24.md. Counters cannot be read from
 # inside this process, so the figure is reported as measured there.
-MEASURED_TRAFFIC = {"double": 56.42e6, "float": 37.45e6}
+MEASURED_TRAFFIC = {"double": 57.88e6, "float": 37.43e6}
 # dt = CFL h / |a|. At CFL 0.1 the RK4 step of this p = 3 operator has
 # spectral radius ~1.47 (power iteration on the oracle, 4^3 periodic mesh), so
 # round-off grows ~1e8-fold over 50 steps; 0.05 lies inside the stability

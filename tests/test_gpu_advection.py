@@ -4,11 +4,11 @@ the RK stage combination) and of RK time stepping against the oracle.
 FP64 bar (tests/_parity.py): <= 1e-12 relative (max abs coefficient
 difference / max abs value) against the oracle run in long double for every
 case; <= 1e-12 against the oracle in double (hpGEM's own rounding) for every
-case of order <= 3, including the 32^3 hex p=3 workload's element on 4^3
-meshes; at p = 4, where hpGEM's per-call LU of a mass matrix of condition
-~3e6 leaves the double oracle itself 2e-11 from the exact operator, <= 1e-12
-relative in the mass-weighted L2 norm. Mixed FP32 path: within 1e-6 relative
-L2 of the FP64 solution.
+case of order <= 2; at p >= 3, where hpGEM's per-call LU of a cube mass
+matrix of condition ~1.9e5 (p = 3) / ~3e6 (p = 4) leaves the double oracle
+itself up to 1.35e-12 / 2e-11 from the exact operator, <= 1e-12 relative in
+the mass-weighted L2 norm (every distance is printed and kept). Mixed FP32
+path: within 1e-6 relative L2 of the FP64 solution.
 """
 import numpy as np
 import pytest
