@@ -43,6 +43,8 @@ SIGNATURES = [
     ("hgdg_get_advection_tiles", ctypes.c_int, [ctypes.c_void_p, _INT, _D]),
     ("hgdg_set_rk_fused", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hgdg_get_rk_fused", ctypes.c_int, [ctypes.c_void_p, _INT, _I64]),
+    ("hgdg_set_tensor_general", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hgdg_get_tensor_general", ctypes.c_int, [ctypes.c_void_p, _INT, _INT]),
     ("hgdg_set_simplex_threads", ctypes.c_int, [ctypes.c_void_p, _I32]),
     ("hgdg_get_simplex_threads", ctypes.c_int, [ctypes.c_void_p, _I32, _D]),
     ("hgdg_state_alloc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),

@@ -161,7 +161,7 @@ class HpgemAPIDevice:
     """
 
     def __init__(self, mesh, polynomial_order, velocity=None, precision="fp64", tableau=RK4,
-                 n_extra_vectors=0, device=0, kind=_lib.OP_ADVECTION_UPWIND, c_inv=None):
+                 n_extra_vectors=0, device=0, kind=_lib.OP_ADVECTION_UPWIND, c_inv=None, tensor_general=False):
         self.lib = _lib.load()
         self.mesh = mesh
         self.p = polynomial_order
@@ -170,6 +170,8 @@ class HpgemAPIDevice:
         h = ctypes.c_void_p()
         check(self.lib.hgdg_ctx_create(ctypes.byref(h), device))
         self.h = h
+        if tensor_general:
+            check(self.lib.hgdg_set_tensor_general(self.h, 1))
         self._order = np.ascontiguousarray(np.broadcast_to(np.asarray(polynomial_order, dtype=np.int32),
                                                            (mesh.n_elements,)))
         desc = _lib.MeshDesc(mesh.shape, mesh.dim, mesh.n_elements, dptr(mesh.coords),
@@ -203,6 +205,14 @@ class HpgemAPIDevice:
             self.close()
         except Exception:
             pass
+
+    def usesTensorGeneral(self):
+        """Whether the mesh runs on the Gauss-point kernels (non-affine
+        quadrilaterals/hexahedra, unaligned neighbours, or forced by
+        tensor_general=True) rather than the sum-factorised ones."""
+        f, u = ctypes.c_int(), ctypes.c_int()
+        check(self.lib.hgdg_get_tensor_general(self.h, ctypes.byref(f), ctypes.byref(u)))
+        return bool(u.value)
 
     # ---- launch shape of the tensor advection kernel ----------------------
     def setAdvectionTiles(self, tiles):

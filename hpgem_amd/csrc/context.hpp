@@ -27,6 +27,14 @@ struct TensorMesh {
     DevBuf<uint32_t> d_mode;         // advection: upwind mode, 3 bits per face
     bool structured = false;         // periodic unit box built by tensor_structured_upload
     int32_t cells[3] = {1, 1, 1};
+    // Gauss-point path (tensor_general.hip): meshes with non-affine elements
+    // or unaligned neighbours, or any mesh when the caller asks for it
+    bool general = false;
+    std::vector<double> gX;          // [e][2^dim][dim] vertex coordinates
+    std::vector<int32_t> gnbr;       // [2 dim][e] neighbour per local face (hpGEM numbering) or -1
+    std::vector<int32_t> ginfo;      // [2 dim][e] neighbour's face, point map, left flag
+    DevBuf<double> d_gX, d_gtab, d_gcinv;
+    DevBuf<int32_t> d_gface;         // gnbr then ginfo
 };
 
 // Tetrahedra with per-element order 1..4, grouped by order on the device.
@@ -102,6 +110,9 @@ struct hgdg_ctx {
     // elsewhere (tensor advection) the RK update fused into each step's last
     // stage; 0: always the graph of separate launches
     int rk_fused = 1;
+    // 1: tensor meshes uploaded from now on take the Gauss-point path even when
+    // every element is affine and every neighbour aligned
+    int tensor_general_forced = 0;
     int64_t rk_fused_launches = 0;
     double adv_tile_ms[4] = {0, 0, 0, 0};  // timed candidates 1, 2, 4, 8 (0 = not timed)
 
@@ -152,6 +163,17 @@ void tensor_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, const
                         const double* shift, double* l2, double* max_abs);
 void tensor_to_device(hgdg_ctx* c, const double* host, std::vector<double>& dev);
 void tensor_from_device(hgdg_ctx* c, const std::vector<double>& dev, double* host);
+
+// tensor_general.hip: the Gauss-point path (non-affine elements, unaligned faces)
+void tensor_general_upload(hgdg_ctx* c, const hgdg_mesh_desc* m);
+void tensor_general_operator_set(hgdg_ctx* c, const hgdg_op_desc* op);
+void tensor_general_rhs(hgdg_ctx* c, const StageIO& io);
+void tensor_general_mass_apply(hgdg_ctx* c, Vec& v, bool inverse);
+void tensor_general_project(hgdg_ctx* c, Vec& v, int field, int n_modes, const double* amp, const double* k,
+                            const double* gauss);
+void tensor_general_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, const double* amp,
+                                const double* k, const double* shift, double* l2, double* max_abs);
+void tensor_general_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* l2, double* max_abs);
 
 // batched tensor launches (serving)
 size_t tensor_batch_param_bytes(const hgdg_ctx* c);

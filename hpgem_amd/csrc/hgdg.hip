@@ -217,7 +217,7 @@ void launch_rhs(hgdg_ctx* c, const int* in_ids, const double* coeffs, int n_in, 
 
 // Whether the operator's RHS kernel performs a fused RK update (StageIO::upd).
 bool rhs_fuses_update(const hgdg_ctx* c) {
-    return !c->simplex() && c->op.kind == HGDG_OP_ADVECTION_UPWIND;
+    return !c->simplex() && !c->tm.general && c->op.kind == HGDG_OP_ADVECTION_UPWIND;
 }
 
 void launch_axpy(hgdg_ctx* c, int to, const std::vector<int>& add, const std::vector<double>& scale) {
@@ -735,6 +735,22 @@ int hgdg_get_rk_fused(hgdg_ctx* ctx, int* enable, int64_t* launches) {
         require(ctx != nullptr && enable != nullptr, HGDG_ERR_ARG, "ctx and enable must be non-NULL");
         *enable = ctx->rk_fused;
         if (launches) *launches = ctx->rk_fused_launches;
+    });
+}
+
+int hgdg_set_tensor_general(hgdg_ctx* ctx, int force) {
+    return run([&] {
+        require(ctx != nullptr, HGDG_ERR_ARG, "ctx is NULL");
+        require(force == 0 || force == 1, HGDG_ERR_ARG, "force must be 0 or 1");
+        ctx->tensor_general_forced = force;
+    });
+}
+
+int hgdg_get_tensor_general(hgdg_ctx* ctx, int* force, int* in_use) {
+    return run([&] {
+        require(ctx != nullptr && force != nullptr && in_use != nullptr, HGDG_ERR_ARG, "NULL argument");
+        *force = ctx->tensor_general_forced;
+        *in_use = ctx->has_mesh && !ctx->simplex() && ctx->tm.general ? 1 : 0;
     });
 }
 

@@ -249,6 +249,66 @@ TensorOps tensor_ops(int p) {
     return ops;
 }
 
+NodalOps nodal_ops(int p) {
+    NodalOps o;
+    const int n = p + 1;
+    o.n1 = n;
+    std::vector<long double> x, w, x2, w2;
+    gauss_legendre_ld(n, x, w);
+    gauss_legendre_ld(n + 1, x2, w2);
+    // barycentric weights of the points: l_m'(x_q) = (bw_m / bw_q) / (x_q - x_m)
+    // for q != m, and sum_{j != m} 1 / (x_m - x_j) on the diagonal
+    std::vector<long double> bw(n, 1.0L);
+    for (int m = 0; m < n; ++m)
+        for (int j = 0; j < n; ++j)
+            if (j != m) bw[m] /= (x[m] - x[j]);
+    std::vector<long double> B(size_t(n) * n);
+    for (int q = 0; q < n; ++q)
+        for (int a = 0; a < n; ++a) B[q * n + a] = psi_ld(a, x[q]);
+    const std::vector<long double> Bi = invert(n, B);
+    auto lagrange = [&](int m, long double t) {
+        long double v = 1;
+        for (int j = 0; j < n; ++j)
+            if (j != m) v *= (t - x[j]) / (x[m] - x[j]);
+        return v;
+    };
+    for (int q = 0; q < n; ++q) {
+        o.x.push_back(static_cast<double>(x[q]));
+        o.w.push_back(static_cast<double>(w[q]));
+        o.lam0.push_back(static_cast<double>(lagrange(q, -1.0L)));
+        o.lam1.push_back(static_cast<double>(lagrange(q, 1.0L)));
+    }
+    for (int i = 0; i < n * n; ++i) {
+        o.B.push_back(static_cast<double>(B[i]));
+        o.Binv.push_back(static_cast<double>(Bi[i]));
+    }
+    for (int q = 0; q < n; ++q)
+        for (int m = 0; m < n; ++m) {
+            long double d = 0;
+            if (q == m) {
+                for (int j = 0; j < n; ++j)
+                    if (j != m) d += 1.0L / (x[m] - x[j]);
+            } else {
+                d = (bw[m] / bw[q]) / (x[q] - x[m]);
+            }
+            o.C.push_back(static_cast<double>(d));
+        }
+    for (int q = 0; q <= n; ++q) {
+        o.x2.push_back(static_cast<double>(x2[q]));
+        o.w2.push_back(static_cast<double>(w2[q]));
+    }
+    for (int a = 0; a < n; ++a)
+        for (int q = 0; q <= n; ++q) o.B2.push_back(static_cast<double>(psi_ld(a, x2[q])));
+    return o;
+}
+
+std::vector<double> NodalOps::packed() const {
+    std::vector<double> t;
+    for (const std::vector<double>* v : {&x, &w, &B, &Binv, &C, &lam0, &lam1, &x2, &w2, &B2})
+        t.insert(t.end(), v->begin(), v->end());
+    return t;
+}
+
 StructuredMesh structured_mesh(int dim, const int32_t* n, const double* lo, const double* hi,
                                const int32_t* periodic, bool simplices) {
     if (dim != 2 && dim != 3) throw std::invalid_argument("structured mesh needs dim 2 or 3");

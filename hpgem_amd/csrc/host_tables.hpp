@@ -47,6 +47,30 @@ struct TensorOps {
 };
 TensorOps tensor_ops(int p);
 
+// Tables of the Gauss-point form of an order-p tensor element, used where the
+// element map is not affine or a neighbour's face is not aligned. The element
+// rule hpGEM selects for order 2p+1 is the n1 = p+1 point Gauss rule per
+// direction, so B(q,a) = psi_a(x_q) is square and invertible and the element
+// mass matrix B^T diag(w |det J|) B (B = B1 (x) B1 (x) B1) has the exact
+// inverse B^{-1} diag(1 / (w |det J|)) B^{-T}. All n1 x n1 tables row-major:
+//   x, w     the n1 Gauss points (ascending) and weights;
+//   B, Binv  psi_a(x_q) [q][a] and its inverse [a][q];
+//   C        C(q,m) = l_m'(x_q), l_m the Lagrange polynomials of the points:
+//            the derivative of the interpolant of point values (C = G B^{-1},
+//            G(q,a) = psi_a'(x_q));
+//   lam0/1   l_q(-1), l_q(+1): B^{-T} e_0, B^{-T} e_1 (a trace of the
+//            hierarchical set at x = -1 / +1 is the coefficient with index 0 / 1);
+//   x2, w2   the n1+1 point rule (error norms, rule order 2p+2), B2 = psi_a at
+//            its points [a][q].
+// Formed in long double.
+struct NodalOps {
+    int n1 = 0;
+    std::vector<double> x, w, B, Binv, C, lam0, lam1, x2, w2, B2;
+    // all of the above in this order, as one table for the device
+    std::vector<double> packed() const;
+};
+NodalOps nodal_ops(int p);
+
 // Structured mesh in hpGEM Preprocessor order
 // (applications/Preprocessor/lib/hpgem.cpp:100-235).
 struct StructuredMesh {

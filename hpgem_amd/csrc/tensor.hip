@@ -1481,8 +1481,13 @@ void tensor_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     c->tm.n1 = p + 1;
     c->tm.nb = dim == 3 ? c->tm.n1 * c->tm.n1 * c->tm.n1 : c->tm.n1 * c->tm.n1;
     c->tm.perm = tensor_permutation(dim, p);
+    if (c->tensor_general_forced) {
+        tensor_general_upload(c, m);
+        return;
+    }
 
-    // geometry: affine hypercubes only; J column d = (X_{2^d} - X_0) / 2
+    // geometry: affine hypercubes here (J column d = (X_{2^d} - X_0) / 2);
+    // any other element sends the mesh to the Gauss-point path
     c->tm.Jinv.assign(size_t(ne) * dim * dim, 0.0);
     c->tm.absdet.assign(ne, 0.0);
     c->tm.X0.assign(size_t(ne) * dim, 0.0);
@@ -1505,8 +1510,10 @@ void tensor_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
                 double pred = X[r];
                 for (int d = 0; d < dim; ++d)
                     if ((v >> d) & 1) pred += E[d][r];
-                require(std::fabs(pred - X[v * dim + r]) <= 1e-12 * scale, HGDG_ERR_UNSUPPORTED,
-                        "element " + std::to_string(e) + " is not affine (parallelogram/parallelepiped)");
+                if (!(std::fabs(pred - X[v * dim + r]) <= 1e-12 * scale)) {
+                    tensor_general_upload(c, m);
+                    return;
+                }
             }
         affine_geometry(c, e, E);
     }
@@ -1549,14 +1556,16 @@ void tensor_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
             const int fo = it->second.second;
             require(o != e, HGDG_ERR_UNSUPPORTED, "element " + std::to_string(e) + " meets itself across a face");
             require(c->tm.nbr[size_t(fo) * ne + o] < 0, HGDG_ERR_ARG, "a face is bounded by more than two elements");
-            // tensor kernels need the neighbour's opposite face in the same
-            // orientation: my face (d, s) against its (d, 1-s), vertex by vertex
-            require(fo / 2 == f / 2 && fo % 2 != f % 2, HGDG_ERR_UNSUPPORTED,
-                    "neighbouring elements are not aligned (different local face directions)");
+            // the sum-factorised kernels need the neighbour's opposite face in
+            // the same orientation: my face (d, s) against its (d, 1-s), vertex
+            // by vertex; any other neighbour sends the mesh to the Gauss-point path
+            bool aligned = fo / 2 == f / 2 && fo % 2 != f % 2;
             const int64_t* onid = m->node_ids + size_t(o) * nv;
-            for (int q = 0; q < fverts; ++q)
-                require(onid[face_vertex(fo, q)] == nid[face_vertex(f, q)], HGDG_ERR_UNSUPPORTED,
-                        "neighbouring elements are not aligned (rotated face)");
+            for (int q = 0; q < fverts && aligned; ++q) aligned = onid[face_vertex(fo, q)] == nid[face_vertex(f, q)];
+            if (!aligned) {
+                tensor_general_upload(c, m);
+                return;
+            }
             c->tm.nbr[size_t(f) * ne + e] = int32_t(o);
             c->tm.nbr[size_t(fo) * ne + o] = int32_t(e);
             open.erase(it);
@@ -1571,6 +1580,10 @@ void tensor_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
 }
 
 void tensor_operator_set(hgdg_ctx* c, const hgdg_op_desc* op) {
+    if (c->tm.general) {
+        tensor_general_operator_set(c, op);
+        return;
+    }
     const int dim = c->dim;
     const int64_t ne = c->n_elem;
     if (op->kind == HGDG_OP_ACOUSTIC_CENTRAL) {
@@ -1683,7 +1696,7 @@ static void launch_advection(hgdg_ctx* c, const RhsParams& p, int tiles) {
 
 void tensor_tune(hgdg_ctx* c) {
     for (double& t : c->adv_tile_ms) t = 0;
-    if (c->op.kind != HGDG_OP_ADVECTION_UPWIND) return;
+    if (c->op.kind != HGDG_OP_ADVECTION_UPWIND || c->tm.general) return;
     if (c->adv_tiles_fixed > 0) {
         c->adv_tiles = c->adv_tiles_fixed;
         return;
@@ -1744,7 +1757,8 @@ constexpr int kRkTinyChunk = 500;  // steps per launch
 
 bool tensor_rk_tiny(hgdg_ctx* c, int solution, const int* aux, int stages, const double* a, const double* b,
                     double dt, int n_steps) {
-    if (c->rk_fused == 0 || c->simplex() || c->op.kind != HGDG_OP_ADVECTION_UPWIND || n_steps < 1) return false;
+    if (c->rk_fused == 0 || c->simplex() || c->tm.general || c->op.kind != HGDG_OP_ADVECTION_UPWIND || n_steps < 1)
+        return false;
     const bool mixed = c->op.precision == HGDG_PREC_MIXED_FP32;
     const int64_t E = c->n_elem, n = c->n_dof();
     const size_t tb = mixed ? 4 : 8;
@@ -1802,6 +1816,10 @@ bool tensor_rk_tiny(hgdg_ctx* c, int solution, const int* aux, int stages, const
 }
 
 void tensor_rhs(hgdg_ctx* c, const StageIO& io) {
+    if (c->tm.general) {
+        tensor_general_rhs(c, io);
+        return;
+    }
     const bool mixed = c->op.precision == HGDG_PREC_MIXED_FP32;
     if (c->op.kind == HGDG_OP_ACOUSTIC_CENTRAL) {
         AcousticParams a{};
@@ -1830,6 +1848,10 @@ void tensor_rhs(hgdg_ctx* c, const StageIO& io) {
 }
 
 void tensor_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, double* max_abs) {
+    if (c->tm.general) {
+        tensor_general_state_diff(c, vx, vy, l2, max_abs);
+        return;
+    }
     DiffParams p{};
     p.n_elem = c->n_elem * c->n_fields;
     p.fields = c->n_fields;
@@ -1862,6 +1884,10 @@ void tensor_state_diff(hgdg_ctx* c, const Vec& vx, const Vec& vy, double* l2, do
 }
 
 void tensor_mass_apply(hgdg_ctx* c, Vec& v, bool inverse) {
+    if (c->tm.general) {
+        tensor_general_mass_apply(c, v, inverse);
+        return;
+    }
     require(!v.is_f32, HGDG_ERR_UNSUPPORTED, "mass apply/solve needs a vector stored in FP64");
     MassParams p{};
     p.n_blocks = c->n_elem * c->n_fields;
@@ -1882,6 +1908,10 @@ void tensor_mass_apply(hgdg_ctx* c, Vec& v, bool inverse) {
 
 void tensor_error_modes(hgdg_ctx* c, const Vec& v, int field, int n_modes, const double* amp, const double* k,
                         const double* shift, double* l2, double* max_abs) {
+    if (c->tm.general) {
+        tensor_general_error_modes(c, v, field, n_modes, amp, k, shift, l2, max_abs);
+        return;
+    }
     const int n1 = c->tm.n1, dim = c->dim;
     const int64_t ne = c->n_elem;
     std::vector<double> x2, w2, x1, w1;
@@ -2033,6 +2063,10 @@ void tensor_batch_launch(hipStream_t stream, int kind, int dim, int n1, bool mix
 
 void tensor_project_device(hgdg_ctx* c, Vec& v, int field, int n_modes, const double* amp, const double* k,
                            const double* gauss) {
+    if (c->tm.general) {
+        tensor_general_project(c, v, field, n_modes, amp, k, gauss);
+        return;
+    }
     const int n1 = c->tm.n1, dim = c->dim;
     std::vector<double> xq, wq;
     gauss_legendre(n1, xq, wq);

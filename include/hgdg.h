@@ -138,6 +138,24 @@ int hgdg_set_rk_fused(hgdg_ctx* ctx, int enable);
  * has made. launches may be NULL. */
 int hgdg_get_rk_fused(hgdg_ctx* ctx, int* enable, int64_t* launches);
 
+/* Quadrilateral and hexahedral meshes take one of two kernel families:
+ * sum-factorised kernels for affine elements (parallelograms,
+ * parallelepipeds) whose neighbours share each face with the opposite local
+ * face in the same orientation (every structured mesh), and a Gauss-point
+ * family for everything else hpGEM accepts: bilinear/trilinear elements
+ * (MappingToPhysHypercubeLinear.cpp:222-304) and neighbours meeting through
+ * any local face in any of the square's orientations (FaceGeometry.cpp:148-155,
+ * ReferenceSquare.cpp:131-180), with hpGEM's pairing of face points. `force`
+ * = 1 makes meshes uploaded afterwards take the Gauss-point family even when
+ * the sum-factorised one applies (both give hpGEM's operator to rounding);
+ * 0 (the default) chooses by the mesh. No counterpart in the reference (a
+ * kernel choice of this implementation). */
+int hgdg_set_tensor_general(hgdg_ctx* ctx, int force);
+
+/* The setting above, and whether the current mesh runs on the Gauss-point
+ * family (1) or the sum-factorised one (0; also for tetrahedra). */
+int hgdg_get_tensor_general(hgdg_ctx* ctx, int* force, int* in_use);
+
 /* Threads per element of the tetrahedral RHS kernel, per order 1..4
  * (threads[p-1]): 0 (the default) lets hgdg_operator_set time both candidates
  * of that order on the mesh and keep the faster; otherwise one of the two
