@@ -374,20 +374,34 @@ def run_maxwell_assembly():
             "host_wall_s_incl_setup_and_copy": wall}
 
 
-def run_serving(precision, n_requests, seed=SEED):
+def run_serving(precision, n_requests, seed=SEED, profile=False):
     """The serving stream: n_requests independent solves (2D/3D, advection or
     acoustic, periodic boxes with side log-uniform in [8, 64], p in 1..4,
     200 RK4 steps, seeded sine data) arriving with Poisson inter-arrival
-    times of mean 5 ms; continuous batching with HBM-footprint admission."""
+    times of mean 5 ms; continuous batching with HBM-footprint admission.
+    With profile, the steps run as separate launches between HIP events and
+    the kernel time per (operator, dim, order) family is returned instead of
+    latencies (the separate launches add host time, so latencies of that run
+    are not the serving figures)."""
     from hpgem_amd.serving import Server, run_stream, serving_stream
 
     reqs = serving_stream(n_requests=n_requests, seed=seed)
     srv = Server(precision=precision, hbm_budget_gb=96.0, max_active=128)
+    if profile:
+        srv.set_profiling(True)
     t0 = time.perf_counter()
     run_stream(srv, reqs)
     wall = time.perf_counter() - t0
     st = srv.stats()
+    fams, upd = srv.family_times() if profile else ({}, 0.0)
     srv.close()
+    if profile:
+        total = sum(ms for ms, _ in fams.values()) + upd
+        split = {f"{k}{d}d_p{o}": {"kernel_ms": ms, "launches": n, "share": ms / total}
+                 for (k, d, o), (ms, n) in sorted(fams.items(), key=lambda kv: -kv[1][0])}
+        split["rk_update"] = {"kernel_ms": upd, "share": upd / total}
+        return {"requests": n_requests, "completed": st["completed"], "kernel_ms_total": total,
+                "wall_s": wall, "families": split}
     return {"requests": n_requests, "completed": st["completed"], "failed": st["failed"],
             "latency_p50_s": st["latency_p50_s"], "latency_p99_s": st["latency_p99_s"],
             "latency_mean_s": st["latency_mean_s"], "gdof_rhs_per_s": st["dof_rhs_per_s"] / 1e9,
@@ -489,6 +503,9 @@ def main():
     serving = None
     if args.serving_requests > 0:
         serving = {prec: run_serving(prec, args.serving_requests) for prec in ("fp64", "mixed")}
+        # kernel time per family, from a separate profiled pass over a shorter stream
+        for prec in ("fp64", "mixed"):
+            serving[prec]["kernel_split"] = run_serving(prec, min(128, args.serving_requests), profile=True)
     cpu = None if args.no_cpu else run_cpu_baseline(args.cpu_seconds)
 
     mixed_ok = gpu["mixed"]["rel_l2_err_vs_fp64"] <= 1e-6

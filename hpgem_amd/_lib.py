@@ -140,6 +140,9 @@ SIGNATURES += [
     ("hgdg_server_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]),
     ("hgdg_server_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), _D, _D, _D]),
     ("hgdg_server_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
+    ("hgdg_server_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hgdg_server_family_times", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _I32, _I32, _I32, _D, _I64,
+                                                 _INT, _D]),
     ("hgdg_server_destroy", ctypes.c_int, [ctypes.c_void_p]),
 ]
 

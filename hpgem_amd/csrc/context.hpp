@@ -43,6 +43,7 @@ struct SimplexMesh {
     int64_t group_dof[5] = {0, 0, 0, 0, 0};      // first DOF of each group
     std::vector<int64_t> dev_of;                 // mesh element -> device element
     std::vector<int64_t> mesh_of;                // device element -> mesh element
+    std::vector<int64_t> rank;                   // mesh element -> processing (locality) rank
     std::vector<int32_t> order;                  // per device element
     std::vector<int64_t> dof_off;                // per device element: first DOF
     std::vector<SimplexOps> ops;                 // per order (index p-1), built on demand

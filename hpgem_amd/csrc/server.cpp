@@ -14,6 +14,7 @@
 // queue. Every element of every solve runs the exact code of the
 // single-problem kernel on its own data, so a result does not depend on what
 // else was resident.
+#include <atomic>
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -145,6 +146,19 @@ struct hgdg_server {
     std::vector<Launch> launches;
     size_t ax_items = 0, ax_off_pos = 0;
     int64_t ax_blocks = 0;
+    // Per-family kernel times: with `profile` set, each tick issues the staged
+    // launches one by one between HIP events instead of replaying the step
+    // graph (the same launches, so the same results) and accumulates the
+    // events' elapsed times per (kind, dim, n1) family and for the updates.
+    std::atomic<bool> profile{false};
+    std::vector<hipEvent_t> events;  // 2 per staged launch + 2 for the update pass
+    std::map<std::array<int, 3>, std::pair<double, int64_t>> family_ms;  // guarded by mu
+    double update_ms = 0;
+    int64_t update_launches = 0;
+    void drop_events() {
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        events.clear();
+    }
 
     void prepare_loop();
     void worker_loop();
@@ -338,12 +352,48 @@ void hgdg_server::tick(std::vector<hgdg::Solve*>& solves, int n_steps) {
     using namespace hgdg;
     bool same = solves.size() == staged.size();
     for (size_t i = 0; same && i < solves.size(); ++i) same = solves[i]->req.ticket == staged[i];
-    if (!same) {
+    const bool timed = profile.load();
+    if (!same || (timed && events.size() < 2 * launches.size() + 2)) {
         std::lock_guard<std::mutex> hk(hip_mu);
-        stage(solves);
+        if (!same) stage(solves);
+        while (timed && events.size() < 2 * launches.size() + 2) {
+            hipEvent_t e;
+            HGDG_HIP(hipEventCreate(&e));
+            events.push_back(e);
+        }
     }
-    for (int step = 0; step < n_steps; ++step) HGDG_HIP(hipGraphLaunch(step_exec, stream));
-    HGDG_HIP(hipStreamSynchronize(stream));
+    if (!timed) {
+        for (int step = 0; step < n_steps; ++step) HGDG_HIP(hipGraphLaunch(step_exec, stream));
+        HGDG_HIP(hipStreamSynchronize(stream));
+        return;
+    }
+    for (int step = 0; step < n_steps; ++step) {
+        for (size_t i = 0; i < launches.size(); ++i) {
+            const Launch& L = launches[i];
+            HGDG_HIP(hipEventRecord(events[2 * i], stream));
+            tensor_batch_launch(stream, L.kind, L.dim, L.n1, mixed, d_blob.ptr + L.params,
+                                reinterpret_cast<const int64_t*>(d_blob.ptr + L.off), L.n, L.blocks);
+            HGDG_HIP(hipEventRecord(events[2 * i + 1], stream));
+        }
+        const size_t u = 2 * launches.size();
+        HGDG_HIP(hipEventRecord(events[u], stream));
+        axpy_batch_launch(stream, reinterpret_cast<const AxpyItem*>(d_blob.ptr + ax_items),
+                          reinterpret_cast<const int64_t*>(d_blob.ptr + ax_off_pos), int(solves.size()), ax_blocks);
+        HGDG_HIP(hipEventRecord(events[u + 1], stream));
+        HGDG_HIP(hipStreamSynchronize(stream));
+        std::lock_guard<std::mutex> lk(mu);
+        for (size_t i = 0; i < launches.size(); ++i) {
+            float ms = 0;
+            HGDG_HIP(hipEventElapsedTime(&ms, events[2 * i], events[2 * i + 1]));
+            auto& f = family_ms[{launches[i].kind, launches[i].dim, launches[i].n1}];
+            f.first += ms;
+            ++f.second;
+        }
+        float ms = 0;
+        HGDG_HIP(hipEventElapsedTime(&ms, events[u], events[u + 1]));
+        update_ms += ms;
+        ++update_launches;
+    }
 }
 
 void hgdg_server::worker_loop() {
@@ -587,6 +637,34 @@ int hgdg_server_get_stats(hgdg_server* s, hgdg_server_stats* out) {
     });
 }
 
+int hgdg_server_set_profiling(hgdg_server* s, int enable) {
+    return hgdg::run([&] {
+        require(s != nullptr, HGDG_ERR_ARG, "NULL argument");
+        require(enable == 0 || enable == 1, HGDG_ERR_ARG, "enable must be 0 or 1");
+        s->profile.store(enable == 1);
+    });
+}
+
+int hgdg_server_family_times(hgdg_server* s, int capacity, int32_t* kind, int32_t* dim, int32_t* order,
+                             double* kernel_ms, int64_t* launches, int* n_families, double* update_ms) {
+    return hgdg::run([&] {
+        require(s != nullptr && n_families != nullptr, HGDG_ERR_ARG, "NULL argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        *n_families = int(s->family_ms.size());
+        if (update_ms) *update_ms = s->update_ms;
+        int i = 0;
+        for (const auto& kv : s->family_ms) {
+            if (i >= capacity) break;
+            if (kind) kind[i] = kv.first[0];
+            if (dim) dim[i] = kv.first[1];
+            if (order) order[i] = kv.first[2] - 1;
+            if (kernel_ms) kernel_ms[i] = kv.second.first;
+            if (launches) launches[i] = kv.second.second;
+            ++i;
+        }
+    });
+}
+
 int hgdg_server_destroy(hgdg_server* s) {
     return hgdg::run([&] {
         if (!s) return;
@@ -603,6 +681,7 @@ int hgdg_server_destroy(hgdg_server* s) {
         }
         (void)hipSetDevice(s->device);
         s->drop_step_graph();
+        s->drop_events();
         if (s->h_blob) (void)hipHostFree(s->h_blob);
         s->d_blob.reset();
         if (s->stream) (void)hipStreamDestroy(s->stream);

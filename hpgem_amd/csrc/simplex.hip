@@ -134,6 +134,12 @@ struct SimplexShape {
     static_assert(FI % 2 == 0 && U % 2 == 0 && W % 2 == 0 && SN % 2 == 0 && SF % 2 == 0, "double2 alignment");
 };
 
+
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ mfma_d4 mfma_step(double a, double b, mfma_d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
 template <typename T>
 struct FluxPair {
     T v, s;  // flux coefficient of the v equation, of the s equations
@@ -235,51 +241,119 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
     __syncthreads();
 
     // ---- phase 2: volume, k_v = sum_r Dr_r g_r ; k_s_d = (1/c_inv) sum_r (J^{-1})_{r,d} Dr_r v
+    // FP64 at orders 3 and 4 on the matrix cores: per direction r, Dr_r times
+    // the block's columns [g_r of elements 0..7 | v of elements 0..7] in
+    // 16 x 16 x 4 tiles (v_mfma_f64_16x16x4f64: A(i, k) from lane i + 16 k,
+    // B(k, j) from lane j + 16 k, D(4 q + lane / 16, lane % 16) in output q),
+    // one row tile per wavefront. Each output accumulates its products in k
+    // order, one fused multiply-add after another, as the row loop below does:
+    // the results are bitwise those of the VALU form (checked on the 24^3 x 5
+    // workload). In FP32 the matrix form measured no faster than the VALU
+    // form, which the mixed path keeps.
     Acc<T> kv, ks[3];
-    T a[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
-    if constexpr (S::DR_LDS) {
-        if (row_ok) {
-#pragma unroll 4
-            for (int j = 0; j < NB; ++j) {
-                const T v = sg[j];
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    const T d = sDr[(r * NB + j) * NB + i];
-                    a[r] += d * sg[(1 + r) * NB + j];
-                    b[r] += d * v;
-                }
-            }
-        }
-    } else {
-        // one direction at a time through LDS (every thread of the block
-        // takes part in the barriers and loads); the sums over j run in the
-        // same order as above
+    if constexpr (P >= 3 && std::is_same<T, double>::value) {
+        static_assert(S::EPB <= 8, "16 columns");
+        constexpr int NTM = (NB + 15) / 16, KS = (NB + 3) / 4;
+        using V4 = mfma_d4;
+        const int lane = tid % 64, wave = tid / 64;
+        const int col = lane % 16, kk = lane / 16;
+        const int ce = col % 8, cv = col / 8;
+        T* sG0 = reinterpret_cast<T*>(lds + S::G);
+        V4 acc[3];
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-            if (r > 0) {
-                __syncthreads();  // the block is done with direction r - 1
-                for (int k = tid; k < NB * NB; k += kBlock) sDr[k] = gDr[r * NB * NB + k];
-                __syncthreads();
+            if constexpr (!S::DR_LDS) {
+                if (r > 0) {
+                    __syncthreads();
+                    for (int k = tid; k < NB * NB; k += kBlock) sDr[k] = gDr[r * NB * NB + k];
+                    __syncthreads();
+                }
             }
-            if (row_ok) {
-#pragma unroll 5
-                for (int j = 0; j < NB; ++j) {
-                    const T d = sDr[j * NB + i];
-                    a[r] += d * sg[(1 + r) * NB + j];
-                    b[r] += d * sg[j];
+            acc[r] = V4{T(0), T(0), T(0), T(0)};
+            if (wave < NTM) {
+                const T* Dr = sDr + (S::DR_LDS ? r * NB * NB : 0);
+#pragma unroll
+                for (int s = 0; s < KS; ++s) {
+                    const int k = 4 * s + kk, i = 16 * wave + col;
+                    const T av = (i < NB && k < NB) ? Dr[k * NB + i] : T(0);
+                    const T bv = (ce < count && k < NB) ? sG0[ce * 4 * NB + (cv ? 0 : 1 + r) * NB + k] : T(0);
+                    acc[r] = mfma_step(av, bv, acc[r]);
                 }
             }
         }
-    }
-    if (row_ok) {
-        T Jt[9];  // J^{-1} again (not kept in registers across the barrier)
+        __syncthreads();  // every wave has read the inputs
+        if (wave < NTM) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) Jt[q] = T(p.jinv[q * E + e]);
-        kv = Acc<T>(a[0] + a[1] + a[2]);
-        const T ci = T(1.0 / cinv);
+            for (int ri = 0; ri < 4; ++ri) {
+                const int row = 16 * wave + 4 * ri + lane / 16;
+                if (row < NB && ce < count) {
+                    T* o = sG0 + ce * 4 * NB;
+                    if (cv == 0) o[row] = acc[0][ri] + acc[1][ri] + acc[2][ri];
+                    else {
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-            ks[d] = Acc<T>((Jt[d] * b[0] + Jt[3 + d] * b[1] + Jt[6 + d] * b[2]) * ci);
+                        for (int r = 0; r < 3; ++r) o[(1 + r) * NB + row] = acc[r][ri];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (row_ok) {
+            T Jt[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) Jt[q] = T(p.jinv[q * E + e]);
+            kv = Acc<T>(sg[i]);
+            const T ci = T(1.0 / cinv);
+            const T bb[3] = {sg[NB + i], sg[2 * NB + i], sg[3 * NB + i]};
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                ks[d] = Acc<T>((Jt[d] * bb[0] + Jt[3 + d] * bb[1] + Jt[6 + d] * bb[2]) * ci);
+        }
+    } else {
+        T a[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
+        if constexpr (S::DR_LDS) {
+            if (row_ok) {
+#pragma unroll 4
+                for (int j = 0; j < NB; ++j) {
+                    const T v = sg[j];
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        const T d = sDr[(r * NB + j) * NB + i];
+                        a[r] += d * sg[(1 + r) * NB + j];
+                        b[r] += d * v;
+                    }
+                }
+            }
+        } else {
+            // one direction at a time through LDS (every thread of the block
+            // takes part in the barriers and loads); the sums over j run in the
+            // same order as above
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                if (r > 0) {
+                    __syncthreads();  // the block is done with direction r - 1
+                    for (int k = tid; k < NB * NB; k += kBlock) sDr[k] = gDr[r * NB * NB + k];
+                    __syncthreads();
+                }
+                if (row_ok) {
+#pragma unroll 5
+                    for (int j = 0; j < NB; ++j) {
+                        const T d = sDr[j * NB + i];
+                        a[r] += d * sg[(1 + r) * NB + j];
+                        b[r] += d * sg[j];
+                    }
+                }
+            }
+        }
+        if (row_ok) {
+            T Jt[9];  // J^{-1} again (not kept in registers across the barrier)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) Jt[q] = T(p.jinv[q * E + e]);
+            kv = Acc<T>(a[0] + a[1] + a[2]);
+            const T ci = T(1.0 / cinv);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                ks[d] = Acc<T>((Jt[d] * b[0] + Jt[3 + d] * b[1] + Jt[6 + d] * b[2]) * ci);
+        }
     }
     __syncthreads();  // the volume data give way to the face data
 
@@ -716,6 +790,53 @@ void group_blocks(const SimplexMesh& sm, int32_t* gb) {
 
 }  // namespace
 
+namespace {
+
+// Processing order of the tetrahedra (the order of the RHS kernel's block
+// schedule, and of the device layout inside each order group): the elements'
+// centroids are cut into 4 x 4 columns across the two shorter extents of the
+// mesh's bounding box and run through column by column, along the longest
+// extent inside a column. A face neighbour is then either in the same column,
+// at most one column layer (a few hundred elements) away in processing order,
+// so its face columns are still in the XCD's L2 when they are gathered, or
+// across a column side; in mesh order (x fastest) every neighbour along the
+// slowest axis was a whole mesh layer back, beyond the L2.
+std::vector<int64_t> locality_rank(const hgdg_mesh_desc* m) {
+    const int64_t ne = m->n_elements;
+    std::vector<std::array<double, 3>> ctr(static_cast<size_t>(ne));
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t e = 0; e < ne; ++e)
+        for (int r = 0; r < 3; ++r) {
+            double s = 0;
+            for (int v = 0; v < 4; ++v) s += m->coords[(size_t(e) * 4 + v) * 3 + r];
+            ctr[size_t(e)][r] = s / 4;
+            lo[r] = std::min(lo[r], s / 4);
+            hi[r] = std::max(hi[r], s / 4);
+        }
+    int ax[3] = {0, 1, 2};  // ax[2]: the longest extent (the last of equal ones)
+    std::stable_sort(ax, ax + 3, [&](int a, int b) { return hi[a] - lo[a] < hi[b] - lo[b]; });
+    constexpr int kCols = 4;
+    auto cell = [&](int64_t e, int r) {
+        const double ext = hi[r] - lo[r];
+        if (!(ext > 0)) return 0;
+        return std::min(kCols - 1, int((ctr[size_t(e)][r] - lo[r]) / ext * kCols));
+    };
+    std::vector<int64_t> idx(static_cast<size_t>(ne));
+    for (int64_t e = 0; e < ne; ++e) idx[size_t(e)] = e;
+    std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
+        const int ca = cell(a, ax[1]) * kCols + cell(a, ax[0]), cb = cell(b, ax[1]) * kCols + cell(b, ax[0]);
+        if (ca != cb) return ca < cb;
+        for (int k = 2; k >= 0; --k)
+            if (ctr[size_t(a)][ax[k]] != ctr[size_t(b)][ax[k]]) return ctr[size_t(a)][ax[k]] < ctr[size_t(b)][ax[k]];
+        return a < b;
+    });
+    std::vector<int64_t> rank(static_cast<size_t>(ne));
+    for (int64_t k = 0; k < ne; ++k) rank[size_t(idx[size_t(k)])] = k;
+    return rank;
+}
+
+}  // namespace
+
 void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     SimplexMesh& sm = c->sm;
     const int64_t ne = m->n_elements;
@@ -725,7 +846,7 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
         require(order[e] >= 1 && order[e] <= kMaxSimplexOrder, HGDG_ERR_UNSUPPORTED,
                 "tetrahedral orders must be in 1..4");
     }
-    // group by order, mesh order inside a group
+    // group by order; inside a group, elements follow the locality rank
     int64_t counts[4] = {0, 0, 0, 0};
     for (int64_t e = 0; e < ne; ++e) ++counts[order[e] - 1];
     sm.group_start[0] = 0;
@@ -733,8 +854,11 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     sm.dev_of.assign(ne, 0);
     sm.mesh_of.assign(ne, 0);
     sm.order.assign(ne, 0);
+    sm.rank = locality_rank(m);
+    std::vector<int64_t> by_rank(ne);
+    for (int64_t e = 0; e < ne; ++e) by_rank[size_t(sm.rank[e])] = e;
     int64_t fill[4] = {sm.group_start[0], sm.group_start[1], sm.group_start[2], sm.group_start[3]};
-    for (int64_t e = 0; e < ne; ++e) {
+    for (int64_t e : by_rank) {
         const int64_t d = fill[order[e] - 1]++;
         sm.dev_of[e] = d;
         sm.mesh_of[d] = e;
@@ -980,7 +1104,7 @@ int simplex_threads(int order, int variant) { return simplex_tpe_of(order, varia
 void simplex_schedule(hgdg_ctx* c) {
     SimplexMesh& sm = c->sm;
     const int64_t ne = c->n_elem;
-    const int64_t slab = std::max<int64_t>(1024, (ne + 255) / 256);
+    const int64_t slab = std::max<int64_t>(1024, (ne + 255) / 256);  // in locality ranks
     int epb[4];
     for (int g = 0; g < 4; ++g) epb[g] = kBlock / simplex_tpe_of(g + 1, sm.tpe_variant[g]);
     std::vector<int32_t> blocks;
@@ -988,9 +1112,9 @@ void simplex_schedule(hgdg_ctx* c) {
     for (int64_t lo = 0; lo < ne; lo += slab) {
         const int64_t hi = std::min(ne, lo + slab);
         for (int g = 0; g < 4; ++g) {
-            // device elements of order g+1 with mesh index below hi (mesh order inside a group)
+            // device elements of order g+1 with locality rank below hi (rank order inside a group)
             int64_t end = cursor[g];
-            while (end < sm.group_start[g + 1] && sm.mesh_of[end] < hi) ++end;
+            while (end < sm.group_start[g + 1] && sm.rank[size_t(sm.mesh_of[end])] < hi) ++end;
             for (int64_t b = cursor[g]; b < end; b += epb[g]) {
                 blocks.push_back(g + 1);
                 blocks.push_back(int32_t(b));

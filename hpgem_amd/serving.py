@@ -55,6 +55,26 @@ class Server:
         check(self.lib.hgdg_server_get_stats(self.h, ctypes.byref(st)))
         return {f: getattr(st, f) for f, _ in st._fields_}
 
+    def set_profiling(self, enable=True):
+        """Time each kernel family with HIP events (separate launches instead
+        of the step graph; results unchanged)."""
+        check(self.lib.hgdg_server_set_profiling(self.h, 1 if enable else 0))
+
+    def family_times(self):
+        """{(kind, dim, order): (kernel ms, launches)} and the update passes' ms."""
+        n = ctypes.c_int()
+        check(self.lib.hgdg_server_family_times(self.h, 0, None, None, None, None, None, ctypes.byref(n), None))
+        cap = n.value
+        kind, dim, order = (np.zeros(cap, dtype=np.int32) for _ in range(3))
+        ms, cnt = np.zeros(cap), np.zeros(cap, dtype=np.int64)
+        upd = ctypes.c_double()
+        i32 = lambda a: a.ctypes.data_as(_lib._I32)
+        check(self.lib.hgdg_server_family_times(self.h, cap, i32(kind), i32(dim), i32(order), dptr(ms),
+                                                cnt.ctypes.data_as(_lib._I64), ctypes.byref(n), ctypes.byref(upd)))
+        names = {v: k for k, v in KINDS.items()}
+        fams = {(names[int(kind[i])], int(dim[i]), int(order[i])): (float(ms[i]), int(cnt[i])) for i in range(cap)}
+        return fams, upd.value
+
     def close(self):
         if getattr(self, "h", None):
             check(self.lib.hgdg_server_destroy(self.h))

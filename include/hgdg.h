@@ -343,6 +343,18 @@ int hgdg_server_wait(hgdg_server* s, int64_t ticket, double timeout_s, int* done
 int hgdg_server_result(hgdg_server* s, int64_t ticket, int64_t* n_dof, double* state, double* norm,
                        double* latency_s);
 int hgdg_server_get_stats(hgdg_server* s, hgdg_server_stats* out);
+/* Kernel time per family: with profiling on, each RK step is issued as its
+ * separate launches between HIP events instead of one graph launch (the same
+ * launches: results do not change; host overhead grows), and the events'
+ * elapsed times are summed per (operator kind, dim, order) family and for the
+ * RK update pass. No counterpart in the reference (a measurement of this
+ * implementation's batched launches). */
+int hgdg_server_set_profiling(hgdg_server* s, int enable);
+/* Up to `capacity` families: kind, dim, order, summed kernel ms and launch
+ * count each (any array may be NULL); n_families = how many there are;
+ * update_ms = summed time of the update passes (may be NULL). */
+int hgdg_server_family_times(hgdg_server* s, int capacity, int32_t* kind, int32_t* dim, int32_t* order,
+                             double* kernel_ms, int64_t* launches, int* n_families, double* update_ms);
 int hgdg_server_destroy(hgdg_server* s);
 
 /* ---- host-only utilities (no device needed) ------------------------------ */

@@ -104,3 +104,32 @@ def test_bad_request_is_rejected_and_server_keeps_serving():
     assert srv.wait(t, 60)
     assert srv.result(t)["n_dof"] == 36 * 9
     srv.close()
+
+
+def test_profiled_serving_times_each_family_and_changes_no_result():
+    """With profiling on, every RK step runs as separate launches between HIP
+    events: the same launches as the step graph, so every result is bitwise
+    the same, and each (kind, dim, order) family of the resident mix reports
+    its kernel time (4 stages per step)."""
+    specs = [(2, "advection", 2, [8, 6], 6), (3, "advection", 3, [5, 4, 6], 4), (3, "acoustic", 2, [4, 5, 4], 5)]
+    reqs = [one_solve(d, kd, p, c, seed=i, steps=st) for i, (d, kd, p, c, st) in enumerate(specs)]
+    states = []
+    for profile in (False, True):
+        srv = Server(precision="fp64", hbm_budget_gb=4, max_active=8)
+        if profile:
+            srv.set_profiling(True)
+        tickets = [submit(srv, r) for r in reqs]
+        assert srv.wait(-1, 120)
+        states.append([srv.result(t, with_state=True)["state"] for t in tickets])
+        if profile:
+            fams, upd = srv.family_times()
+            print(fams, upd)
+            assert set(fams) == {("advection", 2, 2), ("advection", 3, 3), ("acoustic", 3, 2)}
+            assert all(ms > 0 and n % 4 == 0 for ms, n in fams.values()) and upd > 0
+            # every step of every resident family is timed: at least the longest solve's steps
+            assert max(n for _, n in fams.values()) >= 4 * max(r["n_steps"] for r in reqs)
+        else:
+            assert srv.family_times()[0] == {}
+        srv.close()
+    for a, b in zip(*states):
+        np.testing.assert_array_equal(a, b)
