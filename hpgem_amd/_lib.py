@@ -10,7 +10,7 @@ LIB_PATH = os.path.join(HERE, "libhgdg.so")
 
 SHAPE_SQUARE, SHAPE_CUBE, SHAPE_TETRAHEDRON = 2, 4, 5
 SHAPE_LINE, SHAPE_TRIANGLE, SHAPE_PRISM, SHAPE_PYRAMID = 1, 3, 6, 7
-OP_ADVECTION_UPWIND, OP_ACOUSTIC_CENTRAL = 1, 2
+OP_ADVECTION_UPWIND, OP_ACOUSTIC_CENTRAL, OP_MATRIX = 1, 2, 3
 PREC_FP64, PREC_MIXED_FP32 = 0, 1
 
 ERRORS = {1: "invalid argument", 2: "HIP error", 3: "unsupported", 4: "call out of order", 5: "memory"}
@@ -31,6 +31,12 @@ class OpDesc(ctypes.Structure):
                 ("c_inv", _D)]
 
 
+class MatrixDesc(ctypes.Structure):
+    _fields_ = [("n_elements", ctypes.c_int64), ("n_fields", ctypes.c_int32), ("block_size", _I32),
+                ("mass", _D), ("stiffness", _D), ("n_faces", ctypes.c_int64), ("face_elements", _I64),
+                ("face_blocks", _D)]
+
+
 # (name, restype, argtypes) of every entry point declared in include/hgdg.h
 SIGNATURES = [
     ("hgdg_last_error", ctypes.c_char_p, []),
@@ -43,6 +49,7 @@ SIGNATURES = [
     ("hgdg_get_advection_tiles", ctypes.c_int, [ctypes.c_void_p, _INT, _D]),
     ("hgdg_set_rk_fused", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hgdg_get_rk_fused", ctypes.c_int, [ctypes.c_void_p, _INT, _I64]),
+    ("hgdg_operator_set_matrices", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MatrixDesc), ctypes.c_int]),
     ("hgdg_set_tensor_general", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hgdg_get_tensor_general", ctypes.c_int, [ctypes.c_void_p, _INT, _INT]),
     ("hgdg_set_simplex_threads", ctypes.c_int, [ctypes.c_void_p, _I32]),

@@ -393,3 +393,41 @@ class HpgemAPIDevice:
         l2, mx = ctypes.c_double(), ctypes.c_double()
         check(self.lib.hgdg_state_diff(self.h, x, y, ctypes.byref(l2), ctypes.byref(mx)))
         return l2.value, mx.value
+
+
+class HpgemAPIMatrixDevice(HpgemAPIDevice):
+    """HpgemAPILinear's precomputed path on the device: the element mass and
+    stiffness matrices and the face matrices an application builds from its
+    own integrands (createMassMatrices / createStiffnessMatrices) are handed
+    over once; computeTimeDerivative, computeRightHandSide,
+    solveMassMatrixEquations, computeOneTimeStep and the adaptive step then
+    run on the device. Vectors are in hpGEM's layout. mass, stiffness: lists
+    of n_e x n_e arrays; face_elements: [F][2] (left, right or -1); face_blocks:
+    per face the column-major LL, LR, RL, RR blocks concatenated (LL only on
+    boundary faces)."""
+
+    def __init__(self, mass, stiffness, face_elements, face_blocks, n_fields=1, precision="fp64", tableau=RK4,
+                 n_extra_vectors=0, device=0):
+        self.lib = _lib.load()
+        self.tableau = tableau
+        self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        h = ctypes.c_void_p()
+        check(self.lib.hgdg_ctx_create(ctypes.byref(h), device))
+        self.h = h
+        sizes = np.ascontiguousarray([np.asarray(m).shape[0] for m in mass], dtype=np.int32)
+        self._mass = np.ascontiguousarray(np.concatenate([np.asarray(m, dtype=np.float64).ravel(order="F")
+                                                          for m in mass]))
+        self._stiff = np.ascontiguousarray(np.concatenate([np.asarray(a, dtype=np.float64).ravel(order="F")
+                                                           for a in stiffness]))
+        self._fe = np.ascontiguousarray(face_elements, dtype=np.int64).reshape(-1, 2)
+        fb = [np.asarray(b, dtype=np.float64).ravel() for b in face_blocks]
+        self._fb = np.ascontiguousarray(np.concatenate(fb) if fb else np.zeros(1))
+        self._sizes = sizes
+        desc = _lib.MatrixDesc(len(sizes), n_fields, sizes.ctypes.data_as(_lib._I32), dptr(self._mass),
+                               dptr(self._stiff), len(self._fe), self._fe.ctypes.data_as(_lib._I64), dptr(self._fb))
+        check(self.lib.hgdg_operator_set_matrices(self.h, ctypes.byref(desc), self.precision))
+        self._query_info()
+        self.solutionVectorId_ = 0
+        self.auxiliaryVectorIds_ = list(range(1, tableau.stages + 1))
+        self.n_vectors = 1 + tableau.stages + n_extra_vectors
+        check(self.lib.hgdg_state_alloc(self.h, self.n_vectors))

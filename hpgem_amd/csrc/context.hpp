@@ -37,6 +37,18 @@ struct TensorMesh {
     DevBuf<int32_t> d_gface;         // gnbr then ginfo
 };
 
+// A linear operator given by its matrices (matrix_op.hip): vectors in hpGEM's
+// layout, per element row a diagonal block and one block per face neighbour.
+struct MatrixOp {
+    bool on = false;
+    int max_rows = 0;
+    std::vector<int64_t> elem_off, sq_off, row_ptr, blk_off;
+    std::vector<int32_t> blk_col;
+    DevBuf<int64_t> d_elem_off, d_sq_off, d_row_ptr, d_blk_off;
+    DevBuf<int32_t> d_blk_col;
+    DevBuf<double> d_pool, d_minv, d_mass;
+};
+
 // Tetrahedra with per-element order 1..4, grouped by order on the device.
 struct SimplexMesh {
     int64_t group_start[5] = {0, 0, 0, 0, 0};    // device elements of order g+1: [start[g], start[g+1])
@@ -86,6 +98,7 @@ struct hgdg_ctx {
     std::vector<double> cinv;  // mesh order
     hgdg::TensorMesh tm;
     hgdg::SimplexMesh sm;
+    hgdg::MatrixOp mat;
     std::vector<hgdg::Vec> vecs;
     hgdg::DevBuf<double> d_partial;
     bool graph_valid = false;
@@ -117,7 +130,8 @@ struct hgdg_ctx {
     int64_t rk_fused_launches = 0;
     double adv_tile_ms[4] = {0, 0, 0, 0};  // timed candidates 1, 2, 4, 8 (0 = not timed)
 
-    bool simplex() const { return shape == HGDG_SHAPE_TETRAHEDRON; }
+    bool simplex() const { return shape == HGDG_SHAPE_TETRAHEDRON && !mat.on; }
+    bool matrix() const { return mat.on; }
     int64_t n_dof() const { return dof; }
     void drop_graph2() {
         if (graph2_exec) (void)hipGraphExecDestroy(graph2_exec);
@@ -199,6 +213,12 @@ struct AxpyItem {
     int32_t n_x;
 };
 void axpy_batch_launch(hipStream_t stream, const AxpyItem* items_dev, const int64_t* off, int n, int64_t blocks);
+
+// matrix_op.hip
+void matrix_operator_set(hgdg_ctx* c, const hgdg_matrix_desc* d, int precision);
+void matrix_rhs(hgdg_ctx* c, const StageIO& io, bool solve);
+void matrix_mass_apply(hgdg_ctx* c, Vec& v, bool inverse);
+void matrix_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* l2, double* max_abs);
 
 // maxwell.hip
 void maxwell_assemble(hgdg_ctx* c, int order, const double* eps, double stab, int64_t* n_blocks, int32_t* block_size,

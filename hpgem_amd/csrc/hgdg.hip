@@ -157,13 +157,14 @@ void mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     c->n_elem = m->n_elements;
     c->tm = TensorMesh();
     c->sm = SimplexMesh();
+    c->mat = MatrixOp();
     if (c->simplex()) simplex_mesh_upload(c, m);
     else tensor_mesh_upload(c, m);
     c->has_mesh = true;
 }
 
 void operator_set(hgdg_ctx* c, const hgdg_op_desc* op) {
-    require(c->has_mesh, HGDG_ERR_STATE, "upload a mesh before setting the operator");
+    require(c->has_mesh && !c->matrix(), HGDG_ERR_STATE, "upload a mesh before setting the operator");
     require(op != nullptr, HGDG_ERR_ARG, "operator descriptor is NULL");
     require(op->kind == HGDG_OP_ADVECTION_UPWIND || op->kind == HGDG_OP_ACOUSTIC_CENTRAL, HGDG_ERR_ARG,
             "unknown operator kind");
@@ -207,7 +208,8 @@ StageIO make_io(hgdg_ctx* c, const int* in_ids, const double* coeffs, int n_in, 
 }
 
 void dispatch_rhs(hgdg_ctx* c, const StageIO& io) {
-    if (c->simplex()) simplex_rhs(c, io);
+    if (c->matrix()) matrix_rhs(c, io, true);
+    else if (c->simplex()) simplex_rhs(c, io);
     else tensor_rhs(c, io);
 }
 
@@ -217,7 +219,7 @@ void launch_rhs(hgdg_ctx* c, const int* in_ids, const double* coeffs, int n_in, 
 
 // Whether the operator's RHS kernel performs a fused RK update (StageIO::upd).
 bool rhs_fuses_update(const hgdg_ctx* c) {
-    return !c->simplex() && !c->tm.general && c->op.kind == HGDG_OP_ADVECTION_UPWIND;
+    return !c->matrix() && !c->simplex() && !c->tm.general && c->op.kind == HGDG_OP_ADVECTION_UPWIND;
 }
 
 void launch_axpy(hgdg_ctx* c, int to, const std::vector<int>& add, const std::vector<double>& scale) {
@@ -429,7 +431,7 @@ void step_error_and_norm(hgdg_ctx* c, int solution, const int* aux, int stages, 
                          double* error, double* norm) {
     StepNormParams p{};
     p.n_elem = c->n_elem;
-    p.off = c->simplex() ? c->sm.d_elem_off.ptr : nullptr;
+    p.off = c->matrix() ? c->mat.d_elem_off.ptr : c->simplex() ? c->sm.d_elem_off.ptr : nullptr;
     p.block = c->dof / c->n_elem;
     p.u = vec_at(c, solution).f64->ptr;
     for (int j = 0; j < stages; ++j) {
@@ -539,7 +541,7 @@ int step_error_noalloc(hgdg_ctx* c, int solution, const int* aux, int stages, co
     double* ptr = g_scratch[c->device];
     StepNormParams p{};
     p.n_elem = c->n_elem;
-    p.off = c->simplex() ? c->sm.d_elem_off.ptr : nullptr;
+    p.off = c->matrix() ? c->mat.d_elem_off.ptr : c->simplex() ? c->sm.d_elem_off.ptr : nullptr;
     p.block = c->n_elem > 0 ? c->dof / c->n_elem : 0;
     p.u = c->vecs[solution].f64->ptr;
     for (int j = 0; j < stages; ++j) {
@@ -604,7 +606,8 @@ void state_diff(hgdg_ctx* c, int x, int y, double* l2, double* max_abs) {
     require(c->has_mesh, HGDG_ERR_STATE, "no mesh");
     Vec& vx = vec_at(c, x);
     Vec& vy = vec_at(c, y);
-    if (c->simplex()) simplex_state_diff(c, vx, vy, l2, max_abs);
+    if (c->matrix()) matrix_state_diff(c, vx, vy, l2, max_abs);
+    else if (c->simplex()) simplex_state_diff(c, vx, vy, l2, max_abs);
     else tensor_state_diff(c, vx, vy, l2, max_abs);
 }
 
@@ -738,6 +741,14 @@ int hgdg_get_rk_fused(hgdg_ctx* ctx, int* enable, int64_t* launches) {
     });
 }
 
+int hgdg_operator_set_matrices(hgdg_ctx* ctx, const hgdg_matrix_desc* desc, int precision) {
+    return run([&] {
+        require(ctx != nullptr && desc != nullptr, HGDG_ERR_ARG, "NULL argument");
+        HGDG_HIP(hipSetDevice(ctx->device));
+        hgdg::matrix_operator_set(ctx, desc, precision);
+    });
+}
+
 int hgdg_set_tensor_general(hgdg_ctx* ctx, int force) {
     return run([&] {
         require(ctx != nullptr, HGDG_ERR_ARG, "ctx is NULL");
@@ -813,7 +824,8 @@ int hgdg_state_set(hgdg_ctx* ctx, int vec, const double* host) {
         HGDG_HIP(hipSetDevice(ctx->device));
         hgdg::Vec& v = hgdg::vec_at(ctx, vec);
         std::vector<double> dev;
-        if (ctx->simplex()) hgdg::simplex_to_device(ctx, host, dev);
+        if (ctx->matrix()) dev.assign(host, host + ctx->n_dof());
+        else if (ctx->simplex()) hgdg::simplex_to_device(ctx, host, dev);
         else hgdg::tensor_to_device(ctx, host, dev);
         hgdg::upload_vector(ctx, v, dev);
     });
@@ -826,7 +838,8 @@ int hgdg_state_get(hgdg_ctx* ctx, int vec, double* host) {
         hgdg::Vec& v = hgdg::vec_at(ctx, vec);
         std::vector<double> dev;
         hgdg::download_vector(ctx, v, dev);
-        if (ctx->simplex()) hgdg::simplex_from_device(ctx, dev, host);
+        if (ctx->matrix()) std::copy(dev.begin(), dev.end(), host);
+        else if (ctx->simplex()) hgdg::simplex_from_device(ctx, dev, host);
         else hgdg::tensor_from_device(ctx, dev, host);
     });
 }
@@ -860,6 +873,10 @@ int hgdg_right_hand_side(hgdg_ctx* ctx, const int* in_ids, const double* coeffs,
         HGDG_HIP(hipSetDevice(ctx->device));
         hgdg::Vec& out = hgdg::vec_at(ctx, out_id);
         require(!out.is_f32, HGDG_ERR_UNSUPPORTED, "the right-hand side is written to a vector stored in FP64");
+        if (ctx->matrix()) {  // the stiffness product alone, as multiplyStiffnessMatrices
+            hgdg::matrix_rhs(ctx, hgdg::make_io(ctx, in_ids, coeffs, n_in, out_id), false);
+            return;
+        }
         // r = M k: the fused kernel's k, multiplied back by the block-diagonal mass
         hgdg::launch_rhs(ctx, in_ids, coeffs, n_in, out_id);
         if (ctx->simplex()) hgdg::simplex_mass_apply(ctx, out, false);
@@ -873,7 +890,8 @@ int hgdg_solve_mass(hgdg_ctx* ctx, int vec) {
         require(ctx->has_op, HGDG_ERR_STATE, "set an operator before solving with its mass matrix");
         HGDG_HIP(hipSetDevice(ctx->device));
         hgdg::Vec& v = hgdg::vec_at(ctx, vec);
-        if (ctx->simplex()) hgdg::simplex_mass_apply(ctx, v, true);
+        if (ctx->matrix()) hgdg::matrix_mass_apply(ctx, v, true);
+        else if (ctx->simplex()) hgdg::simplex_mass_apply(ctx, v, true);
         else hgdg::tensor_mass_apply(ctx, v, true);
     });
 }
@@ -941,6 +959,8 @@ int hgdg_project_field_modes(hgdg_ctx* ctx, int vec, int field, int n_modes, con
         require(n_modes >= 0, HGDG_ERR_ARG, "negative mode count");
         HGDG_HIP(hipSetDevice(ctx->device));
         require(ctx->has_mesh, HGDG_ERR_STATE, "no mesh");
+        require(!ctx->matrix(), HGDG_ERR_UNSUPPORTED,
+                "a matrix operator has no geometry: project and measure errors with the caller's integrators");
         require(field >= -1 && field < ctx->n_fields, HGDG_ERR_ARG, "field out of range");
         hgdg::Vec& v = hgdg::vec_at(ctx, vec);
         if (ctx->simplex()) hgdg::simplex_project(ctx, v, field, n_modes, amp, k);
@@ -955,6 +975,8 @@ int hgdg_error_modes(hgdg_ctx* ctx, int vec, int field, int n_modes, const doubl
         require(n_modes >= 0, HGDG_ERR_ARG, "negative mode count");
         HGDG_HIP(hipSetDevice(ctx->device));
         require(ctx->has_mesh, HGDG_ERR_STATE, "no mesh");
+        require(!ctx->matrix(), HGDG_ERR_UNSUPPORTED,
+                "a matrix operator has no geometry: project and measure errors with the caller's integrators");
         require(field >= -1 && field < ctx->n_fields, HGDG_ERR_ARG, "field out of range");
         const double zero[3] = {0, 0, 0};
         const hgdg::Vec& v = hgdg::vec_at(ctx, vec);
@@ -970,6 +992,8 @@ int hgdg_project_gaussian(hgdg_ctx* ctx, int vec, int field, const double* cente
                 "the pulse needs a positive width");
         HGDG_HIP(hipSetDevice(ctx->device));
         require(ctx->has_mesh, HGDG_ERR_STATE, "no mesh");
+        require(!ctx->matrix(), HGDG_ERR_UNSUPPORTED,
+                "a matrix operator has no geometry: project and measure errors with the caller's integrators");
         require(field >= -1 && field < ctx->n_fields, HGDG_ERR_ARG, "field out of range");
         hgdg::Vec& v = hgdg::vec_at(ctx, vec);
         const double g[5] = {center[0], center[1], ctx->dim == 3 ? center[2] : 0.0, width, amplitude};

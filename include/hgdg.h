@@ -53,8 +53,9 @@ enum {
 enum {
     HGDG_OP_ADVECTION_UPWIND = 1, /* u_t + a.grad u = 0, upwind flux
                                      (tests/self/Basic/080AdvectionLinear_SelfTest.cpp:108-170) */
-    HGDG_OP_ACOUSTIC_CENTRAL = 2  /* first-order wave system, central flux, reflecting walls
+    HGDG_OP_ACOUSTIC_CENTRAL = 2, /* first-order wave system, central flux, reflecting walls
                                      (applications/ExampleMultipleVariableProblem/AcousticWaveLinear_Impl.h:102-283) */
+    HGDG_OP_MATRIX = 3            /* any linear operator given by its matrices (hgdg_operator_set_matrices) */
 };
 
 /* ---- arithmetic of the bulk contractions ------------------------------- */
@@ -137,6 +138,34 @@ int hgdg_set_rk_fused(hgdg_ctx* ctx, int enable);
 /* The setting above and how many single-workgroup RK launches the context
  * has made. launches may be NULL. */
 int hgdg_get_rk_fused(hgdg_ctx* ctx, int* enable, int64_t* launches);
+
+/* A linear operator given by the matrices HpgemAPILinear builds from the
+ * application's own integrands (createMassMatrices / createStiffnessMatrices,
+ * kernel/Base/HpgemAPILinear_Impl.h:131-139,250-288, from
+ * computeIntegrandMassMatrix and computeIntegrandStiffnessMatrixAtElement /
+ * AtFace): the device then runs multiplyStiffnessMatrices (:360-434),
+ * solveMassMatrixEquations (:143-156) and everything built on them
+ * (hgdg_time_derivative, hgdg_right_hand_side, hgdg_solve_mass, hgdg_rk_steps,
+ * hgdg_rk_adaptive_step, hgdg_step_error, hgdg_state_diff). Vectors are in
+ * hpGEM's layout (elements in order, iVB = iV nB + iB inside each), with no
+ * permutation. All matrices column-major, rows = test functions, columns =
+ * trial functions. Projection and error norms against a function need the
+ * geometry and stay with the caller (hpGEM's own integrators).
+ * Replaces the context's mesh and operator; reallocate vectors afterwards. */
+typedef struct hgdg_matrix_desc {
+    int64_t n_elements;
+    int32_t n_fields;             /* unknowns per basis function (informative)             */
+    const int32_t* block_size;    /* [E] unknowns of element e, nB(e) * n_fields, <= 1024   */
+    const double* mass;           /* M_e, element after element, n_e x n_e each             */
+    const double* stiffness;      /* S_e (the element part), same layout                    */
+    int64_t n_faces;
+    const int64_t* face_elements; /* [F][2] left and right element (right = -1: boundary)   */
+    const double* face_blocks;    /* per face, the FaceMatrix blocks in order LL (nL x nL),
+                                     LR (nL x nR), RL (nR x nL), RR (nR x nR); boundary faces
+                                     LL only (FaceMatrix::getElementMatrix, FaceMatrix.h:61-142) */
+} hgdg_matrix_desc;
+
+int hgdg_operator_set_matrices(hgdg_ctx* ctx, const hgdg_matrix_desc* desc, int precision);
 
 /* Quadrilateral and hexahedral meshes take one of two kernel families:
  * sum-factorised kernels for affine elements (parallelograms,

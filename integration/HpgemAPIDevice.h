@@ -37,6 +37,7 @@
 #include <limits>
 #include <string>
 #include <tuple>
+#include <map>
 #include <vector>
 
 #include "Base/Element.h"
@@ -73,6 +74,22 @@ class HpgemAPIDevice : public HpgemAPILinear<DIM> {
           kind_(kind),
           precision_(precision),
           velocity_(velocity) {
+        check(hgdg_ctx_create(&ctx_, device));
+    }
+
+    // Any linear application of HpgemAPILinear: its own integrand hooks
+    // (computeIntegrandMassMatrix, computeIntegrandStiffnessMatrixAtElement /
+    // AtFace) build the element and face matrices on the host once, through
+    // the reference's createMassMatrices / createStiffnessMatrices, and the
+    // device runs the time loop on them (hgdg_operator_set_matrices).
+    struct MatrixOperator {};
+    HpgemAPIDevice(MatrixOperator, const std::size_t numberOfUnknowns, const std::size_t polynomialOrder,
+                   const TimeIntegration::ButcherTableau* const ptrButcherTableau, int precision = HGDG_PREC_FP64,
+                   int device = 0, const bool useSourceTerm = false, const bool useSourceTermAtBoundary = false)
+        : HpgemAPILinear<DIM>(numberOfUnknowns, polynomialOrder, ptrButcherTableau, 0, useSourceTerm,
+                              useSourceTermAtBoundary),
+          kind_(HGDG_OP_MATRIX),
+          precision_(precision) {
         check(hgdg_ctx_create(&ctx_, device));
     }
 
@@ -360,6 +377,10 @@ class HpgemAPIDevice : public HpgemAPILinear<DIM> {
     // coordinates in the reference geometry's node order and the topological
     // node ids (periodic images share a node).
     void upload() {
+        if (kind_ == HGDG_OP_MATRIX) {
+            uploadMatrices();
+            return;
+        }
         std::vector<double> X;
         std::vector<int64_t> nodes;
         std::vector<int32_t> order;
@@ -398,6 +419,59 @@ class HpgemAPIDevice : public HpgemAPILinear<DIM> {
         check(hgdg_mesh_info(ctx_, nullptr, nullptr, nullptr, &nDof_));
         // every time-integration vector hpGEM keeps on the elements, plus one
         // scratch vector for source terms
+        const std::size_t nVectors = this->meshes_[0]->getElementsList().size() > 0
+                                         ? (*this->meshes_[0]->getElementsList().begin())->getNumberOfTimeIntegrationVectors()
+                                         : 0;
+        scratchId_ = int(nVectors);
+        check(hgdg_state_alloc(ctx_, int(nVectors) + 1));
+        uploaded_ = true;
+    }
+
+    // HpgemAPILinear::tasksBeforeSolving (HpgemAPILinear_Impl.h:500-506) on
+    // the host, with the application's integrands, then every matrix to the
+    // device in hgdg_matrix_desc's layout (column-major, rows = test).
+    void uploadMatrices() {
+        HpgemAPILinear<DIM>::tasksBeforeSolving();
+        std::map<const Element*, int64_t> index;
+        std::vector<int32_t> size;
+        std::vector<double> mass, stiffness;
+        auto append = [](std::vector<double>& to, const LinearAlgebra::MiddleSizeMatrix& A) {
+            for (std::size_t c = 0; c < A.getNumberOfColumns(); ++c)
+                for (std::size_t r = 0; r < A.getNumberOfRows(); ++r) to.push_back(A(r, c));
+        };
+        for (Element* el : this->meshes_[0]->getElementsList()) {
+            index[el] = int64_t(size.size());
+            const LinearAlgebra::MiddleSizeMatrix& M = el->getElementMatrix(this->massMatrixID_);
+            size.push_back(int32_t(M.getNumberOfRows()));
+            append(mass, M);
+            append(stiffness, el->getElementMatrix(this->stiffnessElementMatrixID_));
+        }
+        std::vector<int64_t> faceElements;
+        std::vector<double> blocks;
+        for (Face* face : this->meshes_[0]->getFacesList()) {
+            const FaceMatrix& F = face->getFaceMatrix(this->stiffnessFaceMatrixID_);
+            faceElements.push_back(index.at(face->getPtrElementLeft()));
+            append(blocks, F.getElementMatrix(Side::LEFT, Side::LEFT));
+            if (face->isInternal()) {
+                faceElements.push_back(index.at(face->getPtrElementRight()));
+                append(blocks, F.getElementMatrix(Side::LEFT, Side::RIGHT));
+                append(blocks, F.getElementMatrix(Side::RIGHT, Side::LEFT));
+                append(blocks, F.getElementMatrix(Side::RIGHT, Side::RIGHT));
+            } else {
+                faceElements.push_back(-1);
+            }
+        }
+        hgdg_matrix_desc d;
+        d.n_elements = int64_t(size.size());
+        d.n_fields = int32_t(this->configData_->numberOfUnknowns_);
+        d.block_size = size.data();
+        d.mass = mass.data();
+        d.stiffness = stiffness.data();
+        d.n_faces = int64_t(faceElements.size() / 2);
+        d.face_elements = faceElements.data();
+        d.face_blocks = blocks.data();
+        check(hgdg_operator_set_matrices(ctx_, &d, precision_));
+        check(hgdg_mesh_info(ctx_, nullptr, nullptr, nullptr, &nDof_));
         const std::size_t nVectors = this->meshes_[0]->getElementsList().size() > 0
                                          ? (*this->meshes_[0]->getElementsList().begin())->getNumberOfTimeIntegrationVectors()
                                          : 0;

@@ -1982,4 +1982,27 @@ int orc_mass_matrix(void* h, int64_t e, double* out) {
     })
 }
 
+// Element stiffness S_e (column-major n x n, n = nB(e) * nf) and the four
+// blocks of face f's matrix (LL, LR, RL, RR, each column-major with rows =
+// test side, columns = trial side; boundary faces: LL only), as
+// HpgemAPILinear holds them after createStiffnessMatrices: for checks of a
+// device path that takes the matrices.
+int orc_element_stiffness(void* h, int64_t e, double* out) {
+    ORC_TRY({
+        Problem* P = static_cast<Problem*>(h);
+        if (!P->assembled) assemble(*P);
+        for (size_t k = 0; k < P->S[e].size(); ++k) out[k] = double(P->S[e][k]);
+    })
+}
+
+int orc_face_blocks(void* h, int64_t f, double* out) {
+    ORC_TRY({
+        Problem* P = static_cast<Problem*>(h);
+        if (!P->assembled) assemble(*P);
+        size_t o = 0;
+        for (int b = 0; b < 4; ++b)
+            for (Real v : P->F[f][b]) out[o++] = double(v);
+    })
+}
+
 }  // extern "C"

@@ -323,6 +323,29 @@ class Problem:
         check(self.lib.orc_l2_error_fields(self.h, _dp(u), len(kk), _dp(amp), _dp(kk), _dp(sh), _dp(out)))
         return out
 
+    def matrices(self):
+        """(M_e list, S_e list, face rows [L, R], face blocks concatenated LL, LR,
+        RL, RR (boundary: LL), all column-major) as HpgemAPILinear holds them."""
+        faces = self.faces()
+        M, S = [], []
+        sizes = [self.lib.orc_basis_size(self.shape, int(p)) * self.n_fields for p in self.orders]
+        for e in range(self.n_elem):
+            n = sizes[e]
+            m, st = np.zeros(n * n), np.zeros(n * n)
+            check(self.lib.orc_mass_matrix(self.h, e, _dp(m)))
+            check(self.lib.orc_element_stiffness(self.h, e, _dp(st)))
+            M.append(m)
+            S.append(st)
+        blocks = []
+        for f in range(self.n_faces):
+            L, R = int(faces[f][0]), int(faces[f][1])
+            nL = sizes[L]
+            nR = sizes[R] if R >= 0 else 0
+            out = np.zeros((nL + nR) ** 2)
+            check(self.lib.orc_face_blocks(self.h, f, _dp(out)))
+            blocks.append(out[: nL * nL] if R < 0 else out)
+        return M, S, faces[:, :2].copy(), blocks, sizes
+
     def mass_matrix(self, e):
         nb = self.n_dof // self.n_elem if np.all(self.orders == self.orders[0]) else None
         n = nb
