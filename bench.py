@@ -405,7 +405,9 @@ def run_serving(precision, n_requests, seed=SEED, profile=False):
     return {"requests": n_requests, "completed": st["completed"], "failed": st["failed"],
             "latency_p50_s": st["latency_p50_s"], "latency_p99_s": st["latency_p99_s"],
             "latency_mean_s": st["latency_mean_s"], "gdof_rhs_per_s": st["dof_rhs_per_s"] / 1e9,
-            "wall_s": wall, "peak_hbm_gb": st["peak_bytes"] / 2 ** 30}
+            "wall_s": wall, "peak_hbm_gb": st["peak_bytes"] / 2 ** 30,
+            "worker": {"ticks": st["ticks"], "restages": st["restages"], "restage_s": st["restage_s"],
+                       "tick_s": st["tick_s"]}}
 
 
 def cpu_model():

@@ -137,7 +137,8 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("failed", ctypes.c_int64),
                 ("active", ctypes.c_int64), ("latency_p50_s", ctypes.c_double), ("latency_p99_s", ctypes.c_double),
                 ("latency_mean_s", ctypes.c_double), ("dof_rhs_per_s", ctypes.c_double), ("wall_s", ctypes.c_double),
-                ("peak_bytes", ctypes.c_int64)]
+                ("peak_bytes", ctypes.c_int64), ("ticks", ctypes.c_int64), ("restages", ctypes.c_int64),
+                ("restage_s", ctypes.c_double), ("tick_s", ctypes.c_double)]
 
 
 SIGNATURES += [

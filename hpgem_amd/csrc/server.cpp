@@ -155,6 +155,9 @@ struct hgdg_server {
     std::map<std::array<int, 3>, std::pair<double, int64_t>> family_ms;  // guarded by mu
     double update_ms = 0;
     int64_t update_launches = 0;
+    // worker accounting (guarded by mu): ticks, re-stagings and their wall time
+    int64_t n_ticks = 0, n_restages = 0;
+    double restage_s = 0, tick_s = 0;
     void drop_events() {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         events.clear();
@@ -353,9 +356,16 @@ void hgdg_server::tick(std::vector<hgdg::Solve*>& solves, int n_steps) {
     bool same = solves.size() == staged.size();
     for (size_t i = 0; same && i < solves.size(); ++i) same = solves[i]->req.ticket == staged[i];
     const bool timed = profile.load();
+    const double t0 = now_s();
     if (!same || (timed && events.size() < 2 * launches.size() + 2)) {
         std::lock_guard<std::mutex> hk(hip_mu);
-        if (!same) stage(solves);
+        if (!same) {
+            stage(solves);
+            const double dt = now_s() - t0;
+            std::lock_guard<std::mutex> lk(mu);
+            ++n_restages;
+            restage_s += dt;
+        }
         while (timed && events.size() < 2 * launches.size() + 2) {
             hipEvent_t e;
             HGDG_HIP(hipEventCreate(&e));
@@ -365,6 +375,9 @@ void hgdg_server::tick(std::vector<hgdg::Solve*>& solves, int n_steps) {
     if (!timed) {
         for (int step = 0; step < n_steps; ++step) HGDG_HIP(hipGraphLaunch(step_exec, stream));
         HGDG_HIP(hipStreamSynchronize(stream));
+        std::lock_guard<std::mutex> lk(mu);
+        n_ticks += n_steps;
+        tick_s += now_s() - t0;
         return;
     }
     for (int step = 0; step < n_steps; ++step) {
@@ -393,7 +406,10 @@ void hgdg_server::tick(std::vector<hgdg::Solve*>& solves, int n_steps) {
         HGDG_HIP(hipEventElapsedTime(&ms, events[u], events[u + 1]));
         update_ms += ms;
         ++update_launches;
+        ++n_ticks;
     }
+    std::lock_guard<std::mutex> lk(mu);
+    tick_s += now_s() - t0;
 }
 
 void hgdg_server::worker_loop() {
@@ -633,6 +649,10 @@ int hgdg_server_get_stats(hgdg_server* s, hgdg_server_stats* out) {
         st.wall_s = s->t_first_submit < 0 ? 0 : s->t_last_done - s->t_first_submit;
         st.dof_rhs_per_s = st.wall_s > 0 ? s->dof_rhs / st.wall_s : 0;
         st.peak_bytes = s->peak;
+        st.ticks = s->n_ticks;
+        st.restages = s->n_restages;
+        st.restage_s = s->restage_s;
+        st.tick_s = s->tick_s;
         *out = st;
     });
 }

@@ -357,6 +357,10 @@ typedef struct hgdg_server_stats {
     double dof_rhs_per_s;   /* sum of DOF x RHS evaluations / wall time        */
     double wall_s;          /* first submit -> last completion                 */
     int64_t peak_bytes;     /* peak admitted footprint                         */
+    int64_t ticks;          /* worker ticks (RK steps of the resident set)     */
+    int64_t restages;       /* re-stagings (resident set changed: new graph)   */
+    double restage_s;       /* worker wall time spent re-staging               */
+    double tick_s;          /* worker wall time in ticks, re-staging included  */
 } hgdg_server_stats;
 
 typedef struct hgdg_server hgdg_server;

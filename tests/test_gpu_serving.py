@@ -92,6 +92,10 @@ def test_admission_budget_and_stream_statistics():
     assert st["latency_p99_s"] >= st["latency_p50_s"] > 0
     big = max(srv.result(t)["n_dof"] for t in tickets)
     assert st["peak_bytes"] <= max(0.02 * (1 << 30), 200 * big)
+    # every request takes 8 steps; the resident set steps together
+    assert 8 <= st["ticks"] <= 8 * len(reqs)
+    assert 1 <= st["restages"] <= 2 * len(reqs)
+    assert 0 < st["restage_s"] <= st["tick_s"]
     srv.close()
 
 
