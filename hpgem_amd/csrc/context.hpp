@@ -53,6 +53,8 @@ struct MatrixOp {
 struct SimplexMesh {
     int64_t group_start[5] = {0, 0, 0, 0, 0};    // device elements of order g+1: [start[g], start[g+1])
     int64_t group_dof[5] = {0, 0, 0, 0, 0};      // first DOF of each group
+    int64_t group_tr[5] = {0, 0, 0, 0, 0};       // first face trace of each group (4 (p+1)(p+2)/2 per element)
+    DevBuf<double> d_trace;                      // face traces of the stage input, double2 per coefficient
     std::vector<int64_t> dev_of;                 // mesh element -> device element
     std::vector<int64_t> mesh_of;                // device element -> mesh element
     std::vector<int64_t> rank;                   // mesh element -> processing (locality) rank

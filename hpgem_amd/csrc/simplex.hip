@@ -43,11 +43,13 @@ constexpr int kTixM = 1;             // [nb][nb]: M_ref
 constexpr int kTixCols = 2;          // + f
 constexpr int kTixProj = 6;          // [4 f][6 o][ncol][15 m]: chi_m-coefficients of the face columns' traces
 constexpr int kTixLift = 7;          // [4 f][ncol m][nb i]: (M_ref^{-1} int_f phi chi_m)_i, own face coordinates
-constexpr int kTixStride = 8;
+constexpr int kTixProjId = 8;        // [4 f][ncol c][ncol m]: the trace table at the identity orientation, m < ncol
+constexpr int kTixStride = 9;
 constexpr int kTixRel = 4 * kTixStride;  // [6 own][6 neighbour]: orientation of the neighbour's face seen from ours
 constexpr int kTixIdent = kTixRel + 36;  // the identity orientation
+constexpr int kTixRot = kTixIdent + 1;   // [6 o][55]: the orientation maps on face-basis coefficients, by degree block
 constexpr int kMaxCols = 15;         // face columns of an order-4 element
-constexpr int kTixMax = kTixIdent + 1 + 4 * 4 * kMaxCols;  // whole index table (LDS copy in the kernel)
+constexpr int kTixMax = kTixRot + 1 + 4 * 4 * kMaxCols;  // whole index table
 
 // packed per-face info
 __host__ __device__ constexpr int info_lf(int v) { return v & 3; }
@@ -83,6 +85,8 @@ struct SimplexParams {
     const int32_t* tix;
     int32_t n_tix;
     const int4* blocks;    // per block: order, first device element, element count, threads-per-element variant (informative)
+    double2* trace;        // face traces of the stage input: [element][4 f][ncol(p) m] (n.s, v), own face coordinates
+    int64_t group_tr[5];   // first trace of each order group
 };
 
 __host__ __device__ constexpr int simplex_nb(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
@@ -107,33 +111,32 @@ struct SimplexShape {
     // doubles taken by n values of the arithmetic type (TB bytes), rounded
     // up to an even count (regions start double2-aligned)
     static constexpr int tw(int n) { return (n * TB + 15) / 16 * 2; }
-    // LDS carve-up of the RHS kernel, in doubles. FM, FI and U live through
-    // the whole kernel; the work area W holds in turn the volume data (G, DR)
-    // and the face data (SO, SN, SF); LIFT follows.
+    // LDS carve-up of the RHS kernel, in doubles. FM and SF live through the
+    // whole kernel; the work area W holds the volume data (G, DR) and, at
+    // order 4 in FP64, afterwards one face's lift table; LIFT follows.
     static constexpr int FM = 0;                            // [EPB][4 faces][4]: unit normal, sigma
-    static constexpr int FI = FM + EPB * 16;                // [EPB][4 faces] (int2): neighbour, packed face info
-    static constexpr int U = FI + EPB * 4;                  // [EPB][4][NB] stage input
-    static constexpr int W = U + EPB * 4 * NB;
+    static constexpr int SF = FM + EPB * 16;                // [EPB][4 faces][NCOL] (T pairs): flux coefficients
+    static constexpr int W = SF + tw(EPB * 8 * NCOL);
     static constexpr int G = W;                             // [EPB][4][NB] (T): v, then g_r = sum_d (J^{-1})_{r,d} s_d
     // DrT staged in LDS below order 4, and at order 4 in FP32 (in FP64 its
-    // 29 KB are read through the caches)
+    // 29 KB pass through LDS one direction at a time)
     static constexpr bool DR_LDS = P < 4 || TB == 4;
-    // otherwise (FP64, order 4) one direction's table at a time, [NB][NB]
-    static constexpr int DR = G + tw(EPB * 4 * NB);         // [3][NB][NB] (T): DrT
+    static constexpr int DR = G + tw(EPB * 4 * NB);         // [3][NB][NB] (T): DrT, or [NB][NB] one direction
     static constexpr int VOL_END = DR + (DR_LDS ? tw(3 * NB * NB) : tw(NB * NB));
-    static constexpr int SO = W;                            // [EPB][4 faces][NCOL] (double2): n.s and v at own face columns
-    static constexpr int SN = SO + EPB * 8 * NCOL;          // [EPB][4 faces][kMaxCols] (double2): the same for the neighbours
-    static constexpr int SF = SN + EPB * 8 * kMaxCols;      // [EPB][4 faces][NCOL] (T pairs): flux coefficients
-    static constexpr int FACE_END = SF + tw(EPB * 8 * NCOL);
     static constexpr bool LIFT_LDS = P < 4;                 // lift table staged in LDS for the whole kernel
-    // order 4 in FP64: one face's lift table at a time in the stage-input
-    // area (in FP32 it is read through the caches, which measured faster)
+    // order 4 in FP64: one face's lift table at a time in the work area
+    // once the volume terms are done (in FP32 it is read through the caches,
+    // which measured faster)
     static constexpr bool LIFT_ONE = !LIFT_LDS && TB == 8;
-    static constexpr int LIFT = FACE_END > VOL_END ? FACE_END : VOL_END;  // [4 f][NCOL m][NB i] (T), the whole kernel
+    static constexpr int LIFT = VOL_END;                    // [4 f][NCOL m][NB i] (T), the whole kernel
     static constexpr int TOTAL = LIFT + (LIFT_LDS ? tw(4 * NCOL * NB) : 0);
-    static_assert(FI % 2 == 0 && U % 2 == 0 && W % 2 == 0 && SN % 2 == 0 && SF % 2 == 0, "double2 alignment");
+    static_assert(!LIFT_ONE || tw(NCOL * NB) <= VOL_END - W, "a face's lift table fits the work area");
+    static_assert(SF % 2 == 0 && W % 2 == 0 && DR % 2 == 0 && LIFT % 2 == 0, "double2 alignment");
 };
 
+// Face-basis degree of coefficient m < 15 (degree d holds m in
+// [d(d+1)/2, (d+1)(d+2)/2)).
+__host__ __device__ constexpr int face_degree(int m) { return (m >= 1) + (m >= 3) + (m >= 6) + (m >= 10); }
 
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ mfma_d4 mfma_step(double a, double b, mfma_d4 c) {
@@ -145,45 +148,136 @@ struct FluxPair {
     T v, s;  // flux coefficient of the v equation, of the s equations
 };
 
-// Sum over c < NC of col[15 c] * x[c] for both components of x (a column
-// of a [c][15] table: the threads of consecutive m read consecutive words).
-template <int NC>
-__device__ __forceinline__ void dot_pairs(const double* col, const double2* x, double& a, double& b) {
-    a = 0;
-    b = 0;
+// ---- face traces (the first kernel of a right-hand side)
+// Every element writes, for each local face f, the chi_m-coefficients
+// (m < (p+1)(p+2)/2, degree <= p) of n_f.s and of v on that face in its own
+// face coordinates, n_f its own outward unit normal, into the trace buffer;
+// the element's neighbours read them in the RHS kernel. One block takes EPB
+// elements of one order, TPE threads each: stage input, normals and the
+// identity-orientation trace table into LDS; items (face, column): n.s and v
+// at the face columns; items (face, m): their projections.
+template <int P>
+struct TraceShape {
+    static constexpr int NB = simplex_nb(P), NC = simplex_ncol(P), TPE = simplex_tpe(P), EPB = kBlock / TPE;
+    static constexpr int FM = 0;                    // [EPB][4 f][3] unit normals
+    static constexpr int U = FM + EPB * 12;         // [EPB][4][NB] stage input
+    static constexpr int SO = U + EPB * 4 * NB;     // [EPB][4 f][NC] (double2): n.s and v at the face columns
+    static constexpr int PR = SO + EPB * 8 * NC;    // [4 f][NC c][NC m]: identity-orientation trace table
+    static constexpr int TOTAL = PR + 4 * NC * NC;
+    static_assert(SO % 2 == 0, "double2 alignment");
+};
+
+constexpr int trace_lds_doubles() {
+    constexpr int t[4] = {TraceShape<1>::TOTAL, TraceShape<2>::TOTAL, TraceShape<3>::TOTAL, TraceShape<4>::TOTAL};
+    int m = 0;
+    for (int k = 0; k < 4; ++k) m = t[k] > m ? t[k] : m;
+    return m;
+}
+
+template <int P>
+__device__ __forceinline__ void simplex_trace_block(const SimplexParams& p, int64_t first, int count, double* lds) {
+    using S = TraceShape<P>;
+    constexpr int NB = S::NB, NC = S::NC;
+    const int tid = threadIdx.x;
+    const int le = tid / S::TPE;
+    const int i = tid % S::TPE;
+    const int64_t e = first + le;
+    const bool elem_ok = le < count;
+    const int64_t E = p.n_elem;
+    const int32_t* tx = p.tix + (P - 1) * kTixStride;
+    double* fm = lds + S::FM + le * 12;
+    double* su = lds + S::U + le * 4 * NB;
+    if (elem_ok) {
+        if (i < 12) fm[i] = p.normal[((i % 3) * 4 + i / 3) * E + e];  // fm[3 f + c]
+        if (i < NB) {
+            const int64_t base = p.group_dof[P - 1] + (e - p.group_start[P - 1]) * 4 * NB;
+            const int64_t at[4] = {base + i, base + NB + i, base + 2 * NB + i, base + 3 * NB + i};
+            double sv[4];
+            combine_n(p.io, at, sv);
+#pragma unroll
+            for (int f = 0; f < 4; ++f) su[f * NB + i] = sv[f];
+        }
+    }
+    {
+        const double* gP = p.tab + tx[kTixProjId];
+        double* sP = lds + S::PR;
+#pragma unroll 4
+        for (int k = tid; k < 4 * NC * NC; k += kBlock) sP[k] = gP[k];
+    }
+    __syncthreads();
+    double2* so = reinterpret_cast<double2*>(lds + S::SO) + le * 4 * NC;
+    if (elem_ok) {
+        for (int it = i; it < 4 * NC; it += S::TPE) {
+            const int f = it / NC, c = it % NC;
+            const int col = p.tix[tx[kTixCols + f] + c];
+            so[it] = make_double2(fm[f * 3] * su[NB + col] + fm[f * 3 + 1] * su[2 * NB + col] +
+                                      fm[f * 3 + 2] * su[3 * NB + col],
+                                  su[col]);
+        }
+    }
+    __syncthreads();
+    if (elem_ok) {
+        double2* out = p.trace + p.group_tr[P - 1] + (e - p.group_start[P - 1]) * 4 * NC;
+        for (int it = i; it < 4 * NC; it += S::TPE) {
+            const int f = it / NC, m = it % NC;
+            const double* col = lds + S::PR + f * NC * NC + m;
+            const double2* x = so + f * NC;
+            double a = 0, b = 0;
 #pragma unroll 5
-    for (int c = 0; c < NC; ++c) {
-        const double r = col[c * 15];
-        const double2 v = x[c];
-        a += r * v.x;
-        b += r * v.y;
+            for (int c = 0; c < NC; ++c) {
+                const double r = col[c * NC];
+                const double2 v = x[c];
+                a += r * v.x;
+                b += r * v.y;
+            }
+            out[it] = make_double2(a, b);
+        }
     }
 }
 
-// Tetrahedral elements of order P, one launch per order: a block takes EPB
-// consecutive device elements of that order, TPE threads per element, in
-// phases separated by barriers:
-//  1. the stage input (all four fields), v and g_r = sum_d (J^{-1})_{r,d} s_d
-//     per basis function, the faces' normals, area ratios, neighbours and
-//     orientation data, and the order's derivative tables into LDS;
+// Blocks in order-group order: group g takes blocks [group_block[g],
+// group_block[g + 1]) of 256 / simplex_tpe(g + 1) elements each.
+__global__ __launch_bounds__(kBlock) void simplex_trace(const SimplexParams p) {
+    __shared__ __align__(16) double lds[trace_lds_doubles()];
+    const int b = int(blockIdx.x);
+    int g = 0;
+    while (g < 3 && b >= p.group_block[g + 1]) ++g;
+    const int epb = kBlock / simplex_tpe(g + 1);
+    const int64_t first = p.group_start[g] + int64_t(b - p.group_block[g]) * epb;
+    const int64_t left = p.group_start[g + 1] - first;
+    const int count = left < epb ? int(left) : epb;
+    switch (g) {
+        case 0: simplex_trace_block<1>(p, first, count, lds); break;
+        case 1: simplex_trace_block<2>(p, first, count, lds); break;
+        case 2: simplex_trace_block<3>(p, first, count, lds); break;
+        default: simplex_trace_block<4>(p, first, count, lds); break;
+    }
+}
+
+// ---- the right-hand side (the second kernel)
+// Tetrahedral elements of order P: a block takes EPB consecutive device
+// elements of that order, TPE threads per element, in phases separated by
+// barriers:
+//  1. the stage input (all four fields) as v and g_r = sum_d (J^{-1})_{r,d}
+//     s_d per basis function, the faces' normals and area ratios, and the
+//     order's derivative and lift tables into LDS; items (face, m): the flux
+//     coefficients from the element's own traces and its neighbours' (in the
+//     neighbour's face coordinates, brought into ours by the orientation
+//     map's degree block, R_o; the neighbour's n.s is with its own normal,
+//     the negative of ours), central on internal faces, the wall flux on
+//     reflecting walls;
 //  2. threads of a row: the volume terms;
-//  3. items (face, column): n.s and v at the face columns (the basis
-//     functions whose trace does not vanish there) of the element and, with
-//     the element's normal, of each neighbour;
-//  4. items (face, m): the chi_m-coefficients of both traces in the element's
-//     face coordinates (the neighbour's through the orientation that maps
-//     its face coordinates onto ours) and of the central / wall flux;
-//  5. threads of a row: the lift of the flux coefficients.
+//  3. threads of a row: the lift of the flux coefficients.
 // Every face term is a polynomial of degree <= 4 in the face coordinates and
 // chi spans P_4, so the coefficients represent it exactly; only chi_m of
 // degree <= p reach the lift (chi_m of higher degree are orthogonal to the
 // element's traces), so a face of an order-p element carries (p+1)(p+2)/2
-// coefficients. T = double: FP64; T = float: the volume contractions and the
-// lifts in FP32 on FP32 tables, compensated per-row sums, traces and flux
-// coefficients in FP64.
+// coefficients, and the orientation maps keep degrees apart, so a neighbour of
+// order p_n contributes to degrees <= min(p, p_n) only. T = double: FP64;
+// T = float: the volume contractions and the lifts in FP32 on FP32 tables,
+// compensated per-row sums, traces and flux coefficients in FP64.
 template <int P, int TPE, typename T>
-__device__ __forceinline__ void simplex_block(const SimplexParams& p, const int32_t* tix, int64_t first, int count,
-                                              double* lds) {
+__device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t first, int count, double* lds) {
     using S = SimplexShape<P, TPE, int(sizeof(T))>;
     constexpr int NB = S::NB;
     constexpr int NC = S::NCOL;
@@ -196,14 +290,13 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
     const int64_t E = p.n_elem;
     const int64_t base = p.group_dof[P - 1] + (e - p.group_start[P - 1]) * 4 * NB;
     double* fm = lds + S::FM + le * 16;
-    int2* fi4 = reinterpret_cast<int2*>(lds + S::FI) + le * 4;
-    double* su = lds + S::U + le * 4 * NB;
+    FluxPair<T>* sf = reinterpret_cast<FluxPair<T>*>(lds + S::SF) + le * 4 * NC;
     T* sg = reinterpret_cast<T*>(lds + S::G) + le * 4 * NB;
     T* sDr = reinterpret_cast<T*>(lds + S::DR);
     const T* tabT = std::is_same<T, float>::value ? reinterpret_cast<const T*>(p.tab32)
                                                   : reinterpret_cast<const T*>(p.tab);
 
-    // ---- phase 1: geometry, stage input, v and g, the derivative tables
+    // ---- phase 1: geometry, stage input as v and g, flux coefficients, tables
     double cinv = 1.0;
     if (elem_ok) {
         cinv = p.cinv[e];
@@ -211,7 +304,6 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
             const int f = i / 4, c = i % 4;
             fm[i] = c < 3 ? p.normal[(c * 4 + f) * E + e] : p.sigma[f * E + e];
         }
-        if (i < 4) fi4[i] = make_int2(p.nbr[i * E + e], p.info[i * E + e]);
     }
     if (row_ok) {
         double Ji[9];
@@ -220,11 +312,45 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
         const int64_t at[4] = {base + i, base + NB + i, base + 2 * NB + i, base + 3 * NB + i};
         double sv[4];
         combine_n(p.io, at, sv);
-#pragma unroll
-        for (int f = 0; f < 4; ++f) su[f * NB + i] = sv[f];
         sg[i] = T(sv[0]);
 #pragma unroll
         for (int r = 0; r < 3; ++r) sg[(1 + r) * NB + i] = T(Ji[r * 3] * sv[1] + Ji[r * 3 + 1] * sv[2] + Ji[r * 3 + 2] * sv[3]);
+    }
+    if (elem_ok) {
+        const double2* own = p.trace + p.group_tr[P - 1] + (e - p.group_start[P - 1]) * 4 * NC;
+        const double* rot = p.tab + p.tix[kTixRot];
+#pragma unroll 1
+        for (int it = i; it < 4 * NC; it += S::TPE) {
+            const int f = it / NC, m = it % NC;
+            const int nbr = p.nbr[f * E + e];
+            const double2 a = own[it];
+            double Fv, Fs;
+            if (nbr >= 0) {
+                const int inf = p.info[f * E + e];
+                const int pn = info_pn(inf);
+                const int d = face_degree(m);
+                double bns = 0, bv = 0;
+                if (d <= pn) {
+                    const int o = p.tix[kTixRel + info_oown(inf) * 6 + info_onbr(inf)];
+                    const int m0 = d * (d + 1) / 2, ncn = simplex_ncol(pn);
+                    const double* R = rot + o * 55 + d * (d + 1) * (2 * d + 1) / 6 + (m - m0) * (d + 1);
+                    const double2* x = p.trace + p.group_tr[pn - 1] + (nbr - p.group_start[pn - 1]) * 4 * ncn +
+                                       info_lf(inf) * ncn + m0;
+                    for (int b = 0; b <= d; ++b) {
+                        const double r = R[b];
+                        const double2 v = x[b];
+                        bns += r * v.x;
+                        bv += r * v.y;
+                    }
+                }
+                Fv = -0.5 * (a.x + bns);  // n.(s_own - s_nbr), the neighbour's traces with n_nbr = -n
+                Fs = -0.5 * (a.y - bv);
+            } else {
+                Fv = -a.x;
+                Fs = 0;
+            }
+            sf[it] = FluxPair<T>{T(Fv), T(Fs)};
+        }
     }
     const T* gDr = tabT + p.tix[(P - 1) * kTixStride + kTixDrT];
     {
@@ -355,119 +481,16 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, const int3
                 ks[d] = Acc<T>((Jt[d] * b[0] + Jt[3 + d] * b[1] + Jt[6 + d] * b[2]) * ci);
         }
     }
-    __syncthreads();  // the volume data give way to the face data
 
-    // ---- phase 3: n.s and v at the face columns, own items (f, c < NC)
-    // first, then each neighbour's (f, c < ncol(p_n))
-    double2* so = reinterpret_cast<double2*>(lds + S::SO) + le * 4 * NC;
-    double2* sn = reinterpret_cast<double2*>(lds + S::SN) + le * 4 * kMaxCols;
-    if (elem_ok) {
-#pragma unroll 1
-        for (int it = i; it < 4 * NC; it += S::TPE) {
-            const int f = it / NC, c = it % NC;
-            const int col = tix[tix[(P - 1) * kTixStride + kTixCols + f] + c];
-            so[f * NC + c] = make_double2(fm[f * 4] * su[NB + col] + fm[f * 4 + 1] * su[2 * NB + col] +
-                                              fm[f * 4 + 2] * su[3 * NB + col],
-                                          su[col]);
-        }
-        int nstart[5];
-        nstart[0] = 0;
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            const int2 nf = fi4[f];
-            nstart[f + 1] = nstart[f] + (nf.x >= 0 ? simplex_ncol(info_pn(nf.y)) : 0);
-        }
-        // the neighbours' items, at most 4 * kMaxCols, two per thread at a
-        // time: both items take their addresses first and load together (an
-        // item past the end reads the element's own first value and is not
-        // stored)
-        constexpr int NIT = (4 * kMaxCols + S::TPE - 1) / S::TPE;
-#pragma unroll 1
-        for (int k0 = 0; k0 < NIT && i + k0 * S::TPE < nstart[4]; k0 += 2) {
-            int64_t at[8];
-            int slot[2];
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const int it = i + (k0 + k) * S::TPE;
-                int64_t nbase = base;
-                int nbn = 0;
-                slot[k] = -1;
-                if (it < nstart[4]) {
-                    const int f = it < nstart[2] ? (it < nstart[1] ? 0 : 1) : (it < nstart[3] ? 2 : 3);
-                    const int c = it - nstart[f];
-                    const int2 nf = fi4[f];
-                    const int pn = info_pn(nf.y);
-                    nbn = simplex_nb(pn);
-                    const int col = tix[tix[(pn - 1) * kTixStride + kTixCols + info_lf(nf.y)] + c];
-                    nbase = p.group_dof[pn - 1] + (nf.x - p.group_start[pn - 1]) * 4 * nbn + col;
-                    slot[k] = f * kMaxCols + c;
-                }
-#pragma unroll
-                for (int fld = 0; fld < 4; ++fld) at[4 * k + fld] = nbase + fld * nbn;
-            }
-            double w[8];
-            combine_n(p.io, at, w);
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                if (slot[k] < 0) continue;
-                const int f = slot[k] / kMaxCols;
-                sn[slot[k]] = make_double2(fm[f * 4] * w[4 * k + 1] + fm[f * 4 + 1] * w[4 * k + 2] +
-                                               fm[f * 4 + 2] * w[4 * k + 3],
-                                           w[4 * k]);
-            }
-        }
-    }
-    __syncthreads();
-
-    // ---- phase 4: flux coefficients, items (f, m < NC)
-    FluxPair<T>* sf = reinterpret_cast<FluxPair<T>*>(lds + S::SF) + le * 4 * NC;
-    if (elem_ok) {
-        const int ident = tix[kTixIdent];
-        const int32_t* tx = tix + (P - 1) * kTixStride;
-#pragma unroll 1
-        for (int it = i; it < 4 * NC; it += S::TPE) {
-            const int f = it / NC, m = it % NC;
-            const int2 nf = fi4[f];
-            double a_ns, a_v;
-            dot_pairs<NC>(p.tab + tx[kTixProj] + (f * 6 + ident) * 15 * NC + m, so + f * NC, a_ns, a_v);
-            double Fv, Fs;
-            if (nf.x >= 0) {
-                const int pn = info_pn(nf.y);
-                const int o = tix[kTixRel + info_oown(nf.y) * 6 + info_onbr(nf.y)];
-                const double2* xn = sn + f * kMaxCols;
-                const int lfn = info_lf(nf.y);
-                const double* proj = p.tab + tix[(pn - 1) * kTixStride + kTixProj];
-                // one loop for every neighbour order: the items of a wave
-                // meet neighbours of different orders, and a loop bounded
-                // per thread costs the longest of them, not their sum
-                const int ncn = simplex_ncol(pn);
-                const double* col = proj + (lfn * 6 + o) * 15 * ncn + m;
-                double b_ns = 0, b_v = 0;
-                for (int c = 0; c < ncn; ++c) {
-                    const double r = col[c * 15];
-                    const double2 v = xn[c];
-                    b_ns += r * v.x;
-                    b_v += r * v.y;
-                }
-                Fv = -0.5 * (a_ns - b_ns);
-                Fs = -0.5 * (a_v - b_v);
-            } else {
-                Fv = -a_ns;
-                Fs = 0;
-            }
-            sf[f * NC + m] = FluxPair<T>{T(Fv), T(Fs)};
-        }
-    }
-    T* sL1 = reinterpret_cast<T*>(lds + S::U);  // one face's lift table at order 4
+    // ---- phase 3: k += sigma_f sum_m (M_ref^{-1} int_f phi chi_m)_i F_m
+    // (order 4 in FP64: the lift table passes through the work area, dead
+    // once the volume terms are done, one face at a time)
+    T* sL1 = reinterpret_cast<T*>(lds + S::W);
     if constexpr (S::LIFT_ONE) {
-        static_assert(NC * NB * sizeof(T) <= sizeof(double) * S::EPB * 4 * NB, "a face's lift table fits the stage-input area");
+        __syncthreads();  // the volume data give way to face 0's lift table
         for (int k = tid; k < NC * NB; k += kBlock) sL1[k] = gLift[k];
+        __syncthreads();
     }
-    __syncthreads();
-
-    // ---- phase 5: k += sigma_f sum_m (M_ref^{-1} int_f phi chi_m)_i F_m
-    // (order 4 in FP64: the lift table passes through the stage-input area,
-    // dead since phase 3, one face at a time; face 0's was loaded in phase 4)
     const T ci = T(1.0 / cinv);
 #pragma unroll 1
     for (int f = 0; f < 4; ++f) {
@@ -535,14 +558,12 @@ constexpr int simplex_lds_doubles() {
 template <typename T, int V>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void simplex_acoustic_rhs(const SimplexParams p) {
     __shared__ __align__(16) double lds[simplex_lds_doubles<int(sizeof(T))>()];  // regions start at even doubles
-    __shared__ int32_t tix[kTixMax];                              // table offsets, orientations, face column lists
-    for (int k = threadIdx.x; k < p.n_tix; k += kBlock) tix[k] = p.tix[k];
     const int4 bl = p.blocks[xcd_block(blockIdx.x, gridDim.x)];
     switch (bl.x) {  // the same order for the whole block; bit p-1 of V: the order's threads-per-element variant
-        case 1: simplex_block<1, simplex_tpe_of(1, V & 1), T>(p, tix, bl.y, bl.z, lds); break;
-        case 2: simplex_block<2, simplex_tpe_of(2, (V >> 1) & 1), T>(p, tix, bl.y, bl.z, lds); break;
-        case 3: simplex_block<3, simplex_tpe_of(3, (V >> 2) & 1), T>(p, tix, bl.y, bl.z, lds); break;
-        default: simplex_block<4, simplex_tpe_of(4, (V >> 3) & 1), T>(p, tix, bl.y, bl.z, lds); break;
+        case 1: simplex_block<1, simplex_tpe_of(1, V & 1), T>(p, bl.y, bl.z, lds); break;
+        case 2: simplex_block<2, simplex_tpe_of(2, (V >> 1) & 1), T>(p, bl.y, bl.z, lds); break;
+        case 3: simplex_block<3, simplex_tpe_of(3, (V >> 2) & 1), T>(p, bl.y, bl.z, lds); break;
+        default: simplex_block<4, simplex_tpe_of(4, (V >> 3) & 1), T>(p, bl.y, bl.z, lds); break;
     }
 }
 
@@ -874,6 +895,9 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     }
     c->n_fields = nf;
     c->dof = sm.group_dof[4];
+    sm.group_tr[0] = 0;
+    for (int g = 0; g < 4; ++g) sm.group_tr[g + 1] = sm.group_tr[g] + counts[g] * 4 * simplex_ncol(g + 1);
+    sm.d_trace.alloc(size_t(2 * sm.group_tr[4]));
     std::vector<int64_t> elem_off(sm.dof_off);  // lives until the synchronize at the end
     elem_off.push_back(c->dof);
     sm.d_elem_off.upload(elem_off, c->stream);
@@ -997,7 +1021,7 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
     // the neighbour's face coordinates are o_nbr(o_own^{-1}(s)), itself one
     // of the six orientation maps (they form a group), identified here by
     // its action on the (symmetric) face rule points.
-    std::vector<int32_t> tix(size_t(kTixIdent) + 1, 0);
+    std::vector<int32_t> tix(size_t(kTixRot) + 1, 0);
     int ident = -1;
     for (int o = 0; o < 6 && ident < 0; ++o) {
         bool id = true;
@@ -1039,6 +1063,10 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
             t[kTixCols + f] = int32_t(cols.size());  // relative, fixed below
             cols.insert(cols.end(), ops.cols[f].begin(), ops.cols[f].end());
         }
+        t[kTixProjId] = int32_t(tab.size());  // [f][c < nc][m < nc]
+        for (int f = 0; f < 4; ++f)
+            for (int c = 0; c < nc; ++c)
+                for (int m = 0; m < nc; ++m) tab.push_back(ops.proj[f][ident][size_t(m) * nc + c]);
         t[kTixProj] = int32_t(tab.size());  // [f][o][c < nc][m < kFaceBasis]
         for (int f = 0; f < 4; ++f)
             for (int o = 0; o < 6; ++o) {
@@ -1050,6 +1078,12 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
         for (int f = 0; f < 4; ++f)
             for (int m = 0; m < nc; ++m)
                 for (int i = 0; i < nb; ++i) tab.push_back(ops.lift[f][ident][size_t(i) * kFaceBasis + m]);
+    }
+    {
+        const std::vector<double> rot = face_orientation_blocks();
+        require(rot.size() == 6 * size_t(kFaceRotBlock) && kFaceRotBlock == 55, HGDG_ERR_ARG, "orientation block table size");
+        tix[kTixRot] = int32_t(tab.size());
+        tab.insert(tab.end(), rot.begin(), rot.end());
     }
     const int32_t col_base = int32_t(tix.size());
     for (int q = 0; q < kMaxSimplexOrder; ++q)
@@ -1153,6 +1187,8 @@ SimplexParams simplex_params(hgdg_ctx* c, const StageIO& io) {
     p.tix = sm.d_tab_index.ptr;
     p.n_tix = int32_t(sm.d_tab_index.n);
     p.blocks = reinterpret_cast<const int4*>(sm.d_blocks.ptr);
+    p.trace = reinterpret_cast<double2*>(sm.d_trace.ptr);
+    for (int g = 0; g < 5; ++g) p.group_tr[g] = sm.group_tr[g];
     return p;
 }
 
@@ -1172,6 +1208,8 @@ void launch_simplex(hgdg_ctx* c, const SimplexParams& p) {
     for (int g = 0; g < 4; ++g) v |= (sm.tpe_variant[g] & 1) << g;
     const int grid = int(sm.n_blocks);
     if (grid > 0) {
+        // the face traces of every element, then the right-hand side
+        hipLaunchKernelGGL(simplex_trace, dim3(unsigned(p.group_block[4])), dim3(kBlock), 0, c->stream, p);
         const SimplexKernel k = c->op.precision == HGDG_PREC_MIXED_FP32 ? k32[size_t(v)] : k64[size_t(v)];
         hipLaunchKernelGGL(k, dim3(unsigned(grid)), dim3(kBlock), 0, c->stream, p);
     }

@@ -470,4 +470,42 @@ SimplexOps simplex_ops(int p) {
     return ops;
 }
 
+std::vector<double> face_orientation_blocks() {
+    // R_o(m, m') = int chi_m(t) chi_m'(g_o(t)) dt over the reference triangle
+    // (rule of degree 8: exact). g_o is affine and maps the triangle onto
+    // itself with |det| = 1, so f -> f o g_o is orthogonal in L2 and keeps
+    // every P_d: R_o is block diagonal by degree (checked here).
+    static const FaceBasis fb;
+    std::vector<long double> tp, tw;
+    tri_rule(2 * kMaxSimplexOrder, tp, tw);
+    std::vector<double> out;
+    out.reserve(6 * size_t(kFaceRotBlock));
+    for (int o = 0; o < 6; ++o) {
+        long double R[kFaceBasis][kFaceBasis] = {};
+        for (size_t q = 0; q < tw.size(); ++q) {
+            long double to[2], a[kFaceBasis], b[kFaceBasis];
+            tri_orientation_map(o, &tp[2 * q], to);
+            fb.eval(tp[2 * q], tp[2 * q + 1], a);
+            fb.eval(to[0], to[1], b);
+            for (int m = 0; m < kFaceBasis; ++m)
+                for (int n = 0; n < kFaceBasis; ++n) R[m][n] += tw[q] * a[m] * b[n];
+        }
+        auto degree = [](int m) {
+            int d = 0;
+            while ((d + 1) * (d + 2) / 2 <= m) ++d;
+            return d;
+        };
+        for (int m = 0; m < kFaceBasis; ++m)
+            for (int n = 0; n < kFaceBasis; ++n)
+                if (degree(m) != degree(n) && fabsl(R[m][n]) > 1e-15L)
+                    throw std::logic_error("face orientation map mixes polynomial degrees");
+        for (int d = 0; d <= kMaxSimplexOrder; ++d) {
+            const int m0 = d * (d + 1) / 2;
+            for (int a = 0; a <= d; ++a)
+                for (int b = 0; b <= d; ++b) out.push_back(double(R[m0 + a][m0 + b]));
+        }
+    }
+    return out;
+}
+
 }  // namespace hgdg

@@ -51,6 +51,13 @@ struct SimplexOps {
 // Built in long double with exact collapsed Gauss rules.
 SimplexOps simplex_ops(int p);
 
+// Orientation maps acting on face-basis coefficients: for each orientation o
+// the degree blocks (sizes 1, 2, .., 5; degree d at d(d+1)(2d+1)/6) of
+// R_o(m, m') = int chi_m chi_m' o g_o, [6][kFaceRotBlock] row-major per block.
+// A face polynomial h with coefficients c has h o g_o = sum_m (R_o c)_m chi_m.
+constexpr int kFaceRotBlock = 55;
+std::vector<double> face_orientation_blocks();
+
 // hpGEM's Ainsworth-Coyle H(curl) DG set on the reference tetrahedron
 // (createDGBasisFunctionSet3DAinsworthCoyle, BasisFunctions3DAinsworthCoyle
 // .cpp:417-505) for orders 1 and 2: per degree k = 0..p the six edge
