@@ -308,7 +308,7 @@ def run_other_config(name, steps, warmup):
         make = lambda prec: HpgemAPIDevice(mesh, orders, precision=prec, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv,
                                            tableau=RK4, n_extra_vectors=2)
         init = lambda api: api.setInitialSolutionGaussian(centre, 0.05, 1.0, field=0)
-        kernel = "simplex_acoustic_rhs<{}>"
+        kernel = "simplex_trace + simplex_acoustic_rhs<{}>"
     out, finals = {}, {}
     ref = None
     for prec in ("fp64", "mixed"):

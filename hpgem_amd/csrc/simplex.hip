@@ -92,13 +92,16 @@ struct SimplexParams {
 __host__ __device__ constexpr int simplex_nb(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
 __host__ __device__ constexpr int simplex_ncol(int p) { return (p + 1) * (p + 2) / 2; }
 // Threads per element of the RHS kernel, two candidates per order (a row
-// phase needs at least nb threads): variant 0 = 16 at order 1 (four elements
-// per wavefront), 32 at orders 2 and 3 (two), nb at order 4; variant 1 = 32
-// at order 1, 64 (one element per wavefront) above. The library times both per order when the operator is set and
-// keeps the faster; results do not depend on the choice (every row and item is
-// computed by the same instructions whichever thread takes it).
-__host__ __device__ constexpr int simplex_tpe(int p) { return simplex_nb(p) > 32 ? simplex_nb(p) : p == 1 ? 16 : 32; }
-__host__ __device__ constexpr int simplex_tpe_alt(int p) { return p == 1 ? 32 : 64; }
+// phase needs at least nb threads): variant 0 = 8 at order 1 (32 elements per
+// block), 12 at order 2 (21), 32 at order 3 (8), nb = 35 at order 4 (7);
+// variant 1 = 16 at order 1, 32 at order 2, 64 (one element per wavefront)
+// above. Small orders gain from many elements per block (fewer blocks, each
+// with its barriers and table staging). The library times both per order
+// when the operator is set and keeps the faster; results do not depend on
+// the choice (every row and item is computed by the same instructions
+// whichever thread takes it).
+__host__ __device__ constexpr int simplex_tpe(int p) { return p == 1 ? 8 : p == 2 ? 12 : p == 3 ? 32 : simplex_nb(p); }
+__host__ __device__ constexpr int simplex_tpe_alt(int p) { return p == 1 ? 16 : p == 2 ? 32 : 64; }
 __host__ __device__ constexpr int simplex_tpe_of(int p, int variant) { return variant ? simplex_tpe_alt(p) : simplex_tpe(p); }
 
 template <int P, int TPE_ = simplex_tpe(P), int TB = 8>
@@ -188,7 +191,7 @@ __device__ __forceinline__ void simplex_trace_block(const SimplexParams& p, int6
     double* fm = lds + S::FM + le * 12;
     double* su = lds + S::U + le * 4 * NB;
     if (elem_ok) {
-        if (i < 12) fm[i] = p.normal[((i % 3) * 4 + i / 3) * E + e];  // fm[3 f + c]
+        for (int k = i; k < 12; k += S::TPE) fm[k] = p.normal[((k % 3) * 4 + k / 3) * E + e];  // fm[3 f + c]
         if (i < NB) {
             const int64_t base = p.group_dof[P - 1] + (e - p.group_start[P - 1]) * 4 * NB;
             const int64_t at[4] = {base + i, base + NB + i, base + 2 * NB + i, base + 3 * NB + i};
@@ -300,9 +303,9 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
     double cinv = 1.0;
     if (elem_ok) {
         cinv = p.cinv[e];
-        if (i < 16) {
-            const int f = i / 4, c = i % 4;
-            fm[i] = c < 3 ? p.normal[(c * 4 + f) * E + e] : p.sigma[f * E + e];
+        for (int k = i; k < 16; k += S::TPE) {
+            const int f = k / 4, c = k % 4;
+            fm[k] = c < 3 ? p.normal[(c * 4 + f) * E + e] : p.sigma[f * E + e];
         }
     }
     if (row_ok) {
@@ -626,9 +629,10 @@ __device__ void simplex_diff_block(const SimplexDiffParams& p, int blk, double* 
 }
 
 __global__ __launch_bounds__(kBlock) void simplex_diff_norm(const SimplexDiffParams p) {
-    __shared__ double sd[SimplexShape<4>::EPB * 4 * SimplexShape<4>::NB > SimplexShape<2>::EPB * 4 * SimplexShape<2>::NB
-                             ? SimplexShape<4>::EPB * 4 * SimplexShape<4>::NB
-                             : SimplexShape<2>::EPB * 4 * SimplexShape<2>::NB];
+    constexpr int n1 = SimplexShape<1>::EPB * 4 * SimplexShape<1>::NB, n2 = SimplexShape<2>::EPB * 4 * SimplexShape<2>::NB;
+    constexpr int n3 = SimplexShape<3>::EPB * 4 * SimplexShape<3>::NB, n4 = SimplexShape<4>::EPB * 4 * SimplexShape<4>::NB;
+    constexpr int n12 = n1 > n2 ? n1 : n2, n34 = n3 > n4 ? n3 : n4;
+    __shared__ double sd[n12 > n34 ? n12 : n34];  // [EPB][4][NB] of the largest order group
     __shared__ double red[2 * kBlock];
     const int b = blockIdx.x;
     if (b < p.group_block[1]) simplex_diff_block<1>(p, b - p.group_block[0], sd, red);

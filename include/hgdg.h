@@ -188,8 +188,8 @@ int hgdg_get_tensor_general(hgdg_ctx* ctx, int* force, int* in_use);
 /* Threads per element of the tetrahedral RHS kernel, per order 1..4
  * (threads[p-1]): 0 (the default) lets hgdg_operator_set time both candidates
  * of that order on the mesh and keep the faster; otherwise one of the two
- * candidates, 16 or 32 (order 1), 32 or 64 (orders 2 and 3), 35 or 64
- * (order 4). Results are bitwise the same for every choice. No
+ * candidates, 8 or 16 (order 1), 12 or 32 (order 2), 32 or 64 (order 3),
+ * 35 or 64 (order 4). Results are bitwise the same for every choice. No
  * counterpart in the reference (a launch parameter of this implementation);
  * applies to the next hgdg_operator_set, or at once if an operator is set. */
 int hgdg_set_simplex_threads(hgdg_ctx* ctx, const int32_t* threads);

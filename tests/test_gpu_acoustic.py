@@ -216,7 +216,7 @@ def test_tet_threads_per_element_give_bitwise_identical_rhs(precision):
     cinv = 1.0 / rng.uniform(0.5, 2.0, size=mesh.n_elements) ** 2
     dev = HpgemAPIDevice(mesh, p, precision=precision, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=RK4,
                          n_extra_vectors=2)
-    cand = np.array([[16, 32], [32, 64], [32, 64], [35, 64]], dtype=np.int32)
+    cand = np.array([[8, 16], [12, 32], [32, 64], [35, 64]], dtype=np.int32)
     threads, ms = dev.getSimplexThreads()
     print(f"tuned threads {threads}, ms {ms.tolist()}")
     for g in range(4):
@@ -249,9 +249,9 @@ def test_tet_threads_per_element_give_bitwise_identical_rhs(precision):
 def test_tet_alternative_threads_match_oracle():
     """The larger threads-per-element candidate of every order against the
     oracle (the default candidates are covered by the other parity tests)."""
-    print("[case] tet acoustic n=[3, 2, 2] p=random threads=[32, 64, 64, 64]", flush=True)
+    print("[case] tet acoustic n=[3, 2, 2] p=random threads=[16, 32, 64, 64]", flush=True)
     dev, (ref, ref_ld) = make([3, 2, 2], "random", 41)
-    dev.setSimplexThreads([32, 64, 64, 64])
+    dev.setSimplexThreads([16, 32, 64, 64])
     rng = np.random.default_rng(42)
     u = rng.standard_normal(dev.n_dof)
     k = rng.standard_normal(dev.n_dof)
@@ -260,7 +260,7 @@ def test_tet_alternative_threads_match_oracle():
     dev.computeTimeDerivative([0, 1], [1.0, -0.2], 2)
     assert_parity(dev.getTimeIntegrationVector(2), ref.time_derivative([u, k], [1.0, -0.2]),
                   ref_ld.time_derivative([u, k], [1.0, -0.2]), int(ref.orders.max()), ref,
-                  "tet acoustic n=[3, 2, 2] threads=[32, 64, 64, 64]")
+                  "tet acoustic n=[3, 2, 2] threads=[16, 32, 64, 64]")
     dev.close()
 
 
