@@ -121,19 +121,21 @@ struct SimplexShape {
     static constexpr int SF = FM + EPB * 16;                // [EPB][4 faces][NCOL] (T pairs): flux coefficients
     static constexpr int W = SF + tw(EPB * 8 * NCOL);
     static constexpr int G = W;                             // [EPB][4][NB] (T): v, then g_r = sum_d (J^{-1})_{r,d} s_d
-    // DrT staged in LDS below order 4, and at order 4 in FP32 (in FP64 its
-    // 29 KB pass through LDS one direction at a time)
-    static constexpr bool DR_LDS = P < 4 || TB == 4;
+    // DrT: read through the caches by the matrix-core tiles (FP64, orders 3
+    // and 4); otherwise staged in LDS below order 4, and at order 4 in FP32
+    static constexpr bool DR_GLOBAL = P >= 3 && TB == 8;
+    static constexpr bool DR_LDS = !DR_GLOBAL && (P < 4 || TB == 4);
     static constexpr int DR = G + tw(EPB * 4 * NB);         // [3][NB][NB] (T): DrT, or [NB][NB] one direction
-    static constexpr int VOL_END = DR + (DR_LDS ? tw(3 * NB * NB) : tw(NB * NB));
+    static constexpr int VOL_END = DR + (DR_GLOBAL ? 0 : DR_LDS ? tw(3 * NB * NB) : tw(NB * NB));
     static constexpr bool LIFT_LDS = P < 4;                 // lift table staged in LDS for the whole kernel
-    // order 4 in FP64: one face's lift table at a time in the work area
-    // once the volume terms are done (in FP32 it is read through the caches,
-    // which measured faster)
+    // order 4 in FP64: the lift tables pass through the work area once the
+    // volume terms are done, LIFT_CHUNK faces at a time (in FP32 they are
+    // read through the caches, which measured faster)
     static constexpr bool LIFT_ONE = !LIFT_LDS && TB == 8;
+    static constexpr int LIFT_CHUNK = 2;                    // faces per pass through the work area
     static constexpr int LIFT = VOL_END;                    // [4 f][NCOL m][NB i] (T), the whole kernel
-    static constexpr int TOTAL = LIFT + (LIFT_LDS ? tw(4 * NCOL * NB) : 0);
-    static_assert(!LIFT_ONE || tw(NCOL * NB) <= VOL_END - W, "a face's lift table fits the work area");
+    static constexpr int W_END = LIFT_ONE && W + tw(LIFT_CHUNK * NCOL * NB) > VOL_END ? W + tw(LIFT_CHUNK * NCOL * NB) : VOL_END;
+    static constexpr int TOTAL = LIFT_LDS ? LIFT + tw(4 * NCOL * NB) : W_END;
     static_assert(SF % 2 == 0 && W % 2 == 0 && DR % 2 == 0 && LIFT % 2 == 0, "double2 alignment");
 };
 
@@ -356,7 +358,7 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
         }
     }
     const T* gDr = tabT + p.tix[(P - 1) * kTixStride + kTixDrT];
-    {
+    if constexpr (!S::DR_GLOBAL) {
         constexpr int nd = S::DR_LDS ? 3 * NB * NB : NB * NB;  // all directions, or the first
 #pragma unroll 4
         for (int k = tid; k < nd; k += kBlock) sDr[k] = gDr[k];
@@ -391,7 +393,7 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
         V4 acc[3];
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-            if constexpr (!S::DR_LDS) {
+            if constexpr (!S::DR_LDS && !S::DR_GLOBAL) {
                 if (r > 0) {
                     __syncthreads();
                     for (int k = tid; k < NB * NB; k += kBlock) sDr[k] = gDr[r * NB * NB + k];
@@ -400,7 +402,7 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
             }
             acc[r] = V4{T(0), T(0), T(0), T(0)};
             if (wave < NTM) {
-                const T* Dr = sDr + (S::DR_LDS ? r * NB * NB : 0);
+                const T* Dr = S::DR_GLOBAL ? gDr + r * NB * NB : sDr + (S::DR_LDS ? r * NB * NB : 0);
 #pragma unroll
                 for (int s = 0; s < KS; ++s) {
                     const int k = 4 * s + kk, i = 16 * wave + col;
@@ -487,11 +489,12 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
 
     // ---- phase 3: k += sigma_f sum_m (M_ref^{-1} int_f phi chi_m)_i F_m
     // (order 4 in FP64: the lift table passes through the work area, dead
-    // once the volume terms are done, one face at a time)
+    // once the volume terms are done, LIFT_CHUNK faces at a time)
     T* sL1 = reinterpret_cast<T*>(lds + S::W);
+    constexpr int LC = S::LIFT_CHUNK;
     if constexpr (S::LIFT_ONE) {
-        __syncthreads();  // the volume data give way to face 0's lift table
-        for (int k = tid; k < NC * NB; k += kBlock) sL1[k] = gLift[k];
+        __syncthreads();  // the volume data give way to the first faces' lift tables
+        for (int k = tid; k < LC * NC * NB; k += kBlock) sL1[k] = gLift[k];
         __syncthreads();
     }
     const T ci = T(1.0 / cinv);
@@ -503,12 +506,12 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
         } else if constexpr (!S::LIFT_ONE) {
             lift = gLift + f * NC * NB + i;
         } else {
-            if (f > 0) {
-                __syncthreads();  // the block is done with face f - 1
-                for (int k = tid; k < NC * NB; k += kBlock) sL1[k] = gLift[f * NC * NB + k];
+            if (f > 0 && f % LC == 0) {
+                __syncthreads();  // the block is done with the previous faces
+                for (int k = tid; k < LC * NC * NB; k += kBlock) sL1[k] = gLift[f * NC * NB + k];
                 __syncthreads();
             }
-            lift = sL1 + i;
+            lift = sL1 + (f % LC) * NC * NB + i;
         }
         if (row_ok) {
             const FluxPair<T>* fl = sf + f * NC;
