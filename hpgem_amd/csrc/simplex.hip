@@ -324,14 +324,32 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
     if (elem_ok) {
         const double2* own = p.trace + p.group_tr[P - 1] + (e - p.group_start[P - 1]) * 4 * NC;
         const double* rot = p.tab + p.tix[kTixRot];
-#pragma unroll 1
-        for (int it = i; it < 4 * NC; it += S::TPE) {
-            const int f = it / NC, m = it % NC;
-            const int nbr = p.nbr[f * E + e];
-            const double2 a = own[it];
+        // every item's own trace and face data first, then the neighbours'
+        // coefficients (their addresses need the face data)
+        constexpr int NIT = (4 * NC + S::TPE - 1) / S::TPE;  // items per thread
+        int nb_[NIT], inf_[NIT];
+        double2 a_[NIT];
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+            const int it = i + k * S::TPE;
+            nb_[k] = -1; inf_[k] = 0; a_[k] = make_double2(0, 0);
+            if (it < 4 * NC) {
+                const int f = it / NC;
+                nb_[k] = p.nbr[f * E + e];
+                inf_[k] = p.info[f * E + e];
+                a_[k] = own[it];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+            const int it = i + k * S::TPE;
+            if (it >= 4 * NC) continue;
+            const int m = it % NC;
+            const int nbr = nb_[k];
+            const double2 a = a_[k];
             double Fv, Fs;
             if (nbr >= 0) {
-                const int inf = p.info[f * E + e];
+                const int inf = inf_[k];
                 const int pn = info_pn(inf);
                 const int d = face_degree(m);
                 double bns = 0, bv = 0;
