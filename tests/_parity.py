@@ -25,8 +25,8 @@ reference, over all coefficients.
   operator up to p = 4, so tetrahedra take the plain bar at every order.
 
 Every call prints the three distances and appends them to DISTANCES, which
-the suites' session fixture writes to gpurun_out/parity_distances.json when
-that folder exists.
+the suites' session fixture writes to $HGDG_PARITY_DIR/parity_distances.json
+when that variable names an existing folder.
 """
 import numpy as np
 

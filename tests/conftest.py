@@ -14,10 +14,11 @@ def pytest_configure(config):
 
 
 def pytest_sessionfinish(session, exitstatus):
-    """Keep the parity distances the GPU suites measured (tests/_parity.py)."""
+    """Keep the parity distances the GPU suites measured (tests/_parity.py)
+    in $HGDG_PARITY_DIR/parity_distances.json when that folder is given."""
     mod = sys.modules.get("_parity")
-    out = os.path.join(ROOT, "gpurun_out")
-    if mod is None or not mod.DISTANCES or not os.path.isdir(out):
+    out = os.environ.get("HGDG_PARITY_DIR", "")
+    if mod is None or not mod.DISTANCES or not out or not os.path.isdir(out):
         return
     import json
 
