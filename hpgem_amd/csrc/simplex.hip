@@ -44,7 +44,8 @@ constexpr int kTixCols = 2;          // + f
 constexpr int kTixProj = 6;          // [4 f][6 o][ncol][15 m]: chi_m-coefficients of the face columns' traces
 constexpr int kTixLift = 7;          // [4 f][ncol m][nb i]: (M_ref^{-1} int_f phi chi_m)_i, own face coordinates
 constexpr int kTixProjId = 8;        // [4 f][ncol c][ncol m]: the trace table at the identity orientation, m < ncol
-constexpr int kTixStride = 9;
+constexpr int kTixDrP = 9;           // orders >= 3: [3][4 ceil(nb/4) k][16 ceil(nb/16) i] DrT padded with zeros
+constexpr int kTixStride = 10;
 constexpr int kTixRel = 4 * kTixStride;  // [6 own][6 neighbour]: orientation of the neighbour's face seen from ours
 constexpr int kTixIdent = kTixRel + 36;  // the identity orientation
 constexpr int kTixRot = kTixIdent + 1;   // [6 o][55]: the orientation maps on face-basis coefficients, by degree block
@@ -122,11 +123,10 @@ struct SimplexShape {
     static constexpr int W = SF + tw(EPB * 8 * NCOL);
     static constexpr int G = W;                             // [EPB][4][NB] (T): v, then g_r = sum_d (J^{-1})_{r,d} s_d
     // DrT: read through the caches by the matrix-core tiles (FP64, orders 3
-    // and 4); otherwise staged in LDS below order 4, and at order 4 in FP32
+    // and 4, from a zero-padded copy of the table); otherwise staged in LDS
     static constexpr bool DR_GLOBAL = P >= 3 && TB == 8;
-    static constexpr bool DR_LDS = !DR_GLOBAL && (P < 4 || TB == 4);
-    static constexpr int DR = G + tw(EPB * 4 * NB);         // [3][NB][NB] (T): DrT, or [NB][NB] one direction
-    static constexpr int VOL_END = DR + (DR_GLOBAL ? 0 : DR_LDS ? tw(3 * NB * NB) : tw(NB * NB));
+    static constexpr int DR = G + tw(EPB * 4 * NB);         // [3][NB][NB] (T): DrT
+    static constexpr int VOL_END = DR + (DR_GLOBAL ? 0 : tw(3 * NB * NB));
     static constexpr bool LIFT_LDS = P < 4;                 // lift table staged in LDS for the whole kernel
     // order 4 in FP64: the lift tables pass through the work area once the
     // volume terms are done, LIFT_CHUNK faces at a time (in FP32 they are
@@ -377,9 +377,8 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
     }
     const T* gDr = tabT + p.tix[(P - 1) * kTixStride + kTixDrT];
     if constexpr (!S::DR_GLOBAL) {
-        constexpr int nd = S::DR_LDS ? 3 * NB * NB : NB * NB;  // all directions, or the first
 #pragma unroll 4
-        for (int k = tid; k < nd; k += kBlock) sDr[k] = gDr[k];
+        for (int k = tid; k < 3 * NB * NB; k += kBlock) sDr[k] = gDr[k];
     }
     const T* gLift = tabT + p.tix[(P - 1) * kTixStride + kTixLift];
     T* sLift = reinterpret_cast<T*>(lds + S::LIFT);
@@ -402,7 +401,9 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
     Acc<T> kv, ks[3];
     if constexpr (P >= 3 && std::is_same<T, double>::value) {
         static_assert(S::EPB <= 8, "16 columns");
+        static_assert(S::DR_GLOBAL, "the tiles read the padded table through the caches");
         constexpr int NTM = (NB + 15) / 16, KS = (NB + 3) / 4;
+        constexpr int NBP = 16 * NTM, KP = 4 * KS;  // the padded table: [3][KP][NBP]
         using V4 = mfma_d4;
         const int lane = tid % 64, wave = tid / 64;
         const int col = lane % 16, kk = lane / 16;
@@ -410,23 +411,23 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
         T* sG0 = reinterpret_cast<T*>(lds + S::G);
         V4 acc[3];
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            if constexpr (!S::DR_LDS && !S::DR_GLOBAL) {
-                if (r > 0) {
-                    __syncthreads();
-                    for (int k = tid; k < NB * NB; k += kBlock) sDr[k] = gDr[r * NB * NB + k];
-                    __syncthreads();
-                }
-            }
-            acc[r] = V4{T(0), T(0), T(0), T(0)};
-            if (wave < NTM) {
-                const T* Dr = S::DR_GLOBAL ? gDr + r * NB * NB : sDr + (S::DR_LDS ? r * NB * NB : 0);
+        for (int r = 0; r < 3; ++r) acc[r] = V4{T(0), T(0), T(0), T(0)};
+        if (wave < NTM) {
+            // per direction the A operands first (zeros outside nb x nb come
+            // from the table, so every load is unconditional), then the
+            // accumulation chain
+            const T* Dr = reinterpret_cast<const T*>(p.tab) + p.tix[(P - 1) * kTixStride + kTixDrP] + kk * NBP +
+                          16 * wave + col;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                T av[KS];
+#pragma unroll
+                for (int s = 0; s < KS; ++s) av[s] = Dr[(r * KP + 4 * s) * NBP];
 #pragma unroll
                 for (int s = 0; s < KS; ++s) {
-                    const int k = 4 * s + kk, i = 16 * wave + col;
-                    const T av = (i < NB && k < NB) ? Dr[k * NB + i] : T(0);
+                    const int k = 4 * s + kk;
                     const T bv = (ce < count && k < NB) ? sG0[ce * 4 * NB + (cv ? 0 : 1 + r) * NB + k] : T(0);
-                    acc[r] = mfma_step(av, bv, acc[r]);
+                    acc[r] = mfma_step(av[s], bv, acc[r]);
                 }
             }
         }
@@ -459,37 +460,15 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
         }
     } else {
         T a[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
-        if constexpr (S::DR_LDS) {
-            if (row_ok) {
+        if (row_ok) {
 #pragma unroll 4
-                for (int j = 0; j < NB; ++j) {
-                    const T v = sg[j];
+            for (int j = 0; j < NB; ++j) {
+                const T v = sg[j];
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                        const T d = sDr[(r * NB + j) * NB + i];
-                        a[r] += d * sg[(1 + r) * NB + j];
-                        b[r] += d * v;
-                    }
-                }
-            }
-        } else {
-            // one direction at a time through LDS (every thread of the block
-            // takes part in the barriers and loads); the sums over j run in the
-            // same order as above
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                if (r > 0) {
-                    __syncthreads();  // the block is done with direction r - 1
-                    for (int k = tid; k < NB * NB; k += kBlock) sDr[k] = gDr[r * NB * NB + k];
-                    __syncthreads();
-                }
-                if (row_ok) {
-#pragma unroll 5
-                    for (int j = 0; j < NB; ++j) {
-                        const T d = sDr[j * NB + i];
-                        a[r] += d * sg[(1 + r) * NB + j];
-                        b[r] += d * sg[j];
-                    }
+                for (int r = 0; r < 3; ++r) {
+                    const T d = sDr[(r * NB + j) * NB + i];
+                    a[r] += d * sg[(1 + r) * NB + j];
+                    b[r] += d * v;
                 }
             }
         }
@@ -1087,6 +1066,14 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
             require(int(ops.cols[f].size()) == nc, HGDG_ERR_ARG, "unexpected face column count");
             t[kTixCols + f] = int32_t(cols.size());  // relative, fixed below
             cols.insert(cols.end(), ops.cols[f].begin(), ops.cols[f].end());
+        }
+        t[kTixDrP] = int32_t(tab.size());  // [r][k < 4 ceil(nb/4)][i < 16 ceil(nb/16)], zeros outside nb x nb
+        if (q >= 3) {
+            const int kp = 4 * ((nb + 3) / 4), ip = 16 * ((nb + 15) / 16);
+            for (int r = 0; r < 3; ++r)
+                for (int k = 0; k < kp; ++k)
+                    for (int i = 0; i < ip; ++i)
+                        tab.push_back(k < nb && i < nb ? ops.Dr[(size_t(r) * nb + i) * nb + k] : 0.0);
         }
         t[kTixProjId] = int32_t(tab.size());  // [f][c < nc][m < nc]
         for (int f = 0; f < 4; ++f)
