@@ -359,12 +359,20 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
                     const double* R = rot + o * 55 + d * (d + 1) * (2 * d + 1) / 6 + (m - m0) * (d + 1);
                     const double2* x = p.trace + p.group_tr[pn - 1] + (nbr - p.group_start[pn - 1]) * 4 * ncn +
                                        info_lf(inf) * ncn + m0;
-                    for (int b = 0; b <= d; ++b) {
-                        const double r = R[b];
-                        const double2 v = x[b];
-                        bns += r * v.x;
-                        bv += r * v.y;
-                    }
+                    double rr[5];
+                    double2 xv[5];
+#pragma unroll
+                    for (int b = 0; b < 5; ++b)
+                        if (b <= d) {
+                            rr[b] = R[b];
+                            xv[b] = x[b];
+                        }
+#pragma unroll
+                    for (int b = 0; b < 5; ++b)
+                        if (b <= d) {
+                            bns += rr[b] * xv[b].x;
+                            bv += rr[b] * xv[b].y;
+                        }
                 }
                 Fv = -0.5 * (a.x + bns);  // n.(s_own - s_nbr), the neighbour's traces with n_nbr = -n
                 Fs = -0.5 * (a.y - bv);
