@@ -122,16 +122,15 @@ struct SimplexShape {
     static constexpr int SF = FM + EPB * 16;                // [EPB][4 faces][NCOL] (T pairs): flux coefficients
     static constexpr int W = SF + tw(EPB * 8 * NCOL);
     static constexpr int G = W;                             // [EPB][4][NB] (T): v, then g_r = sum_d (J^{-1})_{r,d} s_d
-    // DrT: read through the caches by the matrix-core tiles (FP64, orders 3
-    // and 4, from a zero-padded copy of the table); otherwise staged in LDS
-    static constexpr bool DR_GLOBAL = P >= 3 && TB == 8;
+    // DrT: read through the caches by the matrix-core tiles (orders 3 and 4,
+    // from a zero-padded copy of the table); otherwise staged in LDS
+    static constexpr bool DR_GLOBAL = P >= 3;
     static constexpr int DR = G + tw(EPB * 4 * NB);         // [3][NB][NB] (T): DrT
     static constexpr int VOL_END = DR + (DR_GLOBAL ? 0 : tw(3 * NB * NB));
     static constexpr bool LIFT_LDS = P < 4;                 // lift table staged in LDS for the whole kernel
-    // order 4 in FP64: the lift tables pass through the work area once the
-    // volume terms are done, LIFT_CHUNK faces at a time (in FP32 they are
-    // read through the caches, which measured faster)
-    static constexpr bool LIFT_ONE = !LIFT_LDS && TB == 8;
+    // order 4: the lift tables pass through the work area once the volume
+    // terms are done, LIFT_CHUNK faces at a time
+    static constexpr bool LIFT_ONE = !LIFT_LDS;
     static constexpr int LIFT_CHUNK = 2;                    // faces per pass through the work area
     static constexpr int LIFT = VOL_END;                    // [4 f][NCOL m][NB i] (T), the whole kernel
     static constexpr int W_END = LIFT_ONE && W + tw(LIFT_CHUNK * NCOL * NB) > VOL_END ? W + tw(LIFT_CHUNK * NCOL * NB) : VOL_END;
@@ -146,6 +145,10 @@ __host__ __device__ constexpr int face_degree(int m) { return (m >= 1) + (m >= 3
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ mfma_d4 mfma_step(double a, double b, mfma_d4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ mfma_f4 mfma_step(float a, float b, mfma_f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
 template <typename T>
@@ -397,22 +400,24 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
     __syncthreads();
 
     // ---- phase 2: volume, k_v = sum_r Dr_r g_r ; k_s_d = (1/c_inv) sum_r (J^{-1})_{r,d} Dr_r v
-    // FP64 at orders 3 and 4 on the matrix cores: per direction r, Dr_r times
-    // the block's columns [g_r of elements 0..7 | v of elements 0..7] in
-    // 16 x 16 x 4 tiles (v_mfma_f64_16x16x4f64: A(i, k) from lane i + 16 k,
-    // B(k, j) from lane j + 16 k, D(4 q + lane / 16, lane % 16) in output q),
-    // one row tile per wavefront. Each output accumulates its products in k
-    // order, one fused multiply-add after another, as the row loop below does:
-    // the results are bitwise those of the VALU form (checked on the 24^3 x 5
-    // workload). In FP32 the matrix form measured no faster than the VALU
-    // form, which the mixed path keeps.
+    // Orders 3 and 4 on the matrix cores: per direction r, Dr_r times the
+    // block's columns [g_r of elements 0..7 | v of elements 0..7] in
+    // 16 x 16 x 4 tiles, one row tile per wavefront (A(i, k) from lane
+    // i + 16 k, B(k, j) from lane j + 16 k; output q of lane l holds
+    // D(4 q + l / 16, l % 16) for v_mfma_f64_16x16x4f64 and
+    // D(4 (l / 16) + q, l % 16) for v_mfma_f32_16x16x4f32). In FP64 each
+    // output accumulates its products in k order, one fused multiply-add
+    // after another, as the row loop below does: the results are bitwise
+    // those of the VALU form (checked on the 24^3 x 5 workload); in FP32 the
+    // 100-step RK4 run of that workload gives the row loop's error against
+    // FP64 to every digit. Orders 1 and 2: the row loop.
     Acc<T> kv, ks[3];
-    if constexpr (P >= 3 && std::is_same<T, double>::value) {
+    if constexpr (P >= 3) {
         static_assert(S::EPB <= 8, "16 columns");
         static_assert(S::DR_GLOBAL, "the tiles read the padded table through the caches");
         constexpr int NTM = (NB + 15) / 16, KS = (NB + 3) / 4;
         constexpr int NBP = 16 * NTM, KP = 4 * KS;  // the padded table: [3][KP][NBP]
-        using V4 = mfma_d4;
+        using V4 = typename std::conditional<std::is_same<T, double>::value, mfma_d4, mfma_f4>::type;
         const int lane = tid % 64, wave = tid / 64;
         const int col = lane % 16, kk = lane / 16;
         const int ce = col % 8, cv = col / 8;
@@ -424,7 +429,7 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
             // per direction the A operands first (zeros outside nb x nb come
             // from the table, so every load is unconditional), then the
             // accumulation chain
-            const T* Dr = reinterpret_cast<const T*>(p.tab) + p.tix[(P - 1) * kTixStride + kTixDrP] + kk * NBP +
+            const T* Dr = tabT + p.tix[(P - 1) * kTixStride + kTixDrP] + kk * NBP +
                           16 * wave + col;
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
@@ -443,7 +448,7 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
         if (wave < NTM) {
 #pragma unroll
             for (int ri = 0; ri < 4; ++ri) {
-                const int row = 16 * wave + 4 * ri + lane / 16;
+                const int row = 16 * wave + (std::is_same<T, double>::value ? 4 * ri + lane / 16 : 4 * (lane / 16) + ri);
                 if (row < NB && ce < count) {
                     T* o = sG0 + ce * 4 * NB;
                     if (cv == 0) o[row] = acc[0][ri] + acc[1][ri] + acc[2][ri];
@@ -493,8 +498,8 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
     }
 
     // ---- phase 3: k += sigma_f sum_m (M_ref^{-1} int_f phi chi_m)_i F_m
-    // (order 4 in FP64: the lift table passes through the work area, dead
-    // once the volume terms are done, LIFT_CHUNK faces at a time)
+    // (order 4: the lift table passes through the work area, dead once the
+    // volume terms are done, LIFT_CHUNK faces at a time)
     T* sL1 = reinterpret_cast<T*>(lds + S::W);
     constexpr int LC = S::LIFT_CHUNK;
     if constexpr (S::LIFT_ONE) {
