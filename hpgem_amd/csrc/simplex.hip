@@ -57,6 +57,9 @@ __host__ __device__ constexpr int info_lf(int v) { return v & 3; }
 __host__ __device__ constexpr int info_oown(int v) { return (v >> 2) & 7; }
 __host__ __device__ constexpr int info_onbr(int v) { return (v >> 5) & 7; }
 __host__ __device__ constexpr int info_pn(int v) { return (v >> 8) & 7; }
+// the orientation that maps the neighbour's face coordinates onto ours,
+// o_nbr o o_own^{-1} (bits set once the orientation group table is built)
+__host__ __device__ constexpr int info_rel(int v) { return (v >> 11) & 7; }
 constexpr int info_pack(int lf, int oown, int onbr, int pn) { return lf | (oown << 2) | (onbr << 5) | (pn << 8); }
 
 struct RefTetRule {
@@ -171,7 +174,8 @@ struct TraceShape {
     static constexpr int U = FM + EPB * 12;         // [EPB][4][NB] stage input
     static constexpr int SO = U + EPB * 4 * NB;     // [EPB][4 f][NC] (double2): n.s and v at the face columns
     static constexpr int PR = SO + EPB * 8 * NC;    // [4 f][NC c][NC m]: identity-orientation trace table
-    static constexpr int TOTAL = PR + 4 * NC * NC;
+    static constexpr int CL = PR + 4 * NC * NC;     // [4 f][NC] (int32, as NC * 2 doubles): the face column lists
+    static constexpr int TOTAL = CL + 2 * NC;
     static_assert(SO % 2 == 0, "double2 alignment");
 };
 
@@ -211,13 +215,15 @@ __device__ __forceinline__ void simplex_trace_block(const SimplexParams& p, int6
         double* sP = lds + S::PR;
 #pragma unroll 4
         for (int k = tid; k < 4 * NC * NC; k += kBlock) sP[k] = gP[k];
+        int32_t* sc = reinterpret_cast<int32_t*>(lds + S::CL);
+        for (int k = tid; k < 4 * NC; k += kBlock) sc[k] = p.tix[tx[kTixCols + k / NC] + k % NC];
     }
     __syncthreads();
     double2* so = reinterpret_cast<double2*>(lds + S::SO) + le * 4 * NC;
     if (elem_ok) {
         for (int it = i; it < 4 * NC; it += S::TPE) {
-            const int f = it / NC, c = it % NC;
-            const int col = p.tix[tx[kTixCols + f] + c];
+            const int f = it / NC;
+            const int col = reinterpret_cast<const int32_t*>(lds + S::CL)[it];
             so[it] = make_double2(fm[f * 3] * su[NB + col] + fm[f * 3 + 1] * su[2 * NB + col] +
                                       fm[f * 3 + 2] * su[3 * NB + col],
                                   su[col]);
@@ -357,7 +363,7 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
                 const int d = face_degree(m);
                 double bns = 0, bv = 0;
                 if (d <= pn) {
-                    const int o = p.tix[kTixRel + info_oown(inf) * 6 + info_onbr(inf)];
+                    const int o = info_rel(inf);
                     const int m0 = d * (d + 1) / 2, ncn = simplex_ncol(pn);
                     const double* R = rot + o * 55 + d * (d + 1) * (2 * d + 1) / 6 + (m - m0) * (d + 1);
                     const double2* x = p.trace + p.group_tr[pn - 1] + (nbr - p.group_start[pn - 1]) * 4 * ncn +
@@ -1061,6 +1067,9 @@ void simplex_mesh_upload(hgdg_ctx* c, const hgdg_mesh_desc* m) {
             tix[size_t(kTixRel) + oo * 6 + on] = rel;
         }
     }
+    for (size_t k = 0; k < sm.info.size(); ++k)
+        if (sm.nbr_dev[k] >= 0)
+            sm.info[k] |= tix[size_t(kTixRel) + info_oown(sm.info[k]) * 6 + info_onbr(sm.info[k])] << 11;
     sm.ops.clear();
     for (int q = 1; q <= kMaxSimplexOrder; ++q) sm.ops.push_back(simplex_ops(q));
     std::vector<double> tab;
