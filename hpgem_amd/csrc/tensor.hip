@@ -564,8 +564,8 @@ struct AcousticParams {
     const double* Mi;
 };
 
-// Four lanes per pencil, one per field (v, s_1 .. s_3), so a lane carries
-// one field's N1 values. Both equations then
+// Four lanes per pencil, one per field (v, s_1 .. s_3; in 2D the fourth lane
+// idles), so a lane carries one field's N1 values. Both equations then
 // run the same instructions:
 //   volume: acc = sum_r scale_r (Dr along r) src_r, with src_r = g_r =
 //     sum_d (J^{-1})_{r,d} s_d and scale_r = 1 on the v lane, src_r = v and
@@ -573,12 +573,15 @@ struct AcousticParams {
 //   faces: each lane forms its own field's jump; the v lane gets
 //     sum_k w_k [s_k] by a reduction over the pencil's four lanes, the s
 //     lanes get [v] from the v lane, both by lane shuffles within the quad.
+constexpr int kQuad2DMinN1 = 3;  // 2D orders whose pencils take the four-lane layout: N1 >= this
+
 template <int DIM, int N1>
 struct AcousticShape {
     static constexpr int NF = DIM + 1;
     static constexpr int NP = Pencil<DIM, N1>::NP;
     static constexpr int NB = Pencil<DIM, N1>::NB;
-    static constexpr int TPE = DIM == 3 ? 4 * NP : NP;  // threads per element
+    static constexpr bool QUAD = DIM == 3 || N1 >= kQuad2DMinN1;
+    static constexpr int TPE = QUAD ? 4 * NP : NP;  // threads per element
     static constexpr int EPB = kBlock / TPE;
 };
 
@@ -786,8 +789,10 @@ __device__ __forceinline__ void acoustic_block_quad(const AcousticParams& p, int
     }
 }
 
-// In 2D (three fields, little work per pencil) one thread per pencil does
-// all fields; the quad layout pays off from 3D on.
+// In 2D at order 1 (three fields, little work per pencil) one thread per
+// pencil does all fields; from order 2 on the four-lane layout above is
+// faster (the one-thread form needs up to 256 registers there: one wave per
+// SIMD at orders 4 and 5).
 template <int DIM, int N1, typename T>
 __device__ __forceinline__ void acoustic_block_pencil(const AcousticParams& p, int64_t blk) {
     using S = Pencil<DIM, N1>;
@@ -989,7 +994,7 @@ __device__ __forceinline__ void acoustic_block_pencil(const AcousticParams& p, i
 
 template <int DIM, int N1, typename T>
 __device__ __forceinline__ void acoustic_block(const AcousticParams& p, int64_t blk) {
-    if constexpr (DIM == 3) acoustic_block_quad<DIM, N1, T>(p, blk);
+    if constexpr (AcousticShape<DIM, N1>::QUAD) acoustic_block_quad<DIM, N1, T>(p, blk);
     else acoustic_block_pencil<DIM, N1, T>(p, blk);
 }
 
