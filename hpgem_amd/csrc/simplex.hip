@@ -418,6 +418,10 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
     // 100-step RK4 run of that workload gives the row loop's error against
     // FP64 to every digit. Orders 1 and 2: the row loop.
     Acc<T> kv, ks[3];
+    // order 4: the first lift chunk is loaded into registers while the
+    // volume terms finish, and stored to LDS once they are done
+    constexpr int LPRE = S::LIFT_ONE ? (S::LIFT_CHUNK * NC * NB + kBlock - 1) / kBlock : 1;
+    T lpre[LPRE];
     if constexpr (P >= 3) {
         static_assert(S::EPB <= 8, "16 columns");
         static_assert(S::DR_GLOBAL, "the tiles read the padded table through the caches");
@@ -448,6 +452,13 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
                     const T bv = (ce < count && k < NB) ? sG0[ce * 4 * NB + (cv ? 0 : 1 + r) * NB + k] : T(0);
                     acc[r] = mfma_step(av[s], bv, acc[r]);
                 }
+            }
+        }
+        if constexpr (S::LIFT_ONE) {
+#pragma unroll
+            for (int j = 0; j < LPRE; ++j) {
+                const int k = tid + j * kBlock;
+                lpre[j] = k < S::LIFT_CHUNK * NC * NB ? gLift[k] : T(0);
             }
         }
         __syncthreads();  // every wave has read the inputs
@@ -510,7 +521,11 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
     constexpr int LC = S::LIFT_CHUNK;
     if constexpr (S::LIFT_ONE) {
         __syncthreads();  // the volume data give way to the first faces' lift tables
-        for (int k = tid; k < LC * NC * NB; k += kBlock) sL1[k] = gLift[k];
+#pragma unroll
+        for (int j = 0; j < LPRE; ++j) {
+            const int k = tid + j * kBlock;
+            if (k < LC * NC * NB) sL1[k] = lpre[j];
+        }
         __syncthreads();
     }
     const T ci = T(1.0 / cinv);
