@@ -165,10 +165,17 @@ __device__ __forceinline__ double load_in(const StageIO& p, int i, int64_t idx) 
 
 // Stage input sum_i coef_i vec_i, first term scaled then accumulated in
 // order (HpgemAPISimplified::getLinearCombinationOfVectors).
+// The first two inputs (an RK stage's u and k) are loaded before either is
+// used, so their loads are in flight together; more inputs follow one by one.
+// The arithmetic and its order are those of the plain sum.
 __device__ __forceinline__ double combine(const StageIO& p, int64_t idx) {
-    double s = p.coef[0] * load_in(p, 0, idx);
+    const double x0 = load_in(p, 0, idx);
+    double x1 = 0;
+    if (p.n_in > 1) x1 = load_in(p, 1, idx);
+    double s = p.coef[0] * x0;
+    if (p.n_in > 1) s += p.coef[1] * x1;
 #pragma unroll
-    for (int i = 1; i < kMaxIn; ++i)
+    for (int i = 2; i < kMaxIn; ++i)
         if (i < p.n_in) s += p.coef[i] * load_in(p, i, idx);
     return s;
 }
@@ -177,26 +184,35 @@ __device__ __forceinline__ double combine(const StageIO& p, int64_t idx) {
 // vector: the K loads of one input are independent and issue together, so a
 // thread waits once per input instead of once per value.
 template <int K>
+__device__ __forceinline__ void load_n(const StageIO& p, int i, const int64_t (&idx)[K], double (&x)[K]) {
+    if ((p.in_f32 >> i) & 1u) {
+        const float* v = static_cast<const float*>(p.in[i]);
+#pragma unroll
+        for (int k = 0; k < K; ++k) x[k] = double(v[idx[k]]);
+    } else {
+        const double* v = static_cast<const double*>(p.in[i]);
+#pragma unroll
+        for (int k = 0; k < K; ++k) x[k] = v[idx[k]];
+    }
+}
+
+template <int K>
 __device__ __forceinline__ void combine_n(const StageIO& p, const int64_t (&idx)[K], double (&s)[K]) {
+    double x0[K], x1[K];
+    load_n(p, 0, idx, x0);
+    if (p.n_in > 1) load_n(p, 1, idx, x1);
 #pragma unroll
-    for (int i = 0; i < kMaxIn; ++i) {
+    for (int k = 0; k < K; ++k) s[k] = p.coef[0] * x0[k];
+    if (p.n_in > 1) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) s[k] += p.coef[1] * x1[k];
+    }
+#pragma unroll
+    for (int i = 2; i < kMaxIn; ++i) {
         if (i >= p.n_in) break;
-        double x[K];
-        if ((p.in_f32 >> i) & 1u) {
-            const float* v = static_cast<const float*>(p.in[i]);
+        load_n(p, i, idx, x0);
 #pragma unroll
-            for (int k = 0; k < K; ++k) x[k] = double(v[idx[k]]);
-        } else {
-            const double* v = static_cast<const double*>(p.in[i]);
-#pragma unroll
-            for (int k = 0; k < K; ++k) x[k] = v[idx[k]];
-        }
-        const double c = p.coef[i];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (i == 0) s[k] = c * x[k];
-            else s[k] += c * x[k];
-        }
+        for (int k = 0; k < K; ++k) s[k] += p.coef[i] * x0[k];
     }
 }
 

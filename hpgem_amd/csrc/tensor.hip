@@ -102,14 +102,21 @@ __device__ __forceinline__ void load_pencil(const StageIO& p, int i, int64_t bas
 }
 
 // Stage input pencil sum_i coef_i vec_i (getLinearCombinationOfVectors).
+// The first two inputs are loaded before either is used (their loads are in
+// flight together); the arithmetic and its order are those of the plain sum.
 template <int N1>
 __device__ __forceinline__ void combine_pencil(const StageIO& p, int64_t base, double (&w)[N1]) {
-    double x[N1];
+    double x[N1], y[N1];
     load_pencil<N1>(p, 0, base, x);
+    if (p.n_in > 1) load_pencil<N1>(p, 1, base, y);
 #pragma unroll
     for (int a = 0; a < N1; ++a) w[a] = p.coef[0] * x[a];
+    if (p.n_in > 1) {
 #pragma unroll
-    for (int i = 1; i < kMaxIn; ++i) {
+        for (int a = 0; a < N1; ++a) w[a] += p.coef[1] * y[a];
+    }
+#pragma unroll
+    for (int i = 2; i < kMaxIn; ++i) {
         if (i >= p.n_in) break;
         load_pencil<N1>(p, i, base, x);
 #pragma unroll
@@ -117,68 +124,98 @@ __device__ __forceinline__ void combine_pencil(const StageIO& p, int64_t base, d
     }
 }
 
-// Stage input at the traces of all 2 DIM faces of a pencil, input vector by
-// input vector with one precision branch each, so that every load of an
-// input issues before any is used: x faces (f < 2) one value at at[f], y/z
-// faces the pencil [at[f], at[f] + N1). Same rounding as combine().
+// Stage input at the traces of all 2 DIM faces of a pencil: x faces (f < 2)
+// one value at at[f], y/z faces the pencil [at[f], at[f] + N1). PAIR: the
+// first two input vectors' loads all issue before any value is used (twice
+// the registers); otherwise, and for further inputs, one input vector at a
+// time. Same rounding as combine().
 template <int DIM, int N1>
-__device__ __forceinline__ void combine_traces(const StageIO& p, const int64_t (&at)[2 * DIM],
-                                               double (&w)[2 * DIM][N1]) {
+__device__ __forceinline__ void load_traces(const StageIO& p, int i, const int64_t (&at)[2 * DIM],
+                                            double (&x)[2 * DIM][N1]) {
+    if ((p.in_f32 >> i) & 1u) {
+        const float* v = static_cast<const float*>(p.in[i]);
 #pragma unroll
-    for (int i = 0; i < kMaxIn; ++i) {
-        if (i >= p.n_in) break;
-        double x[2 * DIM][N1];
-        if ((p.in_f32 >> i) & 1u) {
-            const float* v = static_cast<const float*>(p.in[i]);
+        for (int f = 0; f < 2; ++f) x[f][0] = v[at[f]];
 #pragma unroll
-            for (int f = 0; f < 2; ++f) x[f][0] = v[at[f]];
+        for (int f = 2; f < 2 * DIM; ++f) {
+            const float* q = v + at[f];
+            if constexpr (N1 % 4 == 0) {
 #pragma unroll
-            for (int f = 2; f < 2 * DIM; ++f) {
-                const float* q = v + at[f];
-                if constexpr (N1 % 4 == 0) {
-#pragma unroll
-                    for (int a = 0; a < N1; a += 4) {
-                        const float4 t = *reinterpret_cast<const float4*>(q + a);
-                        x[f][a] = t.x; x[f][a + 1] = t.y; x[f][a + 2] = t.z; x[f][a + 3] = t.w;
-                    }
-                } else if constexpr (N1 % 2 == 0) {
-#pragma unroll
-                    for (int a = 0; a < N1; a += 2) {
-                        const float2 t = *reinterpret_cast<const float2*>(q + a);
-                        x[f][a] = t.x; x[f][a + 1] = t.y;
-                    }
-                } else {
-#pragma unroll
-                    for (int a = 0; a < N1; ++a) x[f][a] = q[a];
+                for (int a = 0; a < N1; a += 4) {
+                    const float4 t = *reinterpret_cast<const float4*>(q + a);
+                    x[f][a] = t.x; x[f][a + 1] = t.y; x[f][a + 2] = t.z; x[f][a + 3] = t.w;
                 }
-            }
-        } else {
-            const double* v = static_cast<const double*>(p.in[i]);
+            } else if constexpr (N1 % 2 == 0) {
 #pragma unroll
-            for (int f = 0; f < 2; ++f) x[f][0] = v[at[f]];
-#pragma unroll
-            for (int f = 2; f < 2 * DIM; ++f) {
-                const double* q = v + at[f];
-                if constexpr (N1 % 2 == 0) {
-#pragma unroll
-                    for (int a = 0; a < N1; a += 2) {
-                        const double2 t = *reinterpret_cast<const double2*>(q + a);
-                        x[f][a] = t.x; x[f][a + 1] = t.y;
-                    }
-                } else {
-#pragma unroll
-                    for (int a = 0; a < N1; ++a) x[f][a] = q[a];
+                for (int a = 0; a < N1; a += 2) {
+                    const float2 t = *reinterpret_cast<const float2*>(q + a);
+                    x[f][a] = t.x; x[f][a + 1] = t.y;
                 }
+            } else {
+#pragma unroll
+                for (int a = 0; a < N1; ++a) x[f][a] = q[a];
             }
         }
-        const double c = p.coef[i];
+    } else {
+        const double* v = static_cast<const double*>(p.in[i]);
+#pragma unroll
+        for (int f = 0; f < 2; ++f) x[f][0] = v[at[f]];
+#pragma unroll
+        for (int f = 2; f < 2 * DIM; ++f) {
+            const double* q = v + at[f];
+            if constexpr (N1 % 2 == 0) {
+#pragma unroll
+                for (int a = 0; a < N1; a += 2) {
+                    const double2 t = *reinterpret_cast<const double2*>(q + a);
+                    x[f][a] = t.x; x[f][a + 1] = t.y;
+                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < N1; ++a) x[f][a] = q[a];
+            }
+        }
+    }
+}
+
+template <int DIM, int N1, bool PAIR = true>
+__device__ __forceinline__ void combine_traces(const StageIO& p, const int64_t (&at)[2 * DIM],
+                                               double (&w)[2 * DIM][N1]) {
+    double x[2 * DIM][N1];
+    load_traces<DIM, N1>(p, 0, at, x);
+    if constexpr (PAIR) {
+        double y[2 * DIM][N1];
+        if (p.n_in > 1) load_traces<DIM, N1>(p, 1, at, y);
 #pragma unroll
         for (int f = 0; f < 2 * DIM; ++f)
 #pragma unroll
-            for (int a = 0; a < (f < 2 ? 1 : N1); ++a) {
-                if (i == 0) w[f][a] = c * x[f][a];
-                else w[f][a] += c * x[f][a];
-            }
+            for (int a = 0; a < (f < 2 ? 1 : N1); ++a) w[f][a] = p.coef[0] * x[f][a];
+        if (p.n_in > 1) {
+#pragma unroll
+            for (int f = 0; f < 2 * DIM; ++f)
+#pragma unroll
+                for (int a = 0; a < (f < 2 ? 1 : N1); ++a) w[f][a] += p.coef[1] * y[f][a];
+        }
+    } else {
+#pragma unroll
+        for (int f = 0; f < 2 * DIM; ++f)
+#pragma unroll
+            for (int a = 0; a < (f < 2 ? 1 : N1); ++a) w[f][a] = p.coef[0] * x[f][a];
+        if (p.n_in > 1) {
+            load_traces<DIM, N1>(p, 1, at, x);
+#pragma unroll
+            for (int f = 0; f < 2 * DIM; ++f)
+#pragma unroll
+                for (int a = 0; a < (f < 2 ? 1 : N1); ++a) w[f][a] += p.coef[1] * x[f][a];
+        }
+    }
+#pragma unroll
+    for (int i = 2; i < kMaxIn; ++i) {
+        if (i >= p.n_in) break;
+        load_traces<DIM, N1>(p, i, at, x);
+#pragma unroll
+        for (int f = 0; f < 2 * DIM; ++f)
+#pragma unroll
+            for (int a = 0; a < (f < 2 ? 1 : N1); ++a) w[f][a] += p.coef[i] * x[f][a];
     }
 }
 
@@ -707,7 +744,9 @@ __device__ __forceinline__ void acoustic_block_quad(const AcousticParams& p, int
             const bool glob = lane && nbi[f] >= 0 && !(lnb >= 0 && lnb < S::EPB);
             at[f] = glob ? int64_t(nbi[f]) * NF * NB + fld * NB + nbr_pc * N1 + noff : base + fld * NB + pc * N1;
         }
-        if (lane) combine_traces<DIM, N1>(p.io, at, oth);  // (an inactive thread's own pencil may lie past the end)
+        // (at order 4 in FP32 holding two inputs' traces costs an occupancy step)
+        constexpr bool kPair = !(N1 >= 5 && std::is_same<T, float>::value);
+        if (lane) combine_traces<DIM, N1, kPair>(p.io, at, oth);  // (an inactive thread's own pencil may lie past the end)
 #pragma unroll
         for (int f = 0; f < 2 * DIM; ++f) {
             const int df = f / 2, s = f % 2;
