@@ -130,22 +130,42 @@ __device__ __forceinline__ void load_run(const T* src, double (&x)[N]) {
 
 // The fused update of N consecutive coefficients at o (a multiple of N), with
 // `own` the values of this kernel's output there (as stored).
+// The solution and the first kRkPre terms are loaded before any is summed
+// (their loads in flight together), further terms one by one; the sum runs
+// in term order as written.
+constexpr int kRkPre = 4;
+
+template <int N>
+__device__ __forceinline__ void rk_term(const RkUpdate& q, int j, int64_t o, const double (&own)[N], double (&x)[N]) {
+    if (j == q.self) {
+#pragma unroll
+        for (int a = 0; a < N; ++a) x[a] = own[a];
+    } else if ((q.x_f32 >> j) & 1u) {
+        load_run<N>(static_cast<const float*>(q.x[j]) + o, x);
+    } else {
+        load_run<N>(static_cast<const double*>(q.x[j]) + o, x);
+    }
+}
+
 template <int N>
 __device__ __forceinline__ void rk_update_run(const RkUpdate& q, int64_t o, const double (&own)[N]) {
     double y[N];
     load_run<N>(q.u + o, y);
+    double xs[kRkPre][N];
 #pragma unroll
-    for (int j = 0; j < kMaxIn; ++j) {
+    for (int j = 0; j < kRkPre; ++j)
+        if (j < q.n_x) rk_term<N>(q, j, o, own, xs[j]);
+#pragma unroll
+    for (int j = 0; j < kRkPre; ++j)
+        if (j < q.n_x) {
+#pragma unroll
+            for (int a = 0; a < N; ++a) y[a] += q.c[j] * xs[j][a];
+        }
+#pragma unroll
+    for (int j = kRkPre; j < kMaxIn; ++j) {
         if (j >= q.n_x) break;
         double x[N];
-        if (j == q.self) {
-#pragma unroll
-            for (int a = 0; a < N; ++a) x[a] = own[a];
-        } else if ((q.x_f32 >> j) & 1u) {
-            load_run<N>(static_cast<const float*>(q.x[j]) + o, x);
-        } else {
-            load_run<N>(static_cast<const double*>(q.x[j]) + o, x);
-        }
+        rk_term<N>(q, j, o, own, x);
 #pragma unroll
         for (int a = 0; a < N; ++a) y[a] += q.c[j] * x[a];
     }

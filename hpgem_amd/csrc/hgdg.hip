@@ -39,6 +39,30 @@ struct AxpyParams {
     int32_t n_x;
 };
 
+// y[i] + sum_j c_j x_j[i], summed in term order; y and the first four terms
+// are loaded before any is summed (their loads in flight together).
+template <typename A>
+__device__ __forceinline__ double axpy_value(const A& p, int64_t i) {
+    auto term = [&](int j) {
+        return ((p.x_f32 >> j) & 1u) ? double(static_cast<const float*>(p.x[j])[i])
+                                     : static_cast<const double*>(p.x[j])[i];
+    };
+    double y = p.y[i];
+    double x[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (j < p.n_x) x[j] = term(j);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (j < p.n_x) y += p.c[j] * x[j];
+#pragma unroll
+    for (int j = 4; j < kMaxIn; ++j) {
+        if (j >= p.n_x) break;
+        y += p.c[j] * term(j);
+    }
+    return y;
+}
+
 // Batched RK updates of many problems: block b belongs to item s with
 // off[s] <= b < off[s+1]; each item is a multi_axpy.
 __global__ __launch_bounds__(kBlock) void multi_axpy_batched(const AxpyItem* items, const int64_t* off, int n) {
@@ -52,15 +76,7 @@ __global__ __launch_bounds__(kBlock) void multi_axpy_batched(const AxpyItem* ite
     const AxpyItem& p = items[lo];
     const int64_t i = (b - off[lo]) * kBlock + threadIdx.x;
     if (i >= p.n) return;
-    double y = p.y[i];
-#pragma unroll
-    for (int j = 0; j < kMaxIn; ++j) {
-        if (j >= p.n_x) break;
-        const double x = ((p.x_f32 >> j) & 1u) ? double(static_cast<const float*>(p.x[j])[i])
-                                                : static_cast<const double*>(p.x[j])[i];
-        y += p.c[j] * x;
-    }
-    p.y[i] = y;
+    p.y[i] = axpy_value(p, i);
 }
 
 void axpy_batch_launch(hipStream_t stream, const AxpyItem* items_dev, const int64_t* off, int n, int64_t blocks) {
@@ -72,15 +88,7 @@ void axpy_batch_launch(hipStream_t stream, const AxpyItem* items_dev, const int6
 __global__ __launch_bounds__(kBlock) void multi_axpy(const AxpyParams p) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= p.n) return;
-    double y = p.y[i];
-#pragma unroll
-    for (int j = 0; j < kMaxIn; ++j) {
-        if (j >= p.n_x) break;
-        const double x = ((p.x_f32 >> j) & 1u) ? double(static_cast<const float*>(p.x[j])[i])
-                                                : static_cast<const double*>(p.x[j])[i];
-        y += p.c[j] * x;
-    }
-    p.y[i] = y;
+    p.y[i] = axpy_value(p, i);
 }
 
 // dst = src, 16 bytes per thread (plain streaming copy)
