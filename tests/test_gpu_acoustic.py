@@ -126,15 +126,24 @@ def test_tensor_acoustic_rhs_matches_oracle(n, p, periodic):
     dev.close()
 
 
-def test_tensor_acoustic_mixed_within_1e6():
-    mesh = Mesh.structured([6, 6, 6])
+# every instance's mixed path: the 3D kernels at p = 2, 3, 4 (p = 4 in FP32
+# gathers the neighbour traces one input vector at a time) and the 2D
+# four-lane kernel
+MIXED_CASES = [([6, 6, 6], 2), ([6, 6, 6], 3), ([5, 5, 5], 4), ([12, 12], 3)]
+
+
+@pytest.mark.parametrize("n,p", MIXED_CASES)
+def test_tensor_acoustic_mixed_within_1e6(n, p):
+    print(f"[case] tensor acoustic mixed n={n} p={p}", flush=True)
+    mesh = Mesh.structured(n)
     rng = np.random.default_rng(2)
     cinv = rng.uniform(0.25, 4.0, size=mesh.n_elements)
+    modes = [[1, 2, 1], [2, 1, 3]] if len(n) == 3 else [[1, 2], [2, 1]]
     runs = {}
     for prec in ("fp64", "mixed"):
-        dev = HpgemAPIDevice(mesh, 3, precision=prec, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=RK4,
+        dev = HpgemAPIDevice(mesh, p, precision=prec, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=RK4,
                              n_extra_vectors=2)
-        dev.setInitialSolutionModes([0.7, -0.4], [[1, 2, 1], [2, 1, 3]], field=0)
+        dev.setInitialSolutionModes([0.7, -0.4], modes, field=0)
         dev.computeOneTimeStep(0.0, 2e-3, 20)
         runs[prec] = (dev, dev.getTimeIntegrationVector(0))
     dev64, u64 = runs["fp64"]
@@ -142,7 +151,7 @@ def test_tensor_acoustic_mixed_within_1e6():
     l2d, _ = dev64.difference(0, 5)
     dev64.setTimeIntegrationVector(6, np.zeros_like(u64))
     l2, _ = dev64.difference(0, 6)
-    print(f"acoustic mixed vs fp64 rel L2 {l2d / l2:.3e}")
+    print(f"acoustic mixed vs fp64 n={n} p={p} rel L2 {l2d / l2:.3e}")
     assert l2d / l2 < 1e-6
     for d, _ in runs.values():
         d.close()
