@@ -622,7 +622,7 @@ struct AcousticShape {
     static constexpr int EPB = kBlock / TPE;
 };
 
-template <int DIM, int N1, typename T>
+template <int DIM, int N1, typename T, bool UPD>
 __device__ __forceinline__ void acoustic_block_quad(const AcousticParams& p, int64_t blk) {
     using S = AcousticShape<DIM, N1>;
     constexpr int NF = S::NF;
@@ -817,22 +817,32 @@ __device__ __forceinline__ void acoustic_block_quad(const AcousticParams& p, int
     if (!lane) return;
     const T ci = T(1.0 / cinv);
     const int64_t o = base + fld * NB + pc * N1;
+    double kv[N1];  // the values as stored
     if (p.io.out_f32) {
         float* out = static_cast<float*>(p.io.out);
 #pragma unroll
-        for (int a = 0; a < N1; ++a) out[o + a] = float(fld == 0 ? acc[a].value() : ci * acc[a].value());
+        for (int a = 0; a < N1; ++a) {
+            const float v = float(fld == 0 ? acc[a].value() : ci * acc[a].value());
+            out[o + a] = v;
+            kv[a] = double(v);
+        }
     } else {
         double* out = static_cast<double*>(p.io.out);
 #pragma unroll
-        for (int a = 0; a < N1; ++a) out[o + a] = double(fld == 0 ? acc[a].value() : ci * acc[a].value());
+        for (int a = 0; a < N1; ++a) {
+            const double v = double(fld == 0 ? acc[a].value() : ci * acc[a].value());
+            out[o + a] = v;
+            kv[a] = v;
+        }
     }
+    if constexpr (UPD) rk_update_run<N1>(p.io.upd, o, kv);
 }
 
 // In 2D at order 1 (three fields, little work per pencil) one thread per
 // pencil does all fields; from order 2 on the four-lane layout above is
 // faster (the one-thread form needs up to 256 registers there: one wave per
 // SIMD at orders 4 and 5).
-template <int DIM, int N1, typename T>
+template <int DIM, int N1, typename T, bool UPD>
 __device__ __forceinline__ void acoustic_block_pencil(const AcousticParams& p, int64_t blk) {
     using S = Pencil<DIM, N1>;
     static_assert(AcousticShape<DIM, N1>::EPB == S::EPB, "launch geometry");
@@ -1012,34 +1022,52 @@ __device__ __forceinline__ void acoustic_block_pencil(const AcousticParams& p, i
     }
     const int64_t o = base + pc * N1;
     const T ci = T(1.0 / cinv);
+    double ks_out[NF][N1];  // the values as stored
     if (p.io.out_f32) {
         float* out = static_cast<float*>(p.io.out);
 #pragma unroll
         for (int a = 0; a < N1; ++a) {
-            out[o + a] = float(av[a].value());
+            const float v = float(av[a].value());
+            out[o + a] = v;
+            ks_out[0][a] = double(v);
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) out[o + (1 + d) * NB + a] = float(ci * as[d][a].value());
+            for (int d = 0; d < DIM; ++d) {
+                const float vs = float(ci * as[d][a].value());
+                out[o + (1 + d) * NB + a] = vs;
+                ks_out[1 + d][a] = double(vs);
+            }
         }
     } else {
         double* out = static_cast<double*>(p.io.out);
 #pragma unroll
         for (int a = 0; a < N1; ++a) {
-            out[o + a] = double(av[a].value());
+            const double v = double(av[a].value());
+            out[o + a] = v;
+            ks_out[0][a] = v;
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) out[o + (1 + d) * NB + a] = double(ci * as[d][a].value());
+            for (int d = 0; d < DIM; ++d) {
+                const double vs = double(ci * as[d][a].value());
+                out[o + (1 + d) * NB + a] = vs;
+                ks_out[1 + d][a] = vs;
+            }
         }
+    }
+    if constexpr (UPD) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) rk_update_run<N1>(p.io.upd, o + f * NB, ks_out[f]);
     }
 }
 
-template <int DIM, int N1, typename T>
+// UPD: the RK update fused into this (last) stage, StageIO::upd.
+template <int DIM, int N1, typename T, bool UPD = false>
 __device__ __forceinline__ void acoustic_block(const AcousticParams& p, int64_t blk) {
-    if constexpr (AcousticShape<DIM, N1>::QUAD) acoustic_block_quad<DIM, N1, T>(p, blk);
-    else acoustic_block_pencil<DIM, N1, T>(p, blk);
+    if constexpr (AcousticShape<DIM, N1>::QUAD) acoustic_block_quad<DIM, N1, T, UPD>(p, blk);
+    else acoustic_block_pencil<DIM, N1, T, UPD>(p, blk);
 }
 
-template <int DIM, int N1, typename T>
+template <int DIM, int N1, typename T, bool UPD = false>
 __global__ __launch_bounds__(kBlock) void tensor_acoustic_rhs(const AcousticParams p) {
-    acoustic_block<DIM, N1, T>(p, xcd_block(blockIdx.x, gridDim.x));
+    acoustic_block<DIM, N1, T, UPD>(p, xcd_block(blockIdx.x, gridDim.x));
 }
 
 template <int DIM, int N1, typename T>
@@ -1878,9 +1906,16 @@ void tensor_rhs(hgdg_ctx* c, const StageIO& io) {
             constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
             using S = AcousticShape<DIMc, N1c>;
             const int64_t grid = (c->n_elem + S::EPB - 1) / S::EPB;
-            if (mixed)
+            const bool upd = io.upd.u != nullptr;  // the fused RK update of a step's last stage
+            if (mixed && upd)
+                hipLaunchKernelGGL((tensor_acoustic_rhs<DIMc, N1c, float, true>), dim3(unsigned(grid)), dim3(kBlock),
+                                   0, c->stream, a);
+            else if (mixed)
                 hipLaunchKernelGGL((tensor_acoustic_rhs<DIMc, N1c, float>), dim3(unsigned(grid)), dim3(kBlock), 0,
                                    c->stream, a);
+            else if (upd)
+                hipLaunchKernelGGL((tensor_acoustic_rhs<DIMc, N1c, double, true>), dim3(unsigned(grid)),
+                                   dim3(kBlock), 0, c->stream, a);
             else
                 hipLaunchKernelGGL((tensor_acoustic_rhs<DIMc, N1c, double>), dim3(unsigned(grid)), dim3(kBlock), 0,
                                    c->stream, a);

@@ -404,3 +404,46 @@ def test_right_hand_side_and_mass_solve_match_oracle(kind):
     dev.computeTimeDerivative([0, 1], [1.0, 0.25], 4)
     assert rel(dev.getTimeIntegrationVector(2), dev.getTimeIntegrationVector(4)) < 1e-12
     dev.close()
+
+
+FUSED_CASES = [
+    # (cells, p, periodic, precision, tableau, steps): 3D four-lane kernel at p = 1..4, 2D four-lane (p >= 2)
+    # and one-thread-per-pencil (p = 1) kernels; odd step counts end with one separate step
+    ([6, 6, 6], 3, True, "fp64", "RK4", 6),
+    ([5, 5, 5], 4, False, "mixed", "RK4", 7),
+    ([6, 5, 4], 1, False, "fp64", "RK3_TVD", 5),
+    ([6, 6, 6], 2, True, "mixed", "DORMAND_PRINCE", 4),  # last stage has b = 0
+    ([20, 16], 1, True, "fp64", "RK4", 4),
+    ([16, 12], 3, False, "mixed", "RK4", 2),
+]
+
+
+@pytest.mark.parametrize("n,p,periodic,precision,tab,steps", FUSED_CASES)
+def test_acoustic_fused_update_steps_bitwise_equal(n, p, periodic, precision, tab, steps):
+    """Pairs of RK steps with the update fused into each step's last stage
+    perform the separate update kernel's FP64 operations: the solution and
+    every stage vector come out bitwise the same."""
+    import hpgem_amd.tableaux
+    print(f"[case] acoustic fused update n={n} p={p} {precision} {tab} steps={steps}", flush=True)
+    tableau = getattr(hpgem_amd.tableaux, tab)
+    mesh = Mesh.structured(n, periodic=[int(periodic)] * len(n))
+    rng = np.random.default_rng(p + 7 * len(n))
+    cinv = rng.uniform(0.25, 4.0, size=mesh.n_elements)
+    u0 = None
+    out = {}
+    for fused in (False, True):
+        dev = HpgemAPIDevice(mesh, p, precision=precision, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=tableau,
+                             n_extra_vectors=2)
+        dev.setFusedTimeStepping(fused)
+        if u0 is None:
+            u0 = rng.standard_normal(dev.n_dof)
+        dev.setTimeIntegrationVector(0, u0)
+        dt = 0.02 / max(n)
+        dev.computeOneTimeStep(0.0, dt, steps)
+        dev.computeOneTimeStep(0.0, dt, 3)  # the same graphs again, odd count
+        out[fused] = [dev.getTimeIntegrationVector(v) for v in range(tableau.stages + 1)]
+        dev.close()
+    assert np.all(np.isfinite(out[False][0]))
+    assert not np.array_equal(out[False][0], u0)
+    for v in range(tableau.stages + 1):
+        np.testing.assert_array_equal(out[True][v], out[False][v])
