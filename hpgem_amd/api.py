@@ -227,11 +227,14 @@ class HpgemAPIDevice:
         check(self.lib.hgdg_get_advection_tiles(self.h, ctypes.byref(t), dptr(ms)))
         return t.value, ms
 
-    # ---- single-launch RK steps for meshes that fit in LDS -----------------
+    # ---- fused RK steps ----------------------------------------------------
     def setFusedTimeStepping(self, enable):
         """True (default): computeOneTimeStep may run all its steps in one
         single-workgroup launch when the state fits in LDS (small advection
-        meshes); False: always replay the captured step graph."""
+        meshes), and otherwise, on affine tensor meshes (advection and
+        acoustics), pairs of steps with the RK update in each last-stage
+        kernel; False: always replay the captured one-step graph. Results are
+        bitwise the same."""
         check(self.lib.hgdg_set_rk_fused(self.h, 1 if enable else 0))
 
     def getFusedTimeStepping(self):
