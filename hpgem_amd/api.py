@@ -232,7 +232,7 @@ class HpgemAPIDevice:
         """True (default): computeOneTimeStep may run all its steps in one
         single-workgroup launch when the state fits in LDS (small advection
         meshes), and otherwise, on affine tensor meshes (advection and
-        acoustics), pairs of steps with the RK update in each last-stage
+        acoustics) and tetrahedra, pairs of steps with the RK update in each last-stage
         kernel; False: always replay the captured one-step graph. Results are
         bitwise the same."""
         check(self.lib.hgdg_set_rk_fused(self.h, 1 if enable else 0))

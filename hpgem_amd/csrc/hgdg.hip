@@ -227,8 +227,9 @@ void launch_rhs(hgdg_ctx* c, const int* in_ids, const double* coeffs, int n_in, 
 
 // Whether the operator's RHS kernel performs a fused RK update (StageIO::upd).
 bool rhs_fuses_update(const hgdg_ctx* c) {
-    return !c->matrix() && !c->simplex() && !c->tm.general &&
-           (c->op.kind == HGDG_OP_ADVECTION_UPWIND || c->op.kind == HGDG_OP_ACOUSTIC_CENTRAL);
+    if (c->matrix()) return false;
+    if (c->simplex()) return true;  // (tetrahedra run the acoustic operator only)
+    return !c->tm.general && (c->op.kind == HGDG_OP_ADVECTION_UPWIND || c->op.kind == HGDG_OP_ACOUSTIC_CENTRAL);
 }
 
 void launch_axpy(hgdg_ctx* c, int to, const std::vector<int>& add, const std::vector<double>& scale) {

@@ -565,11 +565,25 @@ __device__ __forceinline__ void simplex_block(const SimplexParams& p, int64_t fi
     if (row_ok) {
         const double vals[4] = {double(kv.value()), double(ks[0].value()), double(ks[1].value()),
                                 double(ks[2].value())};
+        double kept[4];  // the values as stored
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
             const int64_t o = base + f * NB + i;
-            if (p.io.out_f32) static_cast<float*>(p.io.out)[o] = float(vals[f]);
-            else static_cast<double*>(p.io.out)[o] = vals[f];
+            if (p.io.out_f32) {
+                const float v = float(vals[f]);
+                static_cast<float*>(p.io.out)[o] = v;
+                kept[f] = double(v);
+            } else {
+                static_cast<double*>(p.io.out)[o] = vals[f];
+                kept[f] = vals[f];
+            }
+        }
+        if (p.io.upd.u != nullptr) {  // the RK update fused into a step's last stage (StageIO::upd)
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                const double own[1] = {kept[f]};
+                rk_update_run<1>(p.io.upd, base + f * NB + i, own);
+            }
         }
     }
 }

@@ -447,3 +447,34 @@ def test_acoustic_fused_update_steps_bitwise_equal(n, p, periodic, precision, ta
     assert not np.array_equal(out[False][0], u0)
     for v in range(tableau.stages + 1):
         np.testing.assert_array_equal(out[True][v], out[False][v])
+
+
+@pytest.mark.parametrize("precision,tab,steps", [("fp64", "RK4", 5), ("mixed", "RK4", 4),
+                                                 ("fp64", "DORMAND_PRINCE", 4)])
+def test_tet_fused_update_steps_bitwise_equal(precision, tab, steps):
+    """The tetrahedral kernel's fused RK update (hp orders 1..4, walls):
+    bitwise the same as the separate update kernel."""
+    import hpgem_amd.tableaux
+    print(f"[case] tet fused update {precision} {tab} steps={steps}", flush=True)
+    tableau = getattr(hpgem_amd.tableaux, tab)
+    mesh = Mesh.structured([4, 3, 3], periodic=[0, 0, 0], simplices=True)
+    rng = np.random.default_rng(11)
+    orders = rng.integers(1, 5, size=mesh.n_elements).astype(np.int32)
+    cinv = rng.uniform(0.25, 4.0, size=mesh.n_elements)
+    u0 = None
+    out = {}
+    for fused in (False, True):
+        dev = HpgemAPIDevice(mesh, orders, precision=precision, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv,
+                             tableau=tableau, n_extra_vectors=2)
+        dev.setFusedTimeStepping(fused)
+        if u0 is None:
+            u0 = rng.standard_normal(dev.n_dof)
+        dev.setTimeIntegrationVector(0, u0)
+        dev.computeOneTimeStep(0.0, 0.004, steps)
+        dev.computeOneTimeStep(0.0, 0.004, 3)
+        out[fused] = [dev.getTimeIntegrationVector(v) for v in range(tableau.stages + 1)]
+        dev.close()
+    assert np.all(np.isfinite(out[False][0]))
+    assert not np.array_equal(out[False][0], u0)
+    for v in range(tableau.stages + 1):
+        np.testing.assert_array_equal(out[True][v], out[False][v])
