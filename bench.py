@@ -41,7 +41,7 @@ SEED = 1
 # commands and per-kernel figures are in
 # profiles/rhs_kernels_32cube_p3_and_tet24.md. Counters cannot be read from
 # inside this process, so the figure is reported as measured there.
-MEASURED_TRAFFIC = {"double": 57.75e6, "float": 37.49e6}
+MEASURED_TRAFFIC = {"double": 58.60e6, "float": 37.46e6}
 # dt = CFL h / |a|. At CFL 0.1 the RK4 step of this p = 3 operator has
 # spectral radius ~1.47 (power iteration on the oracle, 4^3 periodic mesh), so
 # round-off grows ~1e8-fold over 50 steps; 0.05 lies inside the stability
