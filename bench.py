@@ -488,7 +488,7 @@ def main():
                     help="launch only the RHS and copy kernels (for rocprofv3 counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: the library has no multi-GPU path "
               "(the operator would shard by element partition with a face-trace halo exchange)",
               file=sys.stderr)
         sys.exit(2)
