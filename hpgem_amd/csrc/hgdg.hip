@@ -229,7 +229,7 @@ void launch_rhs(hgdg_ctx* c, const int* in_ids, const double* coeffs, int n_in, 
 bool rhs_fuses_update(const hgdg_ctx* c) {
     if (c->matrix()) return false;
     if (c->simplex()) return true;  // (tetrahedra run the acoustic operator only)
-    return !c->tm.general && (c->op.kind == HGDG_OP_ADVECTION_UPWIND || c->op.kind == HGDG_OP_ACOUSTIC_CENTRAL);
+    return c->op.kind == HGDG_OP_ADVECTION_UPWIND || c->op.kind == HGDG_OP_ACOUSTIC_CENTRAL;
 }
 
 void launch_axpy(hgdg_ctx* c, int to, const std::vector<int>& add, const std::vector<double>& scale) {

@@ -489,11 +489,25 @@ __global__ __launch_bounds__(kGenTarget) void tensor_general_rhs(const GenParams
     }
     if (ok) {
         const int64_t base = e * NF * NP;
+        double kept[NF];  // the values as stored
 #pragma unroll
         for (int fld = 0; fld < NF; ++fld) {
             const double v = Kp[fld * NP + t];
-            if (p.io.out_f32) static_cast<float*>(p.io.out)[base + fld * NP + t] = float(v);
-            else static_cast<double*>(p.io.out)[base + fld * NP + t] = v;
+            if (p.io.out_f32) {
+                const float vf = float(v);
+                static_cast<float*>(p.io.out)[base + fld * NP + t] = vf;
+                kept[fld] = double(vf);
+            } else {
+                static_cast<double*>(p.io.out)[base + fld * NP + t] = v;
+                kept[fld] = v;
+            }
+        }
+        if (p.io.upd.u != nullptr) {  // the RK update fused into a step's last stage (StageIO::upd)
+#pragma unroll
+            for (int fld = 0; fld < NF; ++fld) {
+                const double own[1] = {kept[fld]};
+                rk_update_run<1>(p.io.upd, base + fld * NP + t, own);
+            }
         }
     }
 }

@@ -246,3 +246,45 @@ def test_quarter_turned_cube_faces_run_like_the_reference():
     want = P.l2_error_modes(P.rk4(u, 0.1 / steps, steps), amp, k, 0.1)
     print(f"  L2 error {got:.12e} / oracle {want:.12e}")
     assert abs(got - want) <= 1e-9 * want and want > 0.1
+
+
+GEN_FUSED_CASES = [
+    # (cells, p, kind, variant, precision, tableau, steps) on the Gauss-point kernels
+    ([3, 3, 3], 2, "advection", "both", "fp64", "RK4", 6),
+    ([4, 3], 3, "acoustic", "rotated", "mixed", "RK4", 4),
+    ([3, 3, 3], 1, "acoustic", "perturbed", "fp64", "DORMAND_PRINCE", 4),
+]
+
+
+@pytest.mark.parametrize("n,p,kind,variant,precision,tab,steps", GEN_FUSED_CASES)
+def test_fused_update_steps_bitwise_equal_on_gauss_point_kernels(n, p, kind, variant, precision, tab, steps):
+    """Pairs of RK steps with the update in each last-stage kernel, on
+    rotated and non-affine meshes: bitwise the same as the separate update."""
+    import hpgem_amd.tableaux
+    print(f"[case] general fused update n={n} p={p} {kind} {variant} {precision} {tab}", flush=True)
+    tableau = getattr(hpgem_amd.tableaux, tab)
+    mesh = variant_mesh(n, variant)
+    rng = np.random.default_rng(3)
+    cinv = rng.uniform(0.25, 4.0, size=mesh.n_elements)
+    u0 = None
+    out = {}
+    for fused in (False, True):
+        if kind == "advection":
+            dev = HpgemAPIDevice(mesh, p, velocity=VEL[:len(n)], precision=precision, tableau=tableau,
+                                 n_extra_vectors=2)
+        else:
+            dev = HpgemAPIDevice(mesh, p, precision=precision, kind=OP_ACOUSTIC_CENTRAL, c_inv=cinv, tableau=tableau,
+                                 n_extra_vectors=2)
+        assert dev.usesTensorGeneral()
+        dev.setFusedTimeStepping(fused)
+        if u0 is None:
+            u0 = rng.standard_normal(dev.n_dof)
+        dev.setTimeIntegrationVector(0, u0)
+        dev.computeOneTimeStep(0.0, 0.004, steps)
+        dev.computeOneTimeStep(0.0, 0.004, 3)
+        out[fused] = [dev.getTimeIntegrationVector(v) for v in range(tableau.stages + 1)]
+        dev.close()
+    assert np.all(np.isfinite(out[False][0]))
+    assert not np.array_equal(out[False][0], u0)
+    for v in range(tableau.stages + 1):
+        np.testing.assert_array_equal(out[True][v], out[False][v])
