@@ -231,10 +231,9 @@ class HpgemAPIDevice:
     def setFusedTimeStepping(self, enable):
         """True (default): computeOneTimeStep may run all its steps in one
         single-workgroup launch when the state fits in LDS (small advection
-        meshes), and otherwise, on tensor meshes (advection and acoustics)
-        and tetrahedra, pairs of steps with the RK update in each last-stage
-        kernel; False: always replay the captured one-step graph. Results are
-        bitwise the same."""
+        meshes), and otherwise pairs of steps with the RK update in each
+        last-stage kernel (every operator); False: always replay the captured
+        one-step graph. Results are bitwise the same."""
         check(self.lib.hgdg_set_rk_fused(self.h, 1 if enable else 0))
 
     def getFusedTimeStepping(self):

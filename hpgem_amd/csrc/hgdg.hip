@@ -227,8 +227,7 @@ void launch_rhs(hgdg_ctx* c, const int* in_ids, const double* coeffs, int n_in, 
 
 // Whether the operator's RHS kernel performs a fused RK update (StageIO::upd).
 bool rhs_fuses_update(const hgdg_ctx* c) {
-    if (c->matrix()) return false;
-    if (c->simplex()) return true;  // (tetrahedra run the acoustic operator only)
+    if (c->matrix() || c->simplex()) return true;  // (tetrahedra run the acoustic operator only)
     return c->op.kind == HGDG_OP_ADVECTION_UPWIND || c->op.kind == HGDG_OP_ACOUSTIC_CENTRAL;
 }
 

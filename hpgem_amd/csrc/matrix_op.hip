@@ -92,8 +92,19 @@ __global__ __launch_bounds__(kMatThreads) void matrix_rhs(const MatParams p) {
     for (int s = 0; s < 4; ++s) {
         const int i = threadIdx.x + s * kMatThreads;
         if (i >= n) break;
-        if (p.io.out_f32) static_cast<float*>(p.io.out)[o + i] = float(acc[s]);
-        else static_cast<double*>(p.io.out)[o + i] = acc[s];
+        double kept;  // the value as stored
+        if (p.io.out_f32) {
+            const float v = float(acc[s]);
+            static_cast<float*>(p.io.out)[o + i] = v;
+            kept = double(v);
+        } else {
+            static_cast<double*>(p.io.out)[o + i] = acc[s];
+            kept = acc[s];
+        }
+        if (p.io.upd.u != nullptr) {  // the RK update fused into a step's last stage (StageIO::upd)
+            const double own[1] = {kept};
+            rk_update_run<1>(p.io.upd, o + i, own);
+        }
     }
 }
 

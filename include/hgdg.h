@@ -131,10 +131,10 @@ int hgdg_get_advection_tiles(hgdg_ctx* ctx, int* tiles, double* ms);
 /* Whether hgdg_rk_steps may fuse (1, the default) or always replays the
  * captured one-step graph (0). Fusing means: all steps in one
  * single-workgroup launch when the whole state fits in LDS (tensor advection
- * meshes of a few thousand degrees of freedom); otherwise, on tensor meshes
- * (advection and acoustics) and tetrahedra, pairs of steps whose RK update runs in
- * each step's last-stage kernel. Results are bitwise the same. No
- * counterpart in the reference (a launch choice of this implementation). */
+ * meshes of a few thousand degrees of freedom); otherwise pairs of steps
+ * whose RK update runs in each step's last-stage kernel (every operator).
+ * Results are bitwise the same. No counterpart in the reference (a launch
+ * choice of this implementation). */
 int hgdg_set_rk_fused(hgdg_ctx* ctx, int enable);
 
 /* The setting above and how many single-workgroup RK launches the context

@@ -224,3 +224,28 @@ def test_matrix_operator_refuses_projection_and_mixed_storage_keeps_1e6():
     got = dev.getTimeIntegrationVector(1)
     assert np.linalg.norm(got - k_ref) <= 1e-6 * np.linalg.norm(k_ref)
     dev.close()
+
+
+@pytest.mark.parametrize("precision,tab,steps", [("fp64", "RK4", 6), ("mixed", "RK4", 5),
+                                                 ("fp64", "DORMAND_PRINCE", 4)])
+def test_matrix_operator_fused_update_steps_bitwise_equal(precision, tab, steps):
+    """Pairs of RK steps with the update in each last-stage kernel: bitwise
+    the same as the separate update kernel, for an arbitrary operator."""
+    print(f"[case] matrix operator fused update {precision} {tab} steps={steps}", flush=True)
+    tableau = getattr(hpgem_amd.tableaux, tab)
+    sizes, mass, stiff, faces, blocks = random_operator(11, n_elem=24)
+    u0 = np.random.default_rng(12).standard_normal(int(np.sum(sizes)))
+    out = {}
+    for fused in (False, True):
+        dev = HpgemAPIMatrixDevice(mass, stiff, faces, blocks, precision=precision, tableau=tableau,
+                                   n_extra_vectors=2)
+        dev.setFusedTimeStepping(fused)
+        dev.setTimeIntegrationVector(0, u0)
+        dev.computeOneTimeStep(0.0, 0.002, steps)
+        dev.computeOneTimeStep(0.0, 0.002, 3)
+        out[fused] = [dev.getTimeIntegrationVector(v) for v in range(tableau.stages + 1)]
+        dev.close()
+    assert np.all(np.isfinite(out[False][0]))
+    assert not np.array_equal(out[False][0], u0)
+    for v in range(tableau.stages + 1):
+        np.testing.assert_array_equal(out[True][v], out[False][v])
