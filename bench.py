@@ -399,7 +399,8 @@ def run_serving(precision, n_requests, seed=SEED, profile=False):
         total = sum(ms for ms, _ in fams.values()) + upd
         split = {f"{k}{d}d_p{o}": {"kernel_ms": ms, "launches": n, "share": ms / total}
                  for (k, d, o), (ms, n) in sorted(fams.items(), key=lambda kv: -kv[1][0])}
-        split["rk_update"] = {"kernel_ms": upd, "share": upd / total}
+        if upd > 0:
+            split["rk_update"] = {"kernel_ms": upd, "share": upd / total}
         return {"requests": n_requests, "completed": st["completed"], "kernel_ms_total": total,
                 "wall_s": wall, "families": split}
     return {"requests": n_requests, "completed": st["completed"], "failed": st["failed"],

@@ -196,7 +196,7 @@ void tensor_general_state_diff(hgdg_ctx* c, const Vec& x, const Vec& y, double* 
 size_t tensor_batch_param_bytes(const hgdg_ctx* c);
 int64_t tensor_batch_blocks(const hgdg_ctx* c);
 void tensor_batch_param(hgdg_ctx* c, const StageIO& io, void* out);
-void tensor_batch_launch(hipStream_t stream, int kind, int dim, int n1, bool mixed, const void* params,
+void tensor_batch_launch(hipStream_t stream, int kind, int dim, int n1, bool mixed, bool upd, const void* params,
                          const int64_t* off, int n, int64_t blocks);
 
 // hgdg.hip
@@ -206,15 +206,6 @@ void operator_set(hgdg_ctx* c, const hgdg_op_desc* op);
 void upload_vector(hgdg_ctx* c, Vec& v, const std::vector<double>& dev);
 void download_vector(hgdg_ctx* c, Vec& v, std::vector<double>& dev);
 void state_diff(hgdg_ctx* c, int x, int y, double* l2, double* max_abs);
-struct AxpyItem {
-    int64_t n;
-    double* y;
-    const void* x[kMaxIn];
-    double c[kMaxIn];
-    uint32_t x_f32;
-    int32_t n_x;
-};
-void axpy_batch_launch(hipStream_t stream, const AxpyItem* items_dev, const int64_t* off, int n, int64_t blocks);
 
 // matrix_op.hip
 void matrix_operator_set(hgdg_ctx* c, const hgdg_matrix_desc* d, int precision);

@@ -63,28 +63,6 @@ __device__ __forceinline__ double axpy_value(const A& p, int64_t i) {
     return y;
 }
 
-// Batched RK updates of many problems: block b belongs to item s with
-// off[s] <= b < off[s+1]; each item is a multi_axpy.
-__global__ __launch_bounds__(kBlock) void multi_axpy_batched(const AxpyItem* items, const int64_t* off, int n) {
-    int lo = 0, hi = n - 1;
-    const int64_t b = blockIdx.x;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) / 2;
-        if (off[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    const AxpyItem& p = items[lo];
-    const int64_t i = (b - off[lo]) * kBlock + threadIdx.x;
-    if (i >= p.n) return;
-    p.y[i] = axpy_value(p, i);
-}
-
-void axpy_batch_launch(hipStream_t stream, const AxpyItem* items_dev, const int64_t* off, int n, int64_t blocks) {
-    if (n == 0 || blocks == 0) return;
-    hipLaunchKernelGGL(multi_axpy_batched, dim3(unsigned(blocks)), dim3(kBlock), 0, stream, items_dev, off, n);
-    HGDG_HIP(hipGetLastError());
-}
-
 __global__ __launch_bounds__(kBlock) void multi_axpy(const AxpyParams p) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= p.n) return;

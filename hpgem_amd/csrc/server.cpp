@@ -8,8 +8,9 @@
 // free), builds them (mesh, faces, tables, projected initial state) and hands
 // them to the worker. The worker advances every resident solve by one RK4
 // step per tick: each stage is one batched launch per (operator, dim, order)
-// family that covers all resident solves of that family, and one batched
-// launch does all RK updates. Solves join at the next tick after they are
+// family that covers all resident solves of that family, and the last
+// stage's launches also apply each solve's RK update (the solution
+// ping-pongs between two vectors, as in the single-problem fused steps). Solves join at the next tick after they are
 // built and leave when their step count is reached, freeing memory for the
 // queue. Every element of every solve runs the exact code of the
 // single-problem kernel on its own data, so a result does not depend on what
@@ -55,6 +56,7 @@ struct Solve {
     Request req;
     hgdg_ctx* ctx = nullptr;
     int steps_done = 0;
+    int cur = 0;  // where the solution is: 0 vecs[0], 1 ctx->u_spare
     int64_t bytes = 0;
     double t_start = 0;
 };
@@ -76,7 +78,8 @@ int64_t footprint(const hgdg_solve_desc& d, bool mixed) {
     for (int i = 0; i < d.dim; ++i) nb *= d.order + 1;
     const int nf = d.kind == HGDG_OP_ACOUSTIC_CENTRAL ? d.dim + 1 : 1;
     const int64_t dof = cells * nb * nf;
-    return dof * (8 + 4 * (mixed ? 4 : 8)) + cells * (8 * (d.dim * d.dim + 2) + 4 * 2 * d.dim + 8);
+    // solution and its ping-pong copy in FP64, four stage vectors
+    return dof * (2 * 8 + 4 * (mixed ? 4 : 8)) + cells * (8 * (d.dim * d.dim + 2) + 4 * 2 * d.dim + 8);
 }
 
 // A solve's context runs on a stream of the server's (the preparer's while it
@@ -133,28 +136,35 @@ struct hgdg_server {
         size_t params, off;
         int n;
         int64_t blocks;
+        int phase;  // the step graph it belongs to
+        bool upd;   // last stage: with the RK update
     };
     std::vector<int64_t> staged;  // tickets of the solves the device blob describes (never reused, unlike addresses)
-    hipGraph_t step_graph = nullptr;  // one RK4 step of the staged solves, captured once per resident set
-    hipGraphExec_t step_exec = nullptr;
+    // One RK4 step of the staged solves per parity, captured once per
+    // resident set: graph k takes each solve's solution from where it was at
+    // staging (Solve::cur) when k = 0, from the other vector when k = 1, and
+    // writes the new solution to the other one; the ticks alternate them.
+    hipGraph_t step_graph[2] = {nullptr, nullptr};
+    hipGraphExec_t step_exec[2] = {nullptr, nullptr};
+    int phase = 0;  // the graph of the next step
     void drop_step_graph() {
-        if (step_exec) (void)hipGraphExecDestroy(step_exec);
-        if (step_graph) (void)hipGraphDestroy(step_graph);
-        step_exec = nullptr;
-        step_graph = nullptr;
+        for (int k = 0; k < 2; ++k) {
+            if (step_exec[k]) (void)hipGraphExecDestroy(step_exec[k]);
+            if (step_graph[k]) (void)hipGraphDestroy(step_graph[k]);
+            step_exec[k] = nullptr;
+            step_graph[k] = nullptr;
+        }
     }
     std::vector<Launch> launches;
-    size_t ax_items = 0, ax_off_pos = 0;
-    int64_t ax_blocks = 0;
     // Per-family kernel times: with `profile` set, each tick issues the staged
     // launches one by one between HIP events instead of replaying the step
     // graph (the same launches, so the same results) and accumulates the
-    // events' elapsed times per (kind, dim, n1) family and for the updates.
+    // events' elapsed times per (kind, dim, n1) family; the RK updates run
+    // inside each family's last-stage launch, so they are part of its time.
     std::atomic<bool> profile{false};
-    std::vector<hipEvent_t> events;  // 2 per staged launch + 2 for the update pass
+    std::vector<hipEvent_t> events;  // 2 per staged launch
     std::map<std::array<int, 3>, std::pair<double, int64_t>> family_ms;  // guarded by mu
-    double update_ms = 0;
-    int64_t update_launches = 0;
+    double update_ms = 0;  // separate update passes: none (reported as 0)
     // worker accounting (guarded by mu): ticks, re-stagings and their wall time
     int64_t n_ticks = 0, n_restages = 0;
     double restage_s = 0, tick_s = 0;
@@ -196,6 +206,7 @@ hgdg_ctx* build_solve(int device, bool mixed, const Request& r, hipStream_t stre
     c->vecs.resize(5);
     set_storage(c.get(), c->vecs[0], false);
     for (int i = 1; i < 5; ++i) set_storage(c.get(), c->vecs[i], mixed);
+    c->u_spare.alloc(size_t(c->n_dof()));  // the solution's ping-pong copy
     HGDG_HIP(hipMemsetAsync(c->vecs[0].f64->ptr, 0, size_t(c->n_dof()) * sizeof(double), c->stream));
     tensor_project_device(c.get(), c->vecs[0], d.kind == HGDG_OP_ACOUSTIC_CENTRAL ? 0 : -1, d.n_modes, r.amp.data(),
                           r.k.data());
@@ -259,66 +270,60 @@ void hgdg_server::stage(std::vector<hgdg::Solve*>& solves) {
     // group by kernel family
     std::map<std::array<int, 3>, std::vector<Solve*>> groups;
     for (Solve* s : solves) groups[{s->ctx->op.kind, s->ctx->dim, s->ctx->tm.n1}].push_back(s);
-    // lay out the parameter blob: per stage and family [params][offsets], then the updates
+    // lay out the parameter blob: per graph, stage and family [params][offsets]
     launches.clear();
     std::vector<char> blob;
     auto align = [&](size_t a) { blob.resize((blob.size() + a - 1) / a * a); };
-    for (int stage = 0; stage < 4; ++stage) {
-        for (auto& kv : groups) {
-            const auto& g = kv.second;
-            const size_t pb = tensor_batch_param_bytes(g[0]->ctx);
-            Launch L{kv.first[0], kv.first[1], kv.first[2], 0, 0, int(g.size()), 0};
-            align(64);
-            L.params = blob.size();
-            blob.resize(blob.size() + pb * g.size());
-            std::vector<int64_t> off(g.size() + 1, 0);
-            for (size_t i = 0; i < g.size(); ++i) {
-                hgdg_ctx* c = g[i]->ctx;
-                StageIO io{};
-                io.n_in = stage == 0 ? 1 : 2;
-                io.in[0] = c->vecs[0].ptr();
-                io.coef[0] = 1.0;
-                if (stage > 0) {
-                    io.in[1] = c->vecs[stage].ptr();
-                    io.coef[1] = g[i]->req.d.dt * kA[stage];
-                    if (c->vecs[stage].is_f32) io.in_f32 |= 2u;
+    for (int k = 0; k < 2; ++k) {
+        for (int stage = 0; stage < 4; ++stage) {
+            for (auto& kv : groups) {
+                const auto& g = kv.second;
+                const size_t pb = tensor_batch_param_bytes(g[0]->ctx);
+                Launch L{kv.first[0], kv.first[1], kv.first[2], 0, 0, int(g.size()), 0, k, stage == 3};
+                align(64);
+                L.params = blob.size();
+                blob.resize(blob.size() + pb * g.size());
+                std::vector<int64_t> off(g.size() + 1, 0);
+                for (size_t i = 0; i < g.size(); ++i) {
+                    hgdg_ctx* c = g[i]->ctx;
+                    const bool spare = (g[i]->cur ^ k) != 0;  // the step's solution is in u_spare
+                    double* u_cur = spare ? c->u_spare.ptr : c->vecs[0].f64->ptr;
+                    double* u_next = spare ? c->vecs[0].f64->ptr : c->u_spare.ptr;
+                    StageIO io{};
+                    io.n_in = stage == 0 ? 1 : 2;
+                    io.in[0] = u_cur;
+                    io.coef[0] = 1.0;
+                    if (stage > 0) {
+                        io.in[1] = c->vecs[stage].ptr();
+                        io.coef[1] = g[i]->req.d.dt * kA[stage];
+                        if (c->vecs[stage].is_f32) io.in_f32 |= 2u;
+                    }
+                    io.out = c->vecs[1 + stage].ptr();
+                    io.out_f32 = c->vecs[1 + stage].is_f32;
+                    if (stage == 3) {  // u_next = u_cur + dt sum_j b_j k_j, k_4 from the kernel's registers
+                        RkUpdate& q = io.upd;
+                        q.u = u_cur;
+                        q.y_new = u_next;
+                        q.n_x = 4;
+                        q.self = 3;
+                        for (int j = 0; j < 4; ++j) {
+                            q.x[j] = c->vecs[1 + j].ptr();
+                            q.c[j] = g[i]->req.d.dt * kB[j];
+                            if (c->vecs[1 + j].is_f32) q.x_f32 |= 1u << j;
+                        }
+                    }
+                    tensor_batch_param(c, io, blob.data() + L.params + pb * i);
+                    off[i + 1] = off[i] + tensor_batch_blocks(c);
                 }
-                io.out = c->vecs[1 + stage].ptr();
-                io.out_f32 = c->vecs[1 + stage].is_f32;
-                tensor_batch_param(c, io, blob.data() + L.params + pb * i);
-                off[i + 1] = off[i] + tensor_batch_blocks(c);
+                L.blocks = off.back();
+                align(64);
+                L.off = blob.size();
+                blob.resize(blob.size() + sizeof(int64_t) * off.size());
+                std::memcpy(blob.data() + L.off, off.data(), sizeof(int64_t) * off.size());
+                launches.push_back(L);
             }
-            L.blocks = off.back();
-            align(64);
-            L.off = blob.size();
-            blob.resize(blob.size() + sizeof(int64_t) * off.size());
-            std::memcpy(blob.data() + L.off, off.data(), sizeof(int64_t) * off.size());
-            launches.push_back(L);
         }
     }
-    align(64);
-    ax_items = blob.size();
-    blob.resize(blob.size() + sizeof(AxpyItem) * solves.size());
-    std::vector<int64_t> ax_off(solves.size() + 1, 0);
-    for (size_t i = 0; i < solves.size(); ++i) {
-        hgdg_ctx* c = solves[i]->ctx;
-        AxpyItem it{};
-        it.n = c->n_dof();
-        it.y = c->vecs[0].f64->ptr;
-        it.n_x = 4;
-        for (int j = 0; j < 4; ++j) {
-            it.x[j] = c->vecs[1 + j].ptr();
-            it.c[j] = solves[i]->req.d.dt * kB[j];
-            if (c->vecs[1 + j].is_f32) it.x_f32 |= 1u << j;
-        }
-        std::memcpy(blob.data() + ax_items + sizeof(AxpyItem) * i, &it, sizeof(it));
-        ax_off[i + 1] = ax_off[i] + (it.n + kBlock - 1) / kBlock;
-    }
-    align(64);
-    ax_off_pos = blob.size();
-    ax_blocks = ax_off.back();
-    blob.resize(blob.size() + sizeof(int64_t) * ax_off.size());
-    std::memcpy(blob.data() + ax_off_pos, ax_off.data(), sizeof(int64_t) * ax_off.size());
 
     staged.clear();  // nothing is staged while the blob is replaced
     if (d_blob.n < blob.size()) d_blob.alloc(blob.size() * 2);
@@ -329,35 +334,37 @@ void hgdg_server::stage(std::vector<hgdg::Solve*>& solves) {
     }
     std::memcpy(h_blob, blob.data(), blob.size());
     HGDG_HIP(hipMemcpyAsync(d_blob.ptr, h_blob, blob.size(), hipMemcpyHostToDevice, stream));
-    // one step's launches as a graph: a tick is then one graph launch
+    // each parity's step as a graph: a tick is then one graph launch
     drop_step_graph();
-    HGDG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    try {
-        for (const Launch& L : launches)
-            tensor_batch_launch(stream, L.kind, L.dim, L.n1, mixed, d_blob.ptr + L.params,
-                                reinterpret_cast<const int64_t*>(d_blob.ptr + L.off), L.n, L.blocks);
-        axpy_batch_launch(stream, reinterpret_cast<const AxpyItem*>(d_blob.ptr + ax_items),
-                          reinterpret_cast<const int64_t*>(d_blob.ptr + ax_off_pos), int(solves.size()), ax_blocks);
-    } catch (...) {
-        hipGraph_t g = nullptr;
-        (void)hipStreamEndCapture(stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
+    for (int k = 0; k < 2; ++k) {
+        HGDG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            for (const Launch& L : launches)
+                if (L.phase == k)
+                    tensor_batch_launch(stream, L.kind, L.dim, L.n1, mixed, L.upd, d_blob.ptr + L.params,
+                                        reinterpret_cast<const int64_t*>(d_blob.ptr + L.off), L.n, L.blocks);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HGDG_HIP(hipStreamEndCapture(stream, &step_graph[k]));
+        HGDG_HIP(hipGraphInstantiate(&step_exec[k], step_graph[k], nullptr, nullptr, 0));
     }
-    HGDG_HIP(hipStreamEndCapture(stream, &step_graph));
-    HGDG_HIP(hipGraphInstantiate(&step_exec, step_graph, nullptr, nullptr, 0));
+    phase = 0;
     for (Solve* sv : solves) staged.push_back(sv->req.ticket);
 }
 
-// n_steps RK4 steps of every resident solve (the captured step replayed),
-// then one synchronisation.
+// n_steps RK4 steps of every resident solve (the captured steps replayed,
+// alternating parity), then one synchronisation.
 void hgdg_server::tick(std::vector<hgdg::Solve*>& solves, int n_steps) {
     using namespace hgdg;
     bool same = solves.size() == staged.size();
     for (size_t i = 0; same && i < solves.size(); ++i) same = solves[i]->req.ticket == staged[i];
     const bool timed = profile.load();
     const double t0 = now_s();
-    if (!same || (timed && events.size() < 2 * launches.size() + 2)) {
+    if (!same || (timed && events.size() < 2 * launches.size())) {
         std::lock_guard<std::mutex> hk(hip_mu);
         if (!same) {
             stage(solves);
@@ -366,14 +373,21 @@ void hgdg_server::tick(std::vector<hgdg::Solve*>& solves, int n_steps) {
             ++n_restages;
             restage_s += dt;
         }
-        while (timed && events.size() < 2 * launches.size() + 2) {
+        while (timed && events.size() < 2 * launches.size()) {
             hipEvent_t e;
             HGDG_HIP(hipEventCreate(&e));
             events.push_back(e);
         }
     }
+    auto advance = [&] {  // the step just enqueued moved every solution to its other vector
+        phase ^= 1;
+        for (Solve* s : solves) s->cur ^= 1;
+    };
     if (!timed) {
-        for (int step = 0; step < n_steps; ++step) HGDG_HIP(hipGraphLaunch(step_exec, stream));
+        for (int step = 0; step < n_steps; ++step) {
+            HGDG_HIP(hipGraphLaunch(step_exec[phase], stream));
+            advance();
+        }
         HGDG_HIP(hipStreamSynchronize(stream));
         std::lock_guard<std::mutex> lk(mu);
         n_ticks += n_steps;
@@ -383,30 +397,26 @@ void hgdg_server::tick(std::vector<hgdg::Solve*>& solves, int n_steps) {
     for (int step = 0; step < n_steps; ++step) {
         for (size_t i = 0; i < launches.size(); ++i) {
             const Launch& L = launches[i];
+            if (L.phase != phase) continue;
             HGDG_HIP(hipEventRecord(events[2 * i], stream));
-            tensor_batch_launch(stream, L.kind, L.dim, L.n1, mixed, d_blob.ptr + L.params,
+            tensor_batch_launch(stream, L.kind, L.dim, L.n1, mixed, L.upd, d_blob.ptr + L.params,
                                 reinterpret_cast<const int64_t*>(d_blob.ptr + L.off), L.n, L.blocks);
             HGDG_HIP(hipEventRecord(events[2 * i + 1], stream));
         }
-        const size_t u = 2 * launches.size();
-        HGDG_HIP(hipEventRecord(events[u], stream));
-        axpy_batch_launch(stream, reinterpret_cast<const AxpyItem*>(d_blob.ptr + ax_items),
-                          reinterpret_cast<const int64_t*>(d_blob.ptr + ax_off_pos), int(solves.size()), ax_blocks);
-        HGDG_HIP(hipEventRecord(events[u + 1], stream));
         HGDG_HIP(hipStreamSynchronize(stream));
-        std::lock_guard<std::mutex> lk(mu);
-        for (size_t i = 0; i < launches.size(); ++i) {
-            float ms = 0;
-            HGDG_HIP(hipEventElapsedTime(&ms, events[2 * i], events[2 * i + 1]));
-            auto& f = family_ms[{launches[i].kind, launches[i].dim, launches[i].n1}];
-            f.first += ms;
-            ++f.second;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (size_t i = 0; i < launches.size(); ++i) {
+                if (launches[i].phase != phase) continue;
+                float ms = 0;
+                HGDG_HIP(hipEventElapsedTime(&ms, events[2 * i], events[2 * i + 1]));
+                auto& f = family_ms[{launches[i].kind, launches[i].dim, launches[i].n1}];
+                f.first += ms;
+                ++f.second;
+            }
+            ++n_ticks;
         }
-        float ms = 0;
-        HGDG_HIP(hipEventElapsedTime(&ms, events[u], events[u + 1]));
-        update_ms += ms;
-        ++update_launches;
-        ++n_ticks;
+        advance();
     }
     std::lock_guard<std::mutex> lk(mu);
     tick_s += now_s() - t0;
@@ -473,6 +483,9 @@ void hgdg_server::worker_loop() {
                     hgdg_ctx* c = s->ctx;
                     n_dof = c->n_dof();
                     std::vector<double> dev;
+                    if (s->cur)  // the last step left the solution in the ping-pong copy
+                        HGDG_HIP(hipMemcpyAsync(c->vecs[0].f64->ptr, c->u_spare.ptr, size_t(n_dof) * sizeof(double),
+                                                hipMemcpyDeviceToDevice, c->stream));
                     hgdg::download_vector(c, c->vecs[0], dev);
                     for (double v : dev) norm += v * v;
                     norm = std::sqrt(norm);

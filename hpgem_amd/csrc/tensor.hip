@@ -573,13 +573,14 @@ __device__ __forceinline__ int find_problem(const int64_t* off, int n, int64_t b
 }
 
 // Many independent problems of one kind in one launch: every block runs the
-// exact code of the single-problem kernel on its own problem's data.
-template <int DIM, int N1, typename T>
+// exact code of the single-problem kernel on its own problem's data (UPD:
+// with each problem's fused RK update, StageIO::upd).
+template <int DIM, int N1, typename T, bool UPD = false>
 __global__ __launch_bounds__(kBlock) void tensor_advection_rhs_batched(const RhsParams* ps, const int64_t* off,
                                                                        int n) {
     const int64_t b = xcd_block(blockIdx.x, gridDim.x);
     const int s = find_problem(off, n, b);
-    advection_block<DIM, N1, T>(ps[s], b - off[s]);
+    advection_block<DIM, N1, T, UPD>(ps[s], b - off[s]);
 }
 
 // Acoustic wave system on tensor elements (AcousticWaveLinear_Impl.h:102-283):
@@ -1070,12 +1071,12 @@ __global__ __launch_bounds__(kBlock) void tensor_acoustic_rhs(const AcousticPara
     acoustic_block<DIM, N1, T, UPD>(p, xcd_block(blockIdx.x, gridDim.x));
 }
 
-template <int DIM, int N1, typename T>
+template <int DIM, int N1, typename T, bool UPD = false>
 __global__ __launch_bounds__(kBlock) void tensor_acoustic_rhs_batched(const AcousticParams* ps, const int64_t* off,
                                                                       int n) {
     const int64_t b = xcd_block(blockIdx.x, gridDim.x);
     const int s = find_problem(off, n, b);
-    acoustic_block<DIM, N1, T>(ps[s], b - off[s]);
+    acoustic_block<DIM, N1, T, UPD>(ps[s], b - off[s]);
 }
 
 // L2 projection of f(x) = sum_m amp_m sin(2 pi k_m . x) on affine tensor
@@ -2121,21 +2122,26 @@ void tensor_batch_param(hgdg_ctx* c, const StageIO& io, void* out) {
     }
 }
 
-void tensor_batch_launch(hipStream_t stream, int kind, int dim, int n1, bool mixed, const void* params,
+void tensor_batch_launch(hipStream_t stream, int kind, int dim, int n1, bool mixed, bool upd, const void* params,
                          const int64_t* off, int n, int64_t blocks) {
     if (n == 0 || blocks == 0) return;
     dispatch_tensor(dim, n1, [&](auto D, auto N) {
         constexpr int DIMc = decltype(D)::value, N1c = decltype(N)::value;
         const dim3 grid{unsigned(blocks), 1, 1}, block{unsigned(kBlock), 1, 1};
-        if (kind == HGDG_OP_ACOUSTIC_CENTRAL) {
-            const auto* ps = static_cast<const AcousticParams*>(params);
-            if (mixed) hipLaunchKernelGGL((tensor_acoustic_rhs_batched<DIMc, N1c, float>), grid, block, 0, stream, ps, off, n);
-            else hipLaunchKernelGGL((tensor_acoustic_rhs_batched<DIMc, N1c, double>), grid, block, 0, stream, ps, off, n);
-        } else {
-            const auto* ps = static_cast<const RhsParams*>(params);
-            if (mixed) hipLaunchKernelGGL((tensor_advection_rhs_batched<DIMc, N1c, float>), grid, block, 0, stream, ps, off, n);
-            else hipLaunchKernelGGL((tensor_advection_rhs_batched<DIMc, N1c, double>), grid, block, 0, stream, ps, off, n);
-        }
+        auto go = [&](auto T_, auto U_) {
+            using Tc = decltype(T_);
+            constexpr bool Uc = decltype(U_)::value;
+            if (kind == HGDG_OP_ACOUSTIC_CENTRAL)
+                hipLaunchKernelGGL((tensor_acoustic_rhs_batched<DIMc, N1c, Tc, Uc>), grid, block, 0, stream,
+                                   static_cast<const AcousticParams*>(params), off, n);
+            else
+                hipLaunchKernelGGL((tensor_advection_rhs_batched<DIMc, N1c, Tc, Uc>), grid, block, 0, stream,
+                                   static_cast<const RhsParams*>(params), off, n);
+        };
+        if (mixed && upd) go(float{}, std::true_type{});
+        else if (mixed) go(float{}, std::false_type{});
+        else if (upd) go(double{}, std::true_type{});
+        else go(double{}, std::false_type{});
     });
     HGDG_HIP(hipGetLastError());
 }

@@ -61,7 +61,8 @@ class Server:
         check(self.lib.hgdg_server_set_profiling(self.h, 1 if enable else 0))
 
     def family_times(self):
-        """{(kind, dim, order): (kernel ms, launches)} and the update passes' ms."""
+        """{(kind, dim, order): (kernel ms, launches)} and the separate update
+        passes' ms (0: the RK updates run inside each family's last stage)."""
         n = ctypes.c_int()
         check(self.lib.hgdg_server_family_times(self.h, 0, None, None, None, None, None, ctypes.byref(n), None))
         cap = n.value

@@ -387,7 +387,8 @@ int hgdg_server_get_stats(hgdg_server* s, hgdg_server_stats* out);
 int hgdg_server_set_profiling(hgdg_server* s, int enable);
 /* Up to `capacity` families: kind, dim, order, summed kernel ms and launch
  * count each (any array may be NULL); n_families = how many there are;
- * update_ms = summed time of the update passes (may be NULL). */
+ * update_ms = summed time of separate RK update passes (may be NULL): 0, the
+ * updates run inside each family's last-stage launch and count in its time. */
 int hgdg_server_family_times(hgdg_server* s, int capacity, int32_t* kind, int32_t* dim, int32_t* order,
                              double* kernel_ms, int64_t* launches, int* n_families, double* update_ms);
 int hgdg_server_destroy(hgdg_server* s);
